@@ -1,0 +1,196 @@
+// ref_driver.cpp — a thin extern "C" driver over the reference C++ prover.
+//
+// TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles this file against the
+// reference's own headers and vendored C sources where they lie (the
+// reference repository's cpp/include and cpp/third_party), producing
+// oracle/_ref/libwhirref.so.  It is used to cross-check the C restatement in
+// whir_oracle.c and as the "reference" CPU baseline in bench.py.  Nothing in
+// the product links it.  Each function calls one reference entry point:
+//
+//   ref_rs_encode          algebra/ntt/mod_ntt.hpp:133-141 interleaved_rs_encode
+//   ref_ntt_batch          algebra/ntt/cooley_tukey.hpp:200-259 NttEngine::ntt_batch
+//   ref_commit_leaves_*    protocols/matrix_commit.hpp:169-195 commit_leaves
+//   ref_merkle_build       protocols/merkle_tree.hpp:104-147 build_tree
+//   ref_merkle_open        protocols/merkle_tree.hpp:165-211 open_path
+//   ref_sumcheck_*         algebra/sumcheck.hpp:44-116
+//   ref_horner_base_ext3   algebra/utilities.hpp:195-246 mixed_univariate_evaluate
+//   ref_commit_base        the irs_commit.hpp:344-460 sequence (encode, leaves, tree)
+//                          without the transcript
+#include "whir/algebra/embedding.hpp"
+#include "whir/algebra/goldilocks.hpp"
+#include "whir/algebra/goldilocks_ext3.hpp"
+#include "whir/algebra/ntt/cooley_tukey_goldilocks.hpp"
+#include "whir/algebra/ntt/mod_ntt.hpp"
+#include "whir/algebra/sumcheck.hpp"
+#include "whir/algebra/utilities.hpp"
+#include "whir/hash/blake3_engine.hpp"
+#include "whir/protocols/matrix_commit.hpp"
+#include "whir/protocols/merkle_tree.hpp"
+
+#include <cstdint>
+#include <cstring>
+#include <span>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+using whir::algebra::Goldilocks;
+using whir::algebra::GoldilocksExt3;
+using Hash = whir::hash::Hash;
+
+static_assert(sizeof(Goldilocks) == 8, "Goldilocks must be one u64");
+static_assert(sizeof(GoldilocksExt3) == 24, "Ext3 must be three u64");
+
+std::vector<Goldilocks> to_gl(const uint64_t* p, size_t n) {
+    std::vector<Goldilocks> v(n);
+    for (size_t i = 0; i < n; ++i) v[i] = Goldilocks::from_u64(p[i]);
+    return v;
+}
+
+std::vector<GoldilocksExt3> to_e3(const uint64_t* p, size_t n) {
+    std::vector<GoldilocksExt3> v(n);
+    for (size_t i = 0; i < n; ++i)
+        v[i] = GoldilocksExt3(Goldilocks::from_u64(p[3 * i]), Goldilocks::from_u64(p[3 * i + 1]),
+                              Goldilocks::from_u64(p[3 * i + 2]));
+    return v;
+}
+
+void from_e3(const GoldilocksExt3& x, uint64_t* o) {
+    o[0] = x.c0().as_canonical_u64();
+    o[1] = x.c1().as_canonical_u64();
+    o[2] = x.c2().as_canonical_u64();
+}
+
+const whir::hash::Blake3& blake3() {
+    static const whir::hash::Blake3 b;
+    return b;
+}
+}  // namespace
+
+extern "C" {
+
+int ref_num_threads() {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void ref_ntt_batch(uint64_t* values, size_t total, size_t n) {
+    auto& eng = whir::algebra::ntt::goldilocks_engine();
+    std::span<Goldilocks> s{reinterpret_cast<Goldilocks*>(values), total};
+    eng.ntt_batch(s, n);
+}
+
+void ref_rs_encode(const uint64_t* coeffs, size_t num_polys, size_t poly_size, size_t cw,
+                   size_t depth, uint64_t* out) {
+    std::vector<std::span<const Goldilocks>> polys;
+    for (size_t k = 0; k < num_polys; ++k)
+        polys.emplace_back(reinterpret_cast<const Goldilocks*>(coeffs) + k * poly_size, poly_size);
+    auto m = whir::algebra::ntt::interleaved_rs_encode<Goldilocks>(
+        whir::algebra::ntt::goldilocks_engine(),
+        std::span<const std::span<const Goldilocks>>{polys}, cw, depth);
+    std::memcpy(out, m.data(), m.size() * sizeof(uint64_t));
+}
+
+void ref_rs_encode_ext3(const uint64_t* coeffs, size_t num_polys, size_t poly_size, size_t cw,
+                        size_t depth, uint64_t* out) {
+    std::vector<std::span<const GoldilocksExt3>> polys;
+    for (size_t k = 0; k < num_polys; ++k)
+        polys.emplace_back(reinterpret_cast<const GoldilocksExt3*>(coeffs) + k * poly_size,
+                           poly_size);
+    auto m = whir::algebra::ntt::interleaved_rs_encode<GoldilocksExt3>(
+        whir::algebra::ntt::goldilocks_ext3_engine(),
+        std::span<const std::span<const GoldilocksExt3>>{polys}, cw, depth);
+    std::memcpy(out, m.data(), m.size() * sizeof(GoldilocksExt3));
+}
+
+void ref_commit_leaves_base(const uint64_t* matrix, size_t rows, size_t cols, uint8_t* leaves) {
+    std::vector<Hash> out(rows);
+    whir::protocols::matrix_commit::commit_leaves<Goldilocks>(
+        blake3(), std::span<const Goldilocks>{reinterpret_cast<const Goldilocks*>(matrix), rows * cols},
+        cols, std::span<Hash>{out});
+    std::memcpy(leaves, out.data(), rows * 32);
+}
+
+void ref_commit_leaves_ext3(const uint64_t* matrix, size_t rows, size_t cols, uint8_t* leaves) {
+    std::vector<Hash> out(rows);
+    whir::protocols::matrix_commit::commit_leaves<GoldilocksExt3>(
+        blake3(),
+        std::span<const GoldilocksExt3>{reinterpret_cast<const GoldilocksExt3*>(matrix), rows * cols},
+        cols, std::span<Hash>{out});
+    std::memcpy(leaves, out.data(), rows * 32);
+}
+
+void ref_merkle_build(const uint8_t* leaves, size_t n, uint8_t* nodes) {
+    namespace mt = whir::protocols::merkle_tree;
+    auto cfg = mt::make_config(whir::hash::ENGINE_ID_BLAKE3, n);
+    std::vector<Hash> lv(n);
+    if (n) std::memcpy(lv.data(), leaves, n * 32);
+    auto w = mt::build_tree(cfg, std::move(lv),
+                            [](whir::EngineId) -> const whir::hash::HashEngine& { return blake3(); });
+    std::memcpy(nodes, w.nodes.data(), w.nodes.size() * 32);
+}
+
+size_t ref_merkle_open(const uint8_t* nodes, size_t n_leaves, const uint64_t* idx, size_t n_idx,
+                       uint8_t* hints) {
+    namespace mt = whir::protocols::merkle_tree;
+    auto cfg = mt::make_config(whir::hash::ENGINE_ID_BLAKE3, n_leaves);
+    mt::Witness w;
+    w.nodes.resize(cfg.num_nodes());
+    std::memcpy(w.nodes.data(), nodes, w.nodes.size() * 32);
+    std::vector<size_t> ix(idx, idx + n_idx);
+    auto h = mt::open_path(cfg, w, std::span<const size_t>{ix});
+    if (hints && !h.empty()) std::memcpy(hints, h.data(), h.size() * 32);
+    return h.size();
+}
+
+void ref_sumcheck_compute_ext3(const uint64_t* a, const uint64_t* b, size_t n, uint64_t c0[3],
+                               uint64_t c2[3]) {
+    auto va = to_e3(a, n), vb = to_e3(b, n);
+    auto r = whir::algebra::compute_sumcheck_polynomial<GoldilocksExt3>(va, vb);
+    from_e3(r.first, c0);
+    from_e3(r.second, c2);
+}
+
+void ref_sumcheck_fold_ext3(uint64_t* v, size_t n, const uint64_t r[3]) {
+    auto vv = to_e3(v, n);
+    GoldilocksExt3 w(Goldilocks::from_u64(r[0]), Goldilocks::from_u64(r[1]), Goldilocks::from_u64(r[2]));
+    whir::algebra::fold<GoldilocksExt3>(vv, w);
+    for (size_t i = 0; i < vv.size(); ++i) from_e3(vv[i], v + 3 * i);
+}
+
+void ref_horner_base_ext3(const uint64_t* coeffs, size_t n, const uint64_t z[3], uint64_t out[3]) {
+    whir::algebra::Basefield<GoldilocksExt3> emb;
+    GoldilocksExt3 p(Goldilocks::from_u64(z[0]), Goldilocks::from_u64(z[1]), Goldilocks::from_u64(z[2]));
+    auto r = whir::algebra::mixed_univariate_evaluate<whir::algebra::Basefield<GoldilocksExt3>>(
+        emb, std::span<const Goldilocks>{reinterpret_cast<const Goldilocks*>(coeffs), n}, p);
+    from_e3(r, out);
+}
+
+// One plain commit of a single base-field polynomial: RS encode, row leaves,
+// Merkle tree (irs_commit.hpp:382-460 CPU branch).  The root is written to
+// out_root; the matrix to out_matrix when non-null.
+void ref_commit_base(const uint64_t* coeffs, size_t poly_size, size_t cw, size_t depth,
+                     uint64_t* out_matrix, uint8_t* out_root) {
+    namespace mt = whir::protocols::merkle_tree;
+    std::span<const Goldilocks> poly{reinterpret_cast<const Goldilocks*>(coeffs), poly_size};
+    std::vector<std::span<const Goldilocks>> polys{poly};
+    auto matrix = whir::algebra::ntt::interleaved_rs_encode<Goldilocks>(
+        whir::algebra::ntt::goldilocks_engine(),
+        std::span<const std::span<const Goldilocks>>{polys}, cw, depth);
+    std::vector<Hash> leaves(cw);
+    whir::protocols::matrix_commit::commit_leaves<Goldilocks>(
+        blake3(), std::span<const Goldilocks>{matrix}, depth, std::span<Hash>{leaves});
+    auto cfg = mt::make_config(whir::hash::ENGINE_ID_BLAKE3, cw);
+    auto w = mt::build_tree(cfg, std::move(leaves),
+                            [](whir::EngineId) -> const whir::hash::HashEngine& { return blake3(); });
+    std::memcpy(out_root, w.nodes.back().data(), 32);
+    if (out_matrix) std::memcpy(out_matrix, matrix.data(), matrix.size() * 8);
+}
+
+}  // extern "C"
