@@ -1,0 +1,196 @@
+/*
+ * whir_gpu.h — C ABI of the MI355X (gfx950) WHIR commit-and-fold library
+ * (libwhir_gpu.so).
+ *
+ * Plain C types only.  Field elements are canonical Goldilocks values in
+ * [0, p), p = 2^64 - 2^32 + 1, one uint64_t each; cubic-extension elements
+ * are three uint64_t in the reference's AoS order (c0, c1, c2); hashes are 32
+ * raw bytes.  Matrices are row-major codeword_length x (num_polys * depth)
+ * exactly as the reference's interleaved_rs_encode returns them.
+ *
+ * Status codes (every entry point returning int):
+ *    0  done
+ *   >0  not handled (unsupported shape, dispatch disabled): the caller runs
+ *       its own CPU path, as the reference does when its try_gpu_* returns false
+ *   <0  device error (HIP failure); whir_last_error() describes it
+ *
+ * Two families:
+ *  - whir_gpu_*: host-pointer drop-ins for the reference's whir::cuda::gpu_*
+ *    functions (cpp/cuda/cuda_integration.hpp); each copies inputs to the
+ *    device, runs, and copies results back, like the reference.  They use a
+ *    per-process default context on the current HIP device.
+ *  - whir_dev_*: the device-resident forms.  Pointers are device addresses,
+ *    work is ordered on the context's stream and is asynchronous unless the
+ *    function returns host results (then it synchronizes the stream).
+ *
+ * Citations are relative to the reference repository (hxd77/WhirPlonk).
+ */
+#ifndef WHIR_GPU_H
+#define WHIR_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct whir_ctx whir_ctx;
+
+/* ---------------------------------------------------------------- library */
+
+/* Message of the last error on this thread ("" if none). */
+const char* whir_last_error(void);
+/* 1 when a gfx950 device is visible and usable. */
+int whir_device_available(void);
+
+/* Runtime gates, cpp/cuda/cuda_integration.hpp:51-84.  Dispatch starts
+ * enabled unless WHIR_GPU_DISABLE (or the reference's WHIR_CUDA_DISABLE) is set
+ * to a non-zero value; the threshold (default 65536 elements, :39) applies to
+ * whir_gpu_ntt_batch only, as in the reference's NttEngine::ntt_batch. */
+int whir_gpu_dispatch_enabled(void);
+void whir_set_gpu_dispatch_enabled(int enabled);
+size_t whir_gpu_ntt_threshold(void);
+void whir_set_gpu_ntt_threshold(size_t threshold);
+/* cuda_warmup, cuda_integration.hpp:90-92: create the default context and
+ * load every kernel module. */
+int whir_gpu_warmup(void);
+
+/* ---------------------------------------------------------------- context */
+
+/* An explicit context: device, stream, grow-only scratch pools and cached
+ * twiddle tables (the reference's GpuPool singleton, :142-282, made explicit). */
+int whir_ctx_create(int device, whir_ctx** out);
+int whir_ctx_destroy(whir_ctx* ctx);
+/* Run on a caller-provided stream (hipStream_t passed as void*); NULL restores
+ * the context's own stream. */
+int whir_ctx_set_stream(whir_ctx* ctx, void* hip_stream);
+void* whir_ctx_stream(whir_ctx* ctx);
+int whir_ctx_synchronize(whir_ctx* ctx);
+
+/* Device memory helpers for callers without their own allocator. */
+int whir_dev_malloc(whir_ctx* ctx, size_t bytes, void** out);
+int whir_dev_free(whir_ctx* ctx, void* p);
+int whir_memcpy_h2d(whir_ctx* ctx, void* dst, const void* src, size_t bytes);
+int whir_memcpy_d2h(whir_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+/* ------------------------------------------- host-pointer drop-in functions */
+
+/* NttEngine::ntt_batch GPU branch, cooley_tukey.hpp:200-232 -> gpu_ntt_batch
+ * (cuda_integration.hpp:346-386): forward NTT in place of every contiguous
+ * block of `size` in values[0..total).  Natural order in and out. */
+int whir_gpu_ntt_batch(uint64_t* values, size_t total, size_t size);
+/* NttEngine::intt_batch (cooley_tukey.hpp:767-776): inverse without 1/N. */
+int whir_gpu_intt_batch(uint64_t* values, size_t total, size_t size);
+
+/* gpu_interleaved_rs_encode, cuda_integration.hpp:432-485 (and the CPU
+ * interleaved_rs_encode, mod_ntt.hpp:133-141): coeffs num_polys x poly_size,
+ * out row-major codeword_length x (num_polys * interleaving_depth). */
+int whir_gpu_interleaved_rs_encode(const uint64_t* coeffs, uint64_t* out, size_t num_polys, size_t poly_size,
+                                   size_t codeword_length, size_t interleaving_depth);
+
+/* gpu_interleaved_rs_encode_blake3_matrix_leaves, cuda_integration.hpp:555-621:
+ * as above plus out_hashes[i] = BLAKE3(row i) (32 bytes each).  The reference
+ * refuses rows that are not a multiple of 64 bytes or exceed 1024 bytes; this
+ * library hashes any row length (standard BLAKE3, as the reference's CPU path). */
+int whir_gpu_interleaved_rs_encode_blake3_matrix_leaves(const uint64_t* coeffs, uint64_t* out_matrix,
+                                                        uint8_t* out_hashes, size_t num_polys, size_t poly_size,
+                                                        size_t codeword_length, size_t interleaving_depth);
+
+/* gpu_interleaved_rs_encode_blake3_ext3_matrix_leaves, cuda_integration.hpp:702-788:
+ * Ext3 vectors given as three SoA component arrays, codeword returned as
+ * three SoA component matrices, leaves hashed over the AoS 24-byte limbs. */
+int whir_gpu_interleaved_rs_encode_blake3_ext3_matrix_leaves(const uint64_t* coeff0, const uint64_t* coeff1,
+                                                             const uint64_t* coeff2, uint64_t* out0, uint64_t* out1,
+                                                             uint64_t* out2, uint8_t* out_hashes, size_t num_polys,
+                                                             size_t poly_size, size_t codeword_length,
+                                                             size_t interleaving_depth);
+
+/* gpu_blake3_merkle_tree, cuda_integration.hpp:942-990: zero-padded binary
+ * tree over num_leaves 32-byte leaves; out_nodes (optional) receives the full
+ * witness, (2^(L+1) - 1) * 32 bytes, L = ceil(log2 num_leaves). */
+int whir_gpu_blake3_merkle_tree(const uint8_t* leaves, size_t num_leaves, uint8_t* out_root, uint8_t* out_nodes);
+
+/* merkle_hint_node_indices, cuda_integration.hpp:854-885: node indices of the
+ * open_path hints (merkle_tree.hpp:165-211) in order.  Returns the count;
+ * writes them when out != NULL. */
+size_t whir_merkle_hint_node_indices(size_t num_leaves, const uint64_t* indices, size_t num_indices, uint64_t* out);
+
+/* gpu_blake3_merkle_open_path, cuda_integration.hpp:1047-1100: root and the
+ * sibling hints of open_path for the given (unsorted, possibly repeated) leaf
+ * indices.  out_hints must hold whir_merkle_hint_node_indices(...) * 32 bytes. */
+int whir_gpu_blake3_merkle_open_path(const uint8_t* leaves, size_t num_leaves, const uint64_t* indices,
+                                     size_t num_indices, uint8_t* out_root, uint8_t* out_hints);
+
+/* gpu_ood_evaluate_ext3_from_base, cuda_integration.hpp:1103-1149: for each
+ * Ext3 point p (3 u64) and vector v, out[(p*num_vectors + v)*3 ..] =
+ * sum_i coeff0[v*vector_size + i] * point_p^i  (mixed_univariate_evaluate). */
+int whir_gpu_ood_evaluate_ext3_from_base(const uint64_t* coeff0, const uint64_t* points, uint64_t* out,
+                                         size_t num_vectors, size_t vector_size, size_t num_points);
+/* gpu_ood_evaluate_ext3_from_ext3, cuda_integration.hpp:1151-1205: SoA Ext3
+ * coefficients. */
+int whir_gpu_ood_evaluate_ext3_from_ext3(const uint64_t* coeff0, const uint64_t* coeff1, const uint64_t* coeff2,
+                                         const uint64_t* points, uint64_t* out, size_t num_vectors,
+                                         size_t vector_size, size_t num_points);
+
+/* gpu_encode_to_bytes, cuda_integration.hpp:1208-1217: u64 -> 8 LE bytes. */
+int whir_gpu_encode_to_bytes(const uint64_t* values, uint8_t* out, size_t count);
+
+/* Blake3::hash_many, hash/blake3_engine.hpp:90-199: out[i] = BLAKE3 of the
+ * i-th `size`-byte message (any size, including 0 and > 1024). */
+int whir_gpu_blake3_hash_many(size_t size, const uint8_t* input, size_t count, uint8_t* output);
+
+/* compute_sumcheck_polynomial over Ext3, algebra/sumcheck.hpp:43-98 (no GPU
+ * counterpart in the reference): a, b are n Ext3 AoS values (a_is_base: a is
+ * n base-field values, lifted).  Writes c0[3], c2[3]. */
+int whir_gpu_sumcheck_compute(const uint64_t* a, int a_is_base, const uint64_t* b, size_t n, uint64_t* out_c0,
+                              uint64_t* out_c2);
+/* fold, algebra/sumcheck.hpp:103-116: out[i] = v[i] + (v[i+n/2] - v[i]) * r,
+ * i < n/2, Ext3 out (v base when v_is_base). */
+int whir_gpu_sumcheck_fold(const uint64_t* v, int v_is_base, size_t n, const uint64_t* r, uint64_t* out);
+
+/* ----------------------------------------------- device-resident functions */
+
+/* Interleaved RS encode on device buffers (row-major output). */
+int whir_dev_rs_encode(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_polys, size_t poly_size,
+                       size_t codeword_length, size_t interleaving_depth, uint64_t* d_out);
+/* Ext3 AoS in and out. */
+int whir_dev_rs_encode_ext3(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_polys, size_t poly_size,
+                            size_t codeword_length, size_t interleaving_depth, uint64_t* d_out);
+/* In-place forward (inverse != 0: inverse, unscaled) NTT of blocks of size. */
+int whir_dev_ntt_batch(whir_ctx* ctx, uint64_t* d_values, size_t total, size_t size, int inverse);
+/* d_leaves[i] = BLAKE3 of row i of a rows x row_u64 u64 matrix. */
+int whir_dev_commit_leaves(whir_ctx* ctx, const uint64_t* d_matrix, size_t rows, size_t row_u64, uint8_t* d_leaves);
+int whir_dev_blake3_hash_many(whir_ctx* ctx, size_t size, const uint8_t* d_input, size_t count, uint8_t* d_output);
+/* Number of nodes of the tree over num_leaves leaves (merkle_tree.hpp:58-60). */
+size_t whir_merkle_num_nodes(size_t num_leaves);
+/* Build the tree in d_nodes (whir_merkle_num_nodes * 32 bytes) whose first
+ * num_leaves entries already hold the leaves; root = last node. */
+int whir_dev_merkle_build(whir_ctx* ctx, uint8_t* d_nodes, size_t num_leaves);
+/* d_out[i] = d_nodes[d_node_idx[i]] (32 bytes each). */
+int whir_dev_gather_hashes(whir_ctx* ctx, const uint8_t* d_nodes, const uint64_t* d_node_idx, size_t count,
+                           uint8_t* d_out);
+/* The commit hot path of irs_commit::Config::commit (irs_commit.hpp:344-460)
+ * for base-field vectors: RS encode into d_matrix, row leaves into the first
+ * codeword_length nodes of d_nodes, the tree, and the root copied to out_root
+ * (host, 32 bytes; the call synchronizes the stream). */
+int whir_dev_commit_base(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_polys, size_t poly_size,
+                         size_t codeword_length, size_t interleaving_depth, uint64_t* d_matrix, uint8_t* d_nodes,
+                         uint8_t* out_root);
+/* Same for Ext3 AoS vectors (24-byte limbs per element in each leaf). */
+int whir_dev_commit_ext3(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_polys, size_t poly_size,
+                         size_t codeword_length, size_t interleaving_depth, uint64_t* d_matrix, uint8_t* d_nodes,
+                         uint8_t* out_root);
+/* Sumcheck round polynomial on device vectors; c0, c2 copied to the host. */
+int whir_dev_sumcheck_compute(whir_ctx* ctx, const uint64_t* d_a, int a_is_base, const uint64_t* d_b, size_t n,
+                              uint64_t* out_c0, uint64_t* out_c2);
+/* Fold on device (d_out may equal d_v when v is Ext3). */
+int whir_dev_sumcheck_fold(whir_ctx* ctx, const uint64_t* d_v, int v_is_base, size_t n, const uint64_t* r,
+                           uint64_t* d_out);
+/* sum_i f_i z^i, f base (f_is_ext3 = 0) or Ext3 AoS; z and out on the host. */
+int whir_dev_horner(whir_ctx* ctx, const uint64_t* d_f, int f_is_ext3, size_t n, const uint64_t* z, uint64_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

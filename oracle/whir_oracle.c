@@ -31,10 +31,16 @@ uint64_t or_gl_sub(uint64_t a, uint64_t b) {
 uint64_t or_gl_neg(uint64_t a) { return a == 0 ? 0 : OR_P - a; }
 
 uint64_t or_gl_mul(uint64_t a, uint64_t b) {
-    /* goldilocks.hpp:116-124: the full 128-bit product reduced mod p.  The
-     * reduction here is the plain remainder; the reference's 2^64 = eps,
-     * 2^96 = -1 folding gives the same canonical value. */
-    return (uint64_t)(((u128)a * b) % OR_P);
+    /* goldilocks.hpp:116-124: the full 128-bit product reduced mod p, here by
+     * splitting x = x3*2^96 + x2*2^64 + lo and using 2^64 = 2^32 - 1 and
+     * 2^96 = -1, with exact 128-bit intermediate sums (no lazy wrap-around):
+     * x = lo + x2*(2^32 - 1) - x3 (mod p). */
+    u128 x = (u128)a * b;
+    uint64_t lo = (uint64_t)x, hi = (uint64_t)(x >> 64);
+    u128 t = (u128)lo + (u128)(hi & 0xFFFFFFFFULL) * 0xFFFFFFFFULL + OR_P; /* < 2^66 */
+    t -= (hi >> 32);
+    while (t >= OR_P) t -= OR_P; /* t < 4p: at most three subtractions */
+    return (uint64_t)t;
 }
 
 uint64_t or_gl_pow(uint64_t a, uint64_t e) {

@@ -1,0 +1,332 @@
+"""whirplonk_amd — MI355X (gfx950) WHIR commit-and-fold path.
+
+Python view of the C ABI in include/whir_gpu.h (libwhir_gpu.so, built
+in-tree by ``whirplonk_amd.build``).  The library is the product; this module
+only loads it with ctypes and converts numpy arrays.  Importing never falls
+back to anything: if the shared library is missing or fails to load, every
+call raises.
+
+Mirrors the reference's GPU entry points (cpp/cuda/cuda_integration.hpp) and
+its CPU algebra calls (ntt_batch, interleaved_rs_encode, commit_leaves,
+merkle build/open, compute_sumcheck_polynomial, fold,
+mixed_univariate_evaluate) with the same argument meaning.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libwhir_gpu.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "whir_gpu.h")
+
+P = 0xFFFFFFFF00000001
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_vp = ctypes.c_void_p
+_sz = ctypes.c_size_t
+_int = ctypes.c_int
+
+
+class WhirGpuError(RuntimeError):
+    """A device error (status < 0) or an unsupported request (status > 0)."""
+
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"status {status}: {msg}")
+        self.status = status
+
+
+# (name, restype, argtypes) for every exported symbol of include/whir_gpu.h
+_SIGS = [
+    ("whir_last_error", ctypes.c_char_p, []),
+    ("whir_device_available", _int, []),
+    ("whir_gpu_dispatch_enabled", _int, []),
+    ("whir_set_gpu_dispatch_enabled", None, [_int]),
+    ("whir_gpu_ntt_threshold", _sz, []),
+    ("whir_set_gpu_ntt_threshold", None, [_sz]),
+    ("whir_gpu_warmup", _int, []),
+    ("whir_ctx_create", _int, [_int, ctypes.POINTER(_vp)]),
+    ("whir_ctx_destroy", _int, [_vp]),
+    ("whir_ctx_set_stream", _int, [_vp, _vp]),
+    ("whir_ctx_stream", _vp, [_vp]),
+    ("whir_ctx_synchronize", _int, [_vp]),
+    ("whir_dev_malloc", _int, [_vp, _sz, ctypes.POINTER(_vp)]),
+    ("whir_dev_free", _int, [_vp, _vp]),
+    ("whir_memcpy_h2d", _int, [_vp, _vp, _vp, _sz]),
+    ("whir_memcpy_d2h", _int, [_vp, _vp, _vp, _sz]),
+    ("whir_gpu_ntt_batch", _int, [_u64p, _sz, _sz]),
+    ("whir_gpu_intt_batch", _int, [_u64p, _sz, _sz]),
+    ("whir_gpu_interleaved_rs_encode", _int, [_u64p, _u64p, _sz, _sz, _sz, _sz]),
+    ("whir_gpu_interleaved_rs_encode_blake3_matrix_leaves", _int, [_u64p, _u64p, _u8p, _sz, _sz, _sz, _sz]),
+    ("whir_gpu_interleaved_rs_encode_blake3_ext3_matrix_leaves", _int,
+     [_u64p, _u64p, _u64p, _u64p, _u64p, _u64p, _u8p, _sz, _sz, _sz, _sz]),
+    ("whir_gpu_blake3_merkle_tree", _int, [_u8p, _sz, _u8p, _u8p]),
+    ("whir_merkle_hint_node_indices", _sz, [_sz, _u64p, _sz, _u64p]),
+    ("whir_gpu_blake3_merkle_open_path", _int, [_u8p, _sz, _u64p, _sz, _u8p, _u8p]),
+    ("whir_gpu_ood_evaluate_ext3_from_base", _int, [_u64p, _u64p, _u64p, _sz, _sz, _sz]),
+    ("whir_gpu_ood_evaluate_ext3_from_ext3", _int, [_u64p, _u64p, _u64p, _u64p, _u64p, _sz, _sz, _sz]),
+    ("whir_gpu_encode_to_bytes", _int, [_u64p, _u8p, _sz]),
+    ("whir_gpu_blake3_hash_many", _int, [_sz, _u8p, _sz, _u8p]),
+    ("whir_gpu_sumcheck_compute", _int, [_u64p, _int, _u64p, _sz, _u64p, _u64p]),
+    ("whir_gpu_sumcheck_fold", _int, [_u64p, _int, _sz, _u64p, _u64p]),
+    ("whir_dev_rs_encode", _int, [_vp, _vp, _sz, _sz, _sz, _sz, _vp]),
+    ("whir_dev_rs_encode_ext3", _int, [_vp, _vp, _sz, _sz, _sz, _sz, _vp]),
+    ("whir_dev_ntt_batch", _int, [_vp, _vp, _sz, _sz, _int]),
+    ("whir_dev_commit_leaves", _int, [_vp, _vp, _sz, _sz, _vp]),
+    ("whir_dev_blake3_hash_many", _int, [_vp, _sz, _vp, _sz, _vp]),
+    ("whir_merkle_num_nodes", _sz, [_sz]),
+    ("whir_dev_merkle_build", _int, [_vp, _vp, _sz]),
+    ("whir_dev_gather_hashes", _int, [_vp, _vp, _vp, _sz, _vp]),
+    ("whir_dev_commit_base", _int, [_vp, _vp, _sz, _sz, _sz, _sz, _vp, _vp, _u8p]),
+    ("whir_dev_commit_ext3", _int, [_vp, _vp, _sz, _sz, _sz, _sz, _vp, _vp, _u8p]),
+    ("whir_dev_sumcheck_compute", _int, [_vp, _vp, _int, _vp, _sz, _u64p, _u64p]),
+    ("whir_dev_sumcheck_fold", _int, [_vp, _vp, _int, _sz, _u64p, _vp]),
+    ("whir_dev_horner", _int, [_vp, _vp, _int, _sz, _u64p, _u64p]),
+]
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load libwhir_gpu.so (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing; build it with `python -m whirplonk_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in _SIGS:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def symbols() -> list[str]:
+    return [s[0] for s in _SIGS]
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise WhirGpuError(rc, load().whir_last_error().decode())
+
+
+def _p64(a: np.ndarray):
+    return a.ctypes.data_as(_u64p)
+
+
+def _p8(a: np.ndarray):
+    return a.ctypes.data_as(_u8p)
+
+
+def _u64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint64))
+
+
+def device_available() -> bool:
+    return bool(load().whir_device_available())
+
+
+# ------------------------------------------------------- host-pointer forms
+
+def ntt_batch(values, size: int, inverse: bool = False) -> np.ndarray:
+    v = _u64(values).copy()
+    fn = load().whir_gpu_intt_batch if inverse else load().whir_gpu_ntt_batch
+    _check(fn(_p64(v), v.size, size))
+    return v
+
+
+def interleaved_rs_encode(coeffs, num_polys, poly_size, codeword_length, depth) -> np.ndarray:
+    c = _u64(coeffs)
+    out = np.empty(codeword_length * num_polys * depth, dtype=np.uint64)
+    _check(load().whir_gpu_interleaved_rs_encode(_p64(c), _p64(out), num_polys, poly_size, codeword_length, depth))
+    return out
+
+
+def rs_encode_leaves(coeffs, num_polys, poly_size, codeword_length, depth):
+    c = _u64(coeffs)
+    out = np.empty(codeword_length * num_polys * depth, dtype=np.uint64)
+    leaves = np.empty(32 * codeword_length, dtype=np.uint8)
+    _check(load().whir_gpu_interleaved_rs_encode_blake3_matrix_leaves(
+        _p64(c), _p64(out), _p8(leaves), num_polys, poly_size, codeword_length, depth))
+    return out, leaves
+
+
+def rs_encode_leaves_ext3(coeffs_aos, num_polys, poly_size, codeword_length, depth):
+    """Ext3 AoS coefficients in; AoS matrix and leaves out (the ABI is SoA)."""
+    c = _u64(coeffs_aos)
+    comps = [np.ascontiguousarray(c[k::3]) for k in range(3)]
+    n = codeword_length * num_polys * depth
+    outs = [np.empty(n, dtype=np.uint64) for _ in range(3)]
+    leaves = np.empty(32 * codeword_length, dtype=np.uint8)
+    _check(load().whir_gpu_interleaved_rs_encode_blake3_ext3_matrix_leaves(
+        _p64(comps[0]), _p64(comps[1]), _p64(comps[2]), _p64(outs[0]), _p64(outs[1]), _p64(outs[2]),
+        _p8(leaves), num_polys, poly_size, codeword_length, depth))
+    aos = np.empty(3 * n, dtype=np.uint64)
+    for k in range(3):
+        aos[k::3] = outs[k]
+    return aos, leaves
+
+
+def merkle_tree(leaves: np.ndarray, num_leaves: int, full: bool = True):
+    lv = np.ascontiguousarray(leaves, dtype=np.uint8)
+    if lv.size == 0:
+        lv = np.zeros(32, dtype=np.uint8)
+    root = np.empty(32, dtype=np.uint8)
+    nodes = np.empty(32 * load().whir_merkle_num_nodes(num_leaves), dtype=np.uint8) if full else None
+    _check(load().whir_gpu_blake3_merkle_tree(_p8(lv), num_leaves, _p8(root), _p8(nodes) if full else None))
+    return root, nodes
+
+
+def merkle_hint_node_indices(num_leaves: int, indices) -> np.ndarray:
+    ix = _u64(indices) if len(indices) else np.zeros(1, dtype=np.uint64)
+    n = load().whir_merkle_hint_node_indices(num_leaves, _p64(ix), len(indices), None)
+    out = np.empty(max(n, 1), dtype=np.uint64)
+    load().whir_merkle_hint_node_indices(num_leaves, _p64(ix), len(indices), _p64(out))
+    return out[:n]
+
+
+def merkle_open_path(leaves: np.ndarray, num_leaves: int, indices):
+    lv = np.ascontiguousarray(leaves, dtype=np.uint8)
+    ix = _u64(indices) if len(indices) else np.zeros(1, dtype=np.uint64)
+    nh = len(merkle_hint_node_indices(num_leaves, indices))
+    root = np.empty(32, dtype=np.uint8)
+    hints = np.empty(max(32 * nh, 32), dtype=np.uint8)
+    _check(load().whir_gpu_blake3_merkle_open_path(_p8(lv), num_leaves, _p64(ix), len(indices), _p8(root), _p8(hints)))
+    return root, hints[: 32 * nh]
+
+
+def ood_evaluate_from_base(coeffs, points, num_vectors, vector_size) -> np.ndarray:
+    c, pts = _u64(coeffs), _u64(points)
+    npnt = pts.size // 3
+    out = np.empty(3 * npnt * num_vectors, dtype=np.uint64)
+    _check(load().whir_gpu_ood_evaluate_ext3_from_base(_p64(c), _p64(pts), _p64(out), num_vectors, vector_size, npnt))
+    return out
+
+
+def ood_evaluate_from_ext3(coeffs_aos, points, num_vectors, vector_size) -> np.ndarray:
+    c, pts = _u64(coeffs_aos), _u64(points)
+    comps = [np.ascontiguousarray(c[k::3]) for k in range(3)]
+    npnt = pts.size // 3
+    out = np.empty(3 * npnt * num_vectors, dtype=np.uint64)
+    _check(load().whir_gpu_ood_evaluate_ext3_from_ext3(_p64(comps[0]), _p64(comps[1]), _p64(comps[2]), _p64(pts),
+                                                       _p64(out), num_vectors, vector_size, npnt))
+    return out
+
+
+def blake3_hash_many(size: int, data: bytes, count: int) -> np.ndarray:
+    buf = np.frombuffer(bytes(data) + b"\0" * 16, dtype=np.uint8)
+    out = np.empty(max(32 * count, 32), dtype=np.uint8)
+    _check(load().whir_gpu_blake3_hash_many(size, _p8(buf), count, _p8(out)))
+    return out[: 32 * count]
+
+
+def sumcheck_compute(a, b, a_is_base: bool = False):
+    a, b = _u64(a), _u64(b)
+    n = b.size // 3
+    c0, c2 = np.empty(3, dtype=np.uint64), np.empty(3, dtype=np.uint64)
+    _check(load().whir_gpu_sumcheck_compute(_p64(a), int(a_is_base), _p64(b), n, _p64(c0), _p64(c2)))
+    return c0, c2
+
+
+def sumcheck_fold(v, r, v_is_base: bool = False) -> np.ndarray:
+    v, r = _u64(v), _u64(r)
+    n = v.size if v_is_base else v.size // 3
+    out = np.empty(3 * (n // 2) + 1, dtype=np.uint64)
+    _check(load().whir_gpu_sumcheck_fold(_p64(v), int(v_is_base), n, _p64(r), _p64(out)))
+    return out[: 3 * (n // 2)]
+
+
+# ------------------------------------------------------ device-resident forms
+
+class Context:
+    """An explicit device context (stream + pools) with device buffers."""
+
+    def __init__(self, device: int = 0):
+        h = _vp()
+        _check(load().whir_ctx_create(device, ctypes.byref(h)))
+        self.h = h
+        self._bufs = []
+
+    def close(self):
+        if self.h:
+            for b in self._bufs:
+                load().whir_dev_free(self.h, b)
+            self._bufs = []
+            load().whir_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def alloc(self, nbytes: int) -> int:
+        p = _vp()
+        _check(load().whir_dev_malloc(self.h, nbytes, ctypes.byref(p)))
+        self._bufs.append(p.value)
+        return p.value
+
+    def free(self, ptr: int) -> None:
+        if ptr in self._bufs:
+            self._bufs.remove(ptr)
+        _check(load().whir_dev_free(self.h, ptr))
+
+    def upload(self, arr: np.ndarray) -> int:
+        arr = np.ascontiguousarray(arr)
+        p = self.alloc(max(arr.nbytes, 1))
+        _check(load().whir_memcpy_h2d(self.h, p, arr.ctypes.data, arr.nbytes))
+        return p
+
+    def download(self, ptr: int, count: int, dtype=np.uint64) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        _check(load().whir_memcpy_d2h(self.h, out.ctypes.data, ptr, out.nbytes))
+        return out
+
+    def sync(self):
+        _check(load().whir_ctx_synchronize(self.h))
+
+    def rs_encode(self, d_coeffs, num_polys, poly_size, cw, depth, d_out, ext3=False):
+        fn = load().whir_dev_rs_encode_ext3 if ext3 else load().whir_dev_rs_encode
+        _check(fn(self.h, d_coeffs, num_polys, poly_size, cw, depth, d_out))
+
+    def ntt_batch(self, d_values, total, size, inverse=False):
+        _check(load().whir_dev_ntt_batch(self.h, d_values, total, size, int(inverse)))
+
+    def commit(self, d_coeffs, num_polys, poly_size, cw, depth, d_matrix, d_nodes, ext3=False) -> bytes:
+        root = np.empty(32, dtype=np.uint8)
+        fn = load().whir_dev_commit_ext3 if ext3 else load().whir_dev_commit_base
+        _check(fn(self.h, d_coeffs, num_polys, poly_size, cw, depth, d_matrix, d_nodes, _p8(root)))
+        return root.tobytes()
+
+    def commit_leaves(self, d_matrix, rows, row_u64, d_leaves):
+        _check(load().whir_dev_commit_leaves(self.h, d_matrix, rows, row_u64, d_leaves))
+
+    def merkle_build(self, d_nodes, num_leaves):
+        _check(load().whir_dev_merkle_build(self.h, d_nodes, num_leaves))
+
+    def gather_hashes(self, d_nodes, d_idx, count, d_out):
+        _check(load().whir_dev_gather_hashes(self.h, d_nodes, d_idx, count, d_out))
+
+    def sumcheck_compute(self, d_a, a_is_base, d_b, n):
+        c0, c2 = np.empty(3, dtype=np.uint64), np.empty(3, dtype=np.uint64)
+        _check(load().whir_dev_sumcheck_compute(self.h, d_a, int(a_is_base), d_b, n, _p64(c0), _p64(c2)))
+        return c0, c2
+
+    def sumcheck_fold(self, d_v, v_is_base, n, r, d_out):
+        rr = _u64(r)
+        _check(load().whir_dev_sumcheck_fold(self.h, d_v, int(v_is_base), n, _p64(rr), d_out))
+
+    def horner(self, d_f, f_is_ext3, n, z) -> np.ndarray:
+        zz = _u64(z)
+        out = np.empty(3, dtype=np.uint64)
+        _check(load().whir_dev_horner(self.h, d_f, int(f_is_ext3), n, _p64(zz), _p64(out)))
+        return out
+
+
+def merkle_num_nodes(n: int) -> int:
+    return load().whir_merkle_num_nodes(n)

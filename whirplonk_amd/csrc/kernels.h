@@ -1,0 +1,62 @@
+// kernels.h — host-side launchers of the HIP kernels (internal to the library).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace whir_gpu {
+
+// Twiddle tables of one transform size n = 2^log_n, forward or inverse:
+// lo[x] = w^x (x < 2048), hi[y] = w^(2048 y) (y < max(1, n/2048)) so that
+// w^e = lo[e & 2047] * hi[e >> 11]; t1024[t] = w_1024^t for the sub-DFTs.
+struct Twiddles {
+    uint32_t log_n = 0;
+    uint64_t* lo = nullptr;
+    uint64_t* hi = nullptr;
+    uint64_t* t1024 = nullptr;
+};
+
+// Geometry of one interleaved LDE / batched NTT:
+//   columns c < C, each with m input values at in[(c*m + j)*in_el + in_comp]
+//   (Ext3 AoS input: in_el = 3, in_comp = component), output n = E*m values
+//   per column, written row-major (out[(i*C + c)*out_el + out_comp]) or
+//   column-major (out[((c*n) + i)*out_el + out_comp]).
+struct LdeShape {
+    uint64_t C = 0, m = 0, n = 0;
+    uint32_t in_el = 1, in_comp = 0, out_el = 1, out_comp = 0;
+    bool row_major = true;
+    bool inverse = false;
+};
+
+// Scratch needed by lde_launch (elements of u64).
+size_t lde_scratch_elems(const LdeShape& s);
+// Returns 0, or >0 when the shape is not supported (not a power of two, m > 2^20).
+int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* scratch,
+               const Twiddles& tw, hipStream_t st);
+
+// leaves[i] = BLAKE3(row i), row = row_u64 little-endian u64 words.
+void leaf_hash_launch(const uint64_t* matrix, uint64_t rows, uint32_t row_u64, uint8_t* leaves, hipStream_t st);
+// out[i] = BLAKE3(in[i*size .. (i+1)*size]) for arbitrary byte size.
+void hash_many_launch(const uint8_t* in, uint64_t size, uint64_t count, uint8_t* out, hipStream_t st);
+// Build the flat tree in nodes (leaves already in nodes[0..n), padded to 2^L).
+void merkle_build_launch(uint8_t* nodes, uint64_t n_leaves, hipStream_t st);
+// out[i] = nodes[idx[i]] (32 bytes each)
+void gather_hashes_launch(const uint8_t* nodes, const uint64_t* idx, uint64_t count, uint8_t* out, hipStream_t st);
+void fill_zero_launch(uint8_t* p, uint64_t bytes, hipStream_t st);
+
+// sumcheck over Ext3 (a may be base-field, lifted on the fly)
+// partials must hold 2*3*SUMCHECK_MAX_BLOCKS u64; result c0[3], c2[3] left in
+// partials[0..6) on the device.
+constexpr int SUMCHECK_MAX_BLOCKS = 2048;
+void sumcheck_compute_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint64_t n,
+                             uint64_t* partials, hipStream_t st);
+void sumcheck_fold_launch(const uint64_t* in, bool in_base, uint64_t n, const uint64_t r[3],
+                          uint64_t* out, hipStream_t st);
+
+// sum_i f_i z^i over Ext3 (f base or Ext3 AoS); the result lands in out[0..3)
+// on the device.  zpow[l] = z^(2^l) for l < 64 (Ext3 AoS, 192 u64) on device.
+constexpr int HORNER_MAX_BLOCKS = 1024;
+void horner_launch(const uint64_t* f, bool f_ext3, uint64_t n, const uint64_t* zpow,
+                   uint64_t* partials, uint64_t* out, hipStream_t st);
+
+}  // namespace whir_gpu

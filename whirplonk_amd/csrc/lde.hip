@@ -1,0 +1,281 @@
+// lde.hip — interleaved Reed-Solomon encode (low-degree extension) and batched
+// NTT for Goldilocks on gfx950.
+//
+// Computes, for every column c (one interleaved block of one polynomial,
+// algebra/ntt/mod_ntt.hpp:49-130),
+//     Y_c[i] = sum_{j<m} f_c[j] * w_n^{ij},   i < n = E*m,
+// i.e. the size-n NTT of the zero-padded block, without touching the zeros.
+//
+// Two passes through HBM per coset of the output (four-step NTT), with the
+// register/LDS sub-transforms of dft.h:
+//   j = j2 + M2*j1 (j1 < M1, j2 < M2, m = M1*M2),  i = i1 + L*i2 (L = E*M1)
+//   Y[i1 + L i2] = sum_{j2} w_M2^{i2 j2} * [ w_n^{i1 j2} * sum_{j1} f[j2 + M2 j1] w_L^{i1 j1} ]
+// and i1 = s + E*k splits the inner length-L transform into E cosets:
+//   sum_{j1} f[..] w_L^{i1 j1} = DFT_M1( f[j2 + M2 j1] * w_L^{s j1} )[k].
+// Pass A (per coset s): column tiles of TA consecutive j2 (128-byte coalesced
+// reads), DFT_M1 in LDS, inter-pass twiddle, write Z_s[k][c][j2].
+// Pass B (per coset s): for each k, the C*M2 contiguous values of Z_s,
+// DFT_M2 in LDS, write rows i = s + E k + L i2 of the output, all C columns
+// of a row together (128-byte rows for C = 16).
+// Only the m nonzero inputs are ever read; the scratch Z_s holds m*C values
+// per coset in flight.
+#include <algorithm>
+
+#include "dft.h"
+#include "kernels.h"
+
+namespace whir_gpu {
+namespace {
+
+constexpr int THREADS = 512;
+
+struct LdeArgs {
+    const uint64_t* in;
+    uint64_t* out;
+    uint64_t* z;
+    const uint64_t* tw_lo;
+    const uint64_t* tw_hi;
+    const uint64_t* t1024;
+    uint64_t C, m, n, M1, M2, E, L, zstride;
+    uint32_t in_el, in_comp, out_el, out_comp;
+    uint32_t TA, TB, CB, row_major;
+};
+
+// w_n^e for e < n from the two-level table
+__device__ __forceinline__ uint64_t twid(const LdeArgs& a, uint64_t e) {
+    uint64_t x = a.tw_lo[e & 2047];
+    uint64_t h = e >> 11;
+    return h ? gl::mul(x, a.tw_hi[h]) : x;
+}
+
+template <int M1, bool INV>
+__global__ __launch_bounds__(THREADS) void lde_pass_a(LdeArgs a, uint64_t s0) {
+    using S = dft::Split<M1>;
+    constexpr int R1 = S::R1, R2 = S::R2, PAD = S::PADDED + 1;
+    extern __shared__ uint64_t lds[];
+    const uint32_t TA = a.TA;
+    const uint64_t nj = a.M2 / TA;
+    const uint64_t c = blockIdx.x / nj;
+    const uint64_t J = (blockIdx.x % nj) * TA;
+    const uint64_t s = s0 + blockIdx.y;
+    uint64_t* z = a.z + blockIdx.y * a.zstride;
+    const uint64_t* col = a.in + (c * a.m) * a.in_el + a.in_comp;
+
+    for (uint32_t q = threadIdx.x; q < TA * M1; q += blockDim.x) {
+        uint32_t t = q % TA, j1 = q / TA;
+        uint64_t x = col[(J + t + a.M2 * j1) * a.in_el];
+        if (s != 0 && j1 != 0) x = gl::mul(x, twid(a, (s * j1 * a.M2) & (a.n - 1)));
+        lds[t * PAD + dft::pidx<M1>(j1)] = x;
+    }
+    __syncthreads();
+    for (uint32_t w = threadIdx.x; w < TA * R2; w += blockDim.x) {
+        uint32_t n2 = w % R2, t = w / R2;
+        dft::lds_dft_step1<M1, INV>(lds + t * PAD, n2, a.t1024);
+    }
+    __syncthreads();
+    for (uint32_t w = threadIdx.x; w < TA * R1; w += blockDim.x) {
+        uint32_t t = w % TA, k1 = w / TA;
+        uint64_t v[R2];
+        dft::lds_dft_step2<M1, INV>(lds + t * PAD, k1, v);
+        const uint64_t j2 = J + t;
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) {
+            uint64_t k = k1 + (uint64_t)R1 * k2;
+            uint64_t e = ((s + a.E * k) * j2) & (a.n - 1);
+            uint64_t zv = v[k2];
+            if (e) zv = gl::mul(zv, twid(a, e));
+            z[(k * a.C + c) * a.M2 + j2] = zv;
+        }
+    }
+}
+
+template <int M2, bool INV>
+__global__ __launch_bounds__(THREADS) void lde_pass_b(LdeArgs a, uint64_t s0) {
+    using S = dft::Split<M2>;
+    constexpr int R1 = S::R1, R2 = S::R2, PAD = S::PADDED + 1;
+    extern __shared__ uint64_t lds[];
+    const uint32_t TB = a.TB, CB = a.CB, DB = TB * CB;
+    const uint64_t nk = a.M1 / TB;
+    const uint64_t K0 = (blockIdx.x % nk) * TB;
+    const uint64_t C0 = (blockIdx.x / nk) * CB;
+    const uint64_t s = s0 + blockIdx.y;
+    const uint64_t* z = a.z + blockIdx.y * a.zstride;
+
+    for (uint32_t q = threadIdx.x; q < DB * M2; q += blockDim.x) {
+        uint32_t j2 = q % M2, d = q / M2;
+        uint32_t tb = a.row_major ? d / CB : d % TB;
+        uint32_t cb = a.row_major ? d % CB : d / TB;
+        lds[d * PAD + dft::pidx<M2>(j2)] = z[((K0 + tb) * a.C + C0 + cb) * a.M2 + j2];
+    }
+    __syncthreads();
+    for (uint32_t w = threadIdx.x; w < DB * R2; w += blockDim.x) {
+        uint32_t n2 = w % R2, d = w / R2;
+        dft::lds_dft_step1<M2, INV>(lds + d * PAD, n2, a.t1024);
+    }
+    __syncthreads();
+    for (uint32_t w = threadIdx.x; w < DB * R1; w += blockDim.x) {
+        uint32_t d = w % DB, k1 = w / DB;
+        uint32_t tb = a.row_major ? d / CB : d % TB;
+        uint32_t cb = a.row_major ? d % CB : d / TB;
+        uint64_t v[R2];
+        dft::lds_dft_step2<M2, INV>(lds + d * PAD, k1, v);
+        const uint64_t i1 = s + a.E * (K0 + tb);
+        const uint64_t c = C0 + cb;
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) {
+            uint64_t i = i1 + a.L * (k1 + (uint64_t)R1 * k2);
+            uint64_t idx = a.row_major ? (i * a.C + c) : (c * a.n + i);
+            a.out[idx * a.out_el + a.out_comp] = v[k2];
+        }
+    }
+}
+
+int ilog2u(uint64_t x) {
+    int k = 0;
+    while ((1ull << k) < x) ++k;
+    return k;
+}
+bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
+uint32_t split_r2(uint64_t M) {
+    int lg = ilog2u(M);
+    return (uint32_t)(M >> (lg / 2));
+}
+uint64_t padded(uint64_t M) { return M + M / split_r2(M) + 1; }
+
+using KernelFn = void (*)(LdeArgs, uint64_t);
+
+template <bool INV>
+KernelFn pick_a(uint64_t M) {
+    switch (M) {
+        case 1: return lde_pass_a<1, INV>;
+        case 2: return lde_pass_a<2, INV>;
+        case 4: return lde_pass_a<4, INV>;
+        case 8: return lde_pass_a<8, INV>;
+        case 16: return lde_pass_a<16, INV>;
+        case 32: return lde_pass_a<32, INV>;
+        case 64: return lde_pass_a<64, INV>;
+        case 128: return lde_pass_a<128, INV>;
+        case 256: return lde_pass_a<256, INV>;
+        case 512: return lde_pass_a<512, INV>;
+        case 1024: return lde_pass_a<1024, INV>;
+    }
+    return nullptr;
+}
+
+template <bool INV>
+KernelFn pick_b(uint64_t M) {
+    switch (M) {
+        case 1: return lde_pass_b<1, INV>;
+        case 2: return lde_pass_b<2, INV>;
+        case 4: return lde_pass_b<4, INV>;
+        case 8: return lde_pass_b<8, INV>;
+        case 16: return lde_pass_b<16, INV>;
+        case 32: return lde_pass_b<32, INV>;
+        case 64: return lde_pass_b<64, INV>;
+        case 128: return lde_pass_b<128, INV>;
+        case 256: return lde_pass_b<256, INV>;
+        case 512: return lde_pass_b<512, INV>;
+        case 1024: return lde_pass_b<1024, INV>;
+    }
+    return nullptr;
+}
+
+struct Plan {
+    uint64_t M1, M2, E, L, G;  // G cosets per launch
+    uint32_t TA, TB, CB;
+};
+
+// Largest power of two dividing x (capped).
+uint64_t pow2_div(uint64_t x, uint64_t cap) {
+    uint64_t d = 1;
+    while ((x % (d * 2)) == 0 && d * 2 <= cap) d *= 2;
+    return d;
+}
+
+constexpr uint64_t SCRATCH_TARGET = 1ull << 22;  // u64 per launch group for small problems
+
+bool make_plan(const LdeShape& s, Plan& p) {
+    if (s.C == 0 || !is_pow2(s.m) || !is_pow2(s.n) || s.n < s.m || s.n > (1ull << 32)) return false;
+    p.M2 = std::min<uint64_t>(s.m, 1024);
+    p.M1 = s.m / p.M2;
+    if (p.M1 > 1024) return false;
+    p.E = s.n / s.m;
+    p.L = p.E * p.M1;
+    p.TA = (uint32_t)std::min<uint64_t>(p.M2, THREADS / split_r2(p.M1));
+    uint64_t DB = std::min<uint64_t>(THREADS / split_r2(p.M2), s.C * p.M1);
+    if (s.row_major) {
+        p.CB = (uint32_t)pow2_div(s.C, DB);
+        p.TB = (uint32_t)std::min<uint64_t>(DB / p.CB, p.M1);
+    } else {
+        p.TB = (uint32_t)std::min<uint64_t>(DB, p.M1);
+        p.CB = (uint32_t)pow2_div(s.C, DB / p.TB);
+    }
+    uint64_t per = s.m * s.C;
+    p.G = std::max<uint64_t>(1, std::min<uint64_t>({p.E, SCRATCH_TARGET / per, 32768}));
+    return true;
+}
+
+}  // namespace
+
+size_t lde_scratch_elems(const LdeShape& s) {
+    Plan p;
+    if (!make_plan(s, p)) return 0;
+    return (size_t)(p.G * s.m * s.C);
+}
+
+int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* scratch,
+               const Twiddles& tw, hipStream_t st) {
+    Plan p;
+    if (!make_plan(s, p)) return 1;
+    if (tw.log_n != (uint32_t)ilog2u(s.n)) return 2;
+    LdeArgs a;
+    a.in = in;
+    a.out = out;
+    a.z = scratch;
+    a.tw_lo = tw.lo;
+    a.tw_hi = tw.hi;
+    a.t1024 = tw.t1024;
+    a.C = s.C;
+    a.m = s.m;
+    a.n = s.n;
+    a.M1 = p.M1;
+    a.M2 = p.M2;
+    a.E = p.E;
+    a.L = p.L;
+    a.zstride = s.m * s.C;
+    a.in_el = s.in_el;
+    a.in_comp = s.in_comp;
+    a.out_el = s.out_el;
+    a.out_comp = s.out_comp;
+    a.TA = p.TA;
+    a.TB = p.TB;
+    a.CB = p.CB;
+    a.row_major = s.row_major ? 1 : 0;
+
+    KernelFn ka = s.inverse ? pick_a<true>(p.M1) : pick_a<false>(p.M1);
+    KernelFn kb = s.inverse ? pick_b<true>(p.M2) : pick_b<false>(p.M2);
+    size_t lds_a = (size_t)p.TA * padded(p.M1) * 8;
+    size_t lds_b = (size_t)p.TB * p.CB * padded(p.M2) * 8;
+    static bool attr_done[2][2][11] = {};
+    int ia = ilog2u(p.M1), ib = ilog2u(p.M2), inv = s.inverse ? 1 : 0;
+    if (!attr_done[inv][0][ia]) {
+        (void)hipFuncSetAttribute((const void*)ka, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr_done[inv][0][ia] = true;
+    }
+    if (!attr_done[inv][1][ib]) {
+        (void)hipFuncSetAttribute((const void*)kb, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr_done[inv][1][ib] = true;
+    }
+    dim3 ga((unsigned)(s.C * (p.M2 / p.TA)), 1, 1);
+    dim3 gb((unsigned)((p.M1 / p.TB) * (s.C / p.CB)), 1, 1);
+    for (uint64_t s0 = 0; s0 < p.E; s0 += p.G) {
+        uint64_t g = std::min<uint64_t>(p.G, p.E - s0);
+        ga.y = (unsigned)g;
+        gb.y = (unsigned)g;
+        hipLaunchKernelGGL(ka, ga, dim3(THREADS), lds_a, st, a, s0);
+        hipLaunchKernelGGL(kb, gb, dim3(THREADS), lds_b, st, a, s0);
+    }
+    return 0;
+}
+
+}  // namespace whir_gpu

@@ -1,0 +1,187 @@
+// merkle.hip — codeword row leaves and the binary Merkle tree, BLAKE3.
+//
+// Layout follows the reference's witness (protocols/merkle_tree.hpp:104-147):
+// nodes = [leaves padded to 2^L with all-zero hashes | level L-1 | ... | root],
+// 32 bytes per node, parent = BLAKE3(left || right) over 64 bytes.  One
+// node (or leaf) per lane; levels wider than TOP_WIDTH are one launch each,
+// the narrow top of the tree is finished by one workgroup.
+#include "blake3_dev.h"
+#include "kernels.h"
+
+namespace whir_gpu {
+namespace {
+
+constexpr uint64_t TOP_WIDTH = 2048;  // levels at most this wide run in one workgroup
+
+__device__ __forceinline__ void store_hash(uint8_t* dst, const uint32_t* h) {
+    uint4* d = reinterpret_cast<uint4*>(dst);
+    d[0] = make_uint4(h[0], h[1], h[2], h[3]);
+    d[1] = make_uint4(h[4], h[5], h[6], h[7]);
+}
+
+__device__ __forceinline__ void load64(const uint8_t* src, uint32_t* m) {
+    const uint4* s = reinterpret_cast<const uint4*>(src);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        uint4 v = s[i];
+        m[4 * i] = v.x;
+        m[4 * i + 1] = v.y;
+        m[4 * i + 2] = v.z;
+        m[4 * i + 3] = v.w;
+    }
+}
+
+// leaves[r] = BLAKE3(LE u64 words of row r) — matrix_commit.hpp:169-195 with
+// encode_into (:73-136): each canonical limb is written as 8 LE bytes, which
+// is the native in-memory layout, so the row bytes are the row itself.
+__global__ __launch_bounds__(256) void leaf_hash_kernel(const uint64_t* __restrict__ mat, uint64_t rows,
+                                                        uint32_t row_u64, uint8_t* __restrict__ leaves) {
+    uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const uint64_t* row = mat + r * row_u64;
+    const uint64_t len = (uint64_t)row_u64 * 8;
+    uint32_t out[8];
+    if (len <= 1024) {
+        uint32_t cv[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) cv[i] = b3::iv(i);
+        uint32_t nblocks = len == 0 ? 1 : (uint32_t)((len + 63) / 64);
+        for (uint32_t b = 0; b < nblocks; ++b) {
+            uint32_t m[16];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                uint32_t w = b * 8 + i;
+                uint64_t x = w < row_u64 ? row[w] : 0;
+                m[2 * i] = (uint32_t)x;
+                m[2 * i + 1] = (uint32_t)(x >> 32);
+            }
+            uint32_t bl = (uint32_t)((len - 64 * b) < 64 ? (len - 64 * b) : 64);
+            uint32_t flags = (b == 0 ? b3::CHUNK_START : 0);
+            if (b + 1 == nblocks) flags |= b3::CHUNK_END | b3::ROOT;
+            b3::compress(cv, m, 0, bl, flags, cv);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) out[i] = cv[i];
+    } else {
+        b3::hash_any(
+            [&](uint64_t w) -> uint32_t {
+                uint64_t q = w >> 1;
+                if (q >= row_u64) return 0;
+                uint64_t x = row[q];
+                return (w & 1) ? (uint32_t)(x >> 32) : (uint32_t)x;
+            },
+            len, out);
+    }
+    store_hash(leaves + 32 * r, out);
+}
+
+// out[i] = BLAKE3(in[i*size .. (i+1)*size]) for any byte size (hash_many,
+// hash/blake3_engine.hpp:90-199; both of its branches compute standard BLAKE3)
+__global__ __launch_bounds__(256) void hash_many_kernel(const uint8_t* __restrict__ in, uint64_t size,
+                                                        uint64_t count, uint8_t* __restrict__ out) {
+    uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= count) return;
+    const uint8_t* msg = in + r * size;
+    uint32_t h[8];
+    b3::hash_any(
+        [&](uint64_t w) -> uint32_t {
+            uint32_t v = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                uint64_t b = 4 * w + k;
+                if (b < size) v |= (uint32_t)msg[b] << (8 * k);
+            }
+            return v;
+        },
+        size, h);
+    store_hash(out + 32 * r, h);
+}
+
+__global__ __launch_bounds__(256) void merkle_level_kernel(uint8_t* __restrict__ nodes, uint64_t child_off,
+                                                           uint64_t parent_off, uint64_t width) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= width) return;
+    uint32_t m[16], h[8];
+    load64(nodes + 32 * (child_off + 2 * i), m);
+    b3::hash64(m, h);
+    store_hash(nodes + 32 * (parent_off + i), h);
+}
+
+// The top of the tree: levels of width <= TOP_WIDTH/2 parents, one workgroup.
+__global__ __launch_bounds__(1024) void merkle_top_kernel(uint8_t* __restrict__ nodes, uint64_t child_off,
+                                                          uint64_t child_width) {
+    while (child_width > 1) {
+        uint64_t width = child_width / 2, parent_off = child_off + child_width;
+        for (uint64_t i = threadIdx.x; i < width; i += blockDim.x) {
+            uint32_t m[16], h[8];
+            load64(nodes + 32 * (child_off + 2 * i), m);
+            b3::hash64(m, h);
+            store_hash(nodes + 32 * (parent_off + i), h);
+        }
+        __syncthreads();
+        child_off = parent_off;
+        child_width = width;
+    }
+}
+
+__global__ __launch_bounds__(256) void gather_hashes_kernel(const uint8_t* __restrict__ nodes,
+                                                            const uint64_t* __restrict__ idx, uint64_t count,
+                                                            uint8_t* __restrict__ out) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const uint4* s = reinterpret_cast<const uint4*>(nodes + 32 * idx[i]);
+    uint4* d = reinterpret_cast<uint4*>(out + 32 * i);
+    d[0] = s[0];
+    d[1] = s[1];
+}
+
+__global__ __launch_bounds__(256) void fill_zero_kernel(uint4* p, uint64_t n16) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n16) p[i] = make_uint4(0, 0, 0, 0);
+}
+
+unsigned blocks_for(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+
+void leaf_hash_launch(const uint64_t* matrix, uint64_t rows, uint32_t row_u64, uint8_t* leaves, hipStream_t st) {
+    if (!rows) return;
+    hipLaunchKernelGGL(leaf_hash_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, st, matrix, rows, row_u64, leaves);
+}
+
+void hash_many_launch(const uint8_t* in, uint64_t size, uint64_t count, uint8_t* out, hipStream_t st) {
+    if (!count) return;
+    hipLaunchKernelGGL(hash_many_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, st, in, size, count, out);
+}
+
+void fill_zero_launch(uint8_t* p, uint64_t bytes, hipStream_t st) {
+    uint64_t n16 = bytes / 16;
+    if (!n16) return;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(blocks_for(n16, 256)), dim3(256), 0, st, reinterpret_cast<uint4*>(p), n16);
+}
+
+void merkle_build_launch(uint8_t* nodes, uint64_t n_leaves, hipStream_t st) {
+    uint64_t width = 1;
+    while (width < n_leaves) width <<= 1;
+    if (n_leaves < width) fill_zero_launch(nodes + 32 * n_leaves, 32 * (width - n_leaves), st);
+    if (n_leaves == 0) {
+        fill_zero_launch(nodes, 32, st);  // empty tree: root is the zero hash
+        return;
+    }
+    uint64_t off = 0;
+    while (width > TOP_WIDTH) {
+        uint64_t parents = width / 2;
+        hipLaunchKernelGGL(merkle_level_kernel, dim3(blocks_for(parents, 256)), dim3(256), 0, st, nodes, off,
+                           off + width, parents);
+        off += width;
+        width = parents;
+    }
+    if (width > 1) hipLaunchKernelGGL(merkle_top_kernel, dim3(1), dim3(1024), 0, st, nodes, off, width);
+}
+
+void gather_hashes_launch(const uint8_t* nodes, const uint64_t* idx, uint64_t count, uint8_t* out, hipStream_t st) {
+    if (!count) return;
+    hipLaunchKernelGGL(gather_hashes_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, st, nodes, idx, count, out);
+}
+
+}  // namespace whir_gpu

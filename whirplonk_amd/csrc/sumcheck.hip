@@ -1,0 +1,186 @@
+// sumcheck.hip — sumcheck round polynomial, fold, and Horner evaluation over
+// the Goldilocks cubic extension.
+//
+// compute (algebra/sumcheck.hpp:43-98), half-split over n:
+//   c0 = sum_{i<n/2} a_i b_i,   c2 = sum_{i<n/2} (a_{i+n/2} - a_i)(b_{i+n/2} - b_i)
+// fold (sumcheck.hpp:103-116):  v'_i = v_i + (v_{i+n/2} - v_i) * r
+// horner (utilities.hpp:195-246, mixed_univariate_evaluate): sum_i f_i z^i
+// Field sums are exact, so any reduction order gives the reference's bits.
+// Reductions: wave64 shuffles, then LDS across the workgroup's waves, then a
+// one-workgroup pass over the per-workgroup partials.
+#include "gl.h"
+#include "kernels.h"
+
+namespace whir_gpu {
+namespace {
+
+using gl::e3;
+
+__device__ __forceinline__ e3 load_e3(const uint64_t* p, uint64_t i) { return {p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+__device__ __forceinline__ void store_e3(uint64_t* p, uint64_t i, e3 v) {
+    p[3 * i] = v.c0;
+    p[3 * i + 1] = v.c1;
+    p[3 * i + 2] = v.c2;
+}
+
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+    uint32_t lo = __shfl_xor((uint32_t)v, m, 64);
+    uint32_t hi = __shfl_xor((uint32_t)(v >> 32), m, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ e3 wave_sum(e3 v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+        v.c0 = gl::add(v.c0, shfl_xor64(v.c0, m));
+        v.c1 = gl::add(v.c1, shfl_xor64(v.c1, m));
+        v.c2 = gl::add(v.c2, shfl_xor64(v.c2, m));
+    }
+    return v;
+}
+
+// Sum of NV Ext3 values per thread over the workgroup; thread 0 gets the totals.
+template <int NV>
+__device__ __forceinline__ void block_sum(e3* v) {
+    __shared__ e3 sh[NV][16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) sh[k][wave] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            e3 s = {0, 0, 0};
+            for (int w = 0; w < nw; ++w) s = gl::e3_add(s, sh[k][w]);
+            v[k] = s;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void sumcheck_compute_kernel(const uint64_t* __restrict__ a, int a_base,
+                                                               const uint64_t* __restrict__ b, uint64_t half,
+                                                               uint64_t* __restrict__ partials) {
+    e3 acc[2] = {{0, 0, 0}, {0, 0, 0}};
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < half; i += stride) {
+        e3 b0 = load_e3(b, i), b1 = load_e3(b, i + half);
+        e3 db = gl::e3_sub(b1, b0);
+        if (a_base) {
+            uint64_t a0 = a[i], a1 = a[i + half];
+            acc[0] = gl::e3_add(acc[0], gl::e3_mul_base(b0, a0));
+            acc[1] = gl::e3_add(acc[1], gl::e3_mul_base(db, gl::sub(a1, a0)));
+        } else {
+            e3 a0 = load_e3(a, i), a1 = load_e3(a, i + half);
+            acc[0] = gl::e3_add(acc[0], gl::e3_mul(a0, b0));
+            acc[1] = gl::e3_add(acc[1], gl::e3_mul(gl::e3_sub(a1, a0), db));
+        }
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        store_e3(partials, 2 * blockIdx.x, acc[0]);
+        store_e3(partials, 2 * blockIdx.x + 1, acc[1]);
+    }
+}
+
+// partials[2*blk], partials[2*blk+1] for blk < nblocks -> partials[0], partials[1]
+__global__ __launch_bounds__(256) void reduce_pairs_kernel(uint64_t* partials, uint32_t nblocks) {
+    e3 acc[2] = {{0, 0, 0}, {0, 0, 0}};
+    for (uint32_t k = threadIdx.x; k < nblocks; k += blockDim.x) {
+        acc[0] = gl::e3_add(acc[0], load_e3(partials, 2 * k));
+        acc[1] = gl::e3_add(acc[1], load_e3(partials, 2 * k + 1));
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        store_e3(partials, 0, acc[0]);
+        store_e3(partials, 1, acc[1]);
+    }
+}
+
+__global__ __launch_bounds__(256) void sumcheck_fold_kernel(const uint64_t* __restrict__ in, int in_base,
+                                                            uint64_t half, e3 r, uint64_t* __restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < half; i += stride) {
+        e3 v;
+        if (in_base) {
+            uint64_t lo = in[i], hi = in[i + half];
+            v = gl::e3_mul_base(r, gl::sub(hi, lo));
+            v.c0 = gl::add(v.c0, lo);
+        } else {
+            e3 lo = load_e3(in, i), hi = load_e3(in, i + half);
+            v = gl::e3_add(lo, gl::e3_mul(gl::e3_sub(hi, lo), r));
+        }
+        store_e3(out, i, v);
+    }
+}
+
+// Thread t of T = 2^logT threads evaluates the interleaved sub-polynomial
+// sum_q f[t + T q] Z^q (Z = z^T) by Horner, scales by z^t, and the grid sums.
+__global__ __launch_bounds__(256) void horner_kernel(const uint64_t* __restrict__ f, int f_ext3, uint64_t n,
+                                                     const uint64_t* __restrict__ zpow, uint32_t logT,
+                                                     uint64_t* __restrict__ partials) {
+    const uint64_t T = 1ull << logT;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    e3 acc[1] = {{0, 0, 0}};
+    if (t < n) {
+        const e3 Z = load_e3(zpow, logT);
+        uint64_t nq = (n - t + T - 1) / T;
+        for (uint64_t q = nq; q-- > 0;) {
+            uint64_t i = t + T * q;
+            acc[0] = gl::e3_mul(acc[0], Z);
+            if (f_ext3) {
+                acc[0] = gl::e3_add(acc[0], load_e3(f, i));
+            } else {
+                acc[0].c0 = gl::add(acc[0].c0, f[i]);
+            }
+        }
+        for (uint32_t l = 0; l < logT; ++l)
+            if ((t >> l) & 1) acc[0] = gl::e3_mul(acc[0], load_e3(zpow, l));
+    }
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) store_e3(partials, blockIdx.x, acc[0]);
+}
+
+__global__ __launch_bounds__(256) void reduce_sum_kernel(const uint64_t* partials, uint32_t nblocks, uint64_t* out) {
+    e3 acc[1] = {{0, 0, 0}};
+    for (uint32_t k = threadIdx.x; k < nblocks; k += blockDim.x) acc[0] = gl::e3_add(acc[0], load_e3(partials, k));
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) store_e3(out, 0, acc[0]);
+}
+
+unsigned grid_for(uint64_t work, unsigned cap) {
+    uint64_t b = (work + 255) / 256;
+    if (b < 1) b = 1;
+    return (unsigned)(b < cap ? b : cap);
+}
+
+}  // namespace
+
+void sumcheck_compute_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint64_t n, uint64_t* partials,
+                             hipStream_t st) {
+    unsigned g = grid_for(n / 2, SUMCHECK_MAX_BLOCKS);
+    hipLaunchKernelGGL(sumcheck_compute_kernel, dim3(g), dim3(256), 0, st, a, a_base ? 1 : 0, b, n / 2, partials);
+    hipLaunchKernelGGL(reduce_pairs_kernel, dim3(1), dim3(256), 0, st, partials, g);
+}
+
+void sumcheck_fold_launch(const uint64_t* in, bool in_base, uint64_t n, const uint64_t r[3], uint64_t* out,
+                          hipStream_t st) {
+    unsigned g = grid_for(n / 2, 8192);
+    e3 rr = {r[0], r[1], r[2]};
+    hipLaunchKernelGGL(sumcheck_fold_kernel, dim3(g), dim3(256), 0, st, in, in_base ? 1 : 0, n / 2, rr, out);
+}
+
+void horner_launch(const uint64_t* f, bool f_ext3, uint64_t n, const uint64_t* zpow, uint64_t* partials,
+                   uint64_t* out, hipStream_t st) {
+    // T = blocks*256 threads, a power of two, about 16+ terms per thread
+    uint32_t logT = 8;
+    while (logT < 18 && (1ull << (logT + 4)) < n) ++logT;
+    unsigned blocks = (unsigned)((1ull << logT) / 256);
+    hipLaunchKernelGGL(horner_kernel, dim3(blocks), dim3(256), 0, st, f, f_ext3 ? 1 : 0, n, zpow, logT, partials);
+    hipLaunchKernelGGL(reduce_sum_kernel, dim3(1), dim3(256), 0, st, partials, blocks, out);
+}
+
+}  // namespace whir_gpu

@@ -1,0 +1,737 @@
+// whir_gpu.cpp — the C ABI (include/whir_gpu.h) over the gfx950 kernels.
+//
+// Host orchestration only: contexts, grow-only device pools, twiddle tables,
+// stream ordering and the host-pointer drop-ins.  There is no CPU compute
+// path here: when the device cannot run a request the call returns a status
+// > 0 ("not handled") or < 0 (device error), never a host-computed result.
+#include "../../include/whir_gpu.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gl.h"
+#include "kernels.h"
+
+using namespace whir_gpu;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const char* what, hipError_t e) {
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return -1;
+}
+
+#define HIPCHK(expr)                                   \
+    do {                                               \
+        hipError_t e_ = (expr);                        \
+        if (e_ != hipSuccess) return fail(#expr, e_);  \
+    } while (0)
+
+int bad_arg(const char* what) {
+    g_err = what;
+    return 3;
+}
+
+bool env_disabled() {
+    for (const char* name : {"WHIR_GPU_DISABLE", "WHIR_CUDA_DISABLE"}) {
+        const char* v = std::getenv(name);
+        if (v && v[0] && std::strcmp(v, "0") != 0) return true;
+    }
+    return false;
+}
+
+std::mutex g_flags_mu;
+bool g_dispatch = !env_disabled();
+size_t g_threshold = 65536;
+
+bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
+uint32_t log2u(uint64_t x) {
+    uint32_t k = 0;
+    while ((1ull << k) < x) ++k;
+    return k;
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct whir_ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    std::recursive_mutex mu;
+    DevBuf scratch, partials, zpow, hin, hout, hnodes, hidx, hcomp;
+    std::map<std::pair<uint32_t, bool>, Twiddles> tw;
+    uint64_t* t1024[2] = {nullptr, nullptr};
+
+    int ensure(DevBuf& b, size_t bytes) {
+        if (b.bytes >= bytes && b.p) return 0;
+        if (b.p) {
+            HIPCHK(hipStreamSynchronize(stream));
+            HIPCHK(hipFree(b.p));
+            b.p = nullptr;
+            b.bytes = 0;
+        }
+        size_t want = std::max<size_t>(bytes, 256);
+        HIPCHK(hipMalloc(&b.p, want));
+        b.bytes = want;
+        return 0;
+    }
+
+    int twiddles(uint32_t log_n, bool inverse, Twiddles& out) {
+        auto key = std::make_pair(log_n, inverse);
+        auto it = tw.find(key);
+        if (it != tw.end()) {
+            out = it->second;
+            return 0;
+        }
+        int inv = inverse ? 1 : 0;
+        if (!t1024[inv]) {
+            uint64_t w = gl::root_of_unity(10);
+            if (inverse) w = gl::inv(w);
+            std::vector<uint64_t> h(1024);
+            uint64_t acc = 1;
+            for (int i = 0; i < 1024; ++i) {
+                h[i] = acc;
+                acc = gl::mul(acc, w);
+            }
+            HIPCHK(hipMalloc(&t1024[inv], 1024 * 8));
+            HIPCHK(hipMemcpy(t1024[inv], h.data(), 1024 * 8, hipMemcpyHostToDevice));
+        }
+        uint64_t n = 1ull << log_n;
+        uint64_t w = gl::root_of_unity(log_n);
+        if (inverse) w = gl::inv(w);
+        std::vector<uint64_t> lo(2048), hi(std::max<uint64_t>(1, n / 2048));
+        uint64_t acc = 1;
+        for (int i = 0; i < 2048; ++i) {
+            lo[i] = acc;
+            acc = gl::mul(acc, w);
+        }
+        uint64_t w2048 = gl::pow(w, 2048);
+        acc = 1;
+        for (size_t i = 0; i < hi.size(); ++i) {
+            hi[i] = acc;
+            acc = gl::mul(acc, w2048);
+        }
+        Twiddles t;
+        t.log_n = log_n;
+        HIPCHK(hipMalloc(&t.lo, lo.size() * 8));
+        HIPCHK(hipMalloc(&t.hi, hi.size() * 8));
+        HIPCHK(hipMemcpy(t.lo, lo.data(), lo.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(t.hi, hi.data(), hi.size() * 8, hipMemcpyHostToDevice));
+        t.t1024 = t1024[inv];
+        tw[key] = t;
+        out = t;
+        return 0;
+    }
+};
+
+namespace {
+
+std::mutex g_default_mu;
+whir_ctx* g_default = nullptr;
+
+int default_ctx(whir_ctx** out) {
+    std::lock_guard<std::mutex> g(g_default_mu);
+    if (!g_default) {
+        int dev = 0;
+        HIPCHK(hipGetDevice(&dev));
+        int rc = whir_ctx_create(dev, &g_default);
+        if (rc) return rc;
+    }
+    *out = g_default;
+    return 0;
+}
+
+bool dispatch_on() {
+    std::lock_guard<std::mutex> g(g_flags_mu);
+    return g_dispatch;
+}
+
+// RS-encode geometry checks shared by the entry points
+int rs_shape(size_t num_polys, size_t poly_size, size_t cw, size_t depth, LdeShape& s) {
+    if (num_polys == 0 || poly_size == 0 || depth == 0 || cw == 0) return bad_arg("empty RS encode shape");
+    if (poly_size % depth) return bad_arg("poly_size must be a multiple of interleaving_depth");
+    s.m = poly_size / depth;
+    s.C = (uint64_t)num_polys * depth;
+    s.n = cw;
+    if (cw % s.m) return bad_arg("codeword_length must be a multiple of the message length");
+    if (!is_pow2(s.m) || !is_pow2(cw)) return bad_arg("message and codeword lengths must be powers of two");
+    if (cw > (1ull << 26)) return bad_arg("codeword_length above 2^26 is not supported");
+    if (s.m > (1ull << 20)) return bad_arg("message length above 2^20 is not supported");
+    s.row_major = true;
+    return 0;
+}
+
+int run_lde(whir_ctx* ctx, LdeShape s, const uint64_t* in, uint64_t* out) {
+    Twiddles tw;
+    int rc = ctx->twiddles(log2u(s.n), s.inverse, tw);
+    if (rc) return rc;
+    rc = ctx->ensure(ctx->scratch, lde_scratch_elems(s) * 8);
+    if (rc) return rc;
+    rc = lde_launch(s, in, out, (uint64_t*)ctx->scratch.p, tw, ctx->stream);
+    if (rc > 0) return bad_arg("LDE shape not supported");
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int rs_encode_any(whir_ctx* ctx, const uint64_t* d_in, size_t num_polys, size_t poly_size, size_t cw, size_t depth,
+                  uint64_t* d_out, int ext) {
+    LdeShape s;
+    int rc = rs_shape(num_polys, poly_size, cw, depth, s);
+    if (rc) return rc;
+    if (!ext) return run_lde(ctx, s, d_in, d_out);
+    s.in_el = 3;
+    s.out_el = 3;
+    for (uint32_t c = 0; c < 3; ++c) {
+        s.in_comp = c;
+        s.out_comp = c;
+        rc = run_lde(ctx, s, d_in, d_out);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+template <typename F>
+int with_default(F f) {
+    if (!dispatch_on()) {
+        g_err = "GPU dispatch disabled";
+        return 1;
+    }
+    whir_ctx* ctx = nullptr;
+    int rc = default_ctx(&ctx);
+    if (rc) return rc;
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return f(ctx);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* whir_last_error(void) { return g_err.c_str(); }
+
+int whir_device_available(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return 0;
+    return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
+}
+
+int whir_gpu_dispatch_enabled(void) { return dispatch_on() ? 1 : 0; }
+void whir_set_gpu_dispatch_enabled(int enabled) {
+    std::lock_guard<std::mutex> g(g_flags_mu);
+    g_dispatch = enabled != 0;
+}
+size_t whir_gpu_ntt_threshold(void) {
+    std::lock_guard<std::mutex> g(g_flags_mu);
+    return g_threshold;
+}
+void whir_set_gpu_ntt_threshold(size_t t) {
+    std::lock_guard<std::mutex> g(g_flags_mu);
+    g_threshold = t;
+}
+
+int whir_gpu_warmup(void) {
+    return with_default([](whir_ctx* ctx) {
+        Twiddles tw;
+        int rc = ctx->twiddles(10, false, tw);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        return 0;
+    });
+}
+
+int whir_ctx_create(int device, whir_ctx** out) {
+    if (!out) return bad_arg("null output");
+    HIPCHK(hipSetDevice(device));
+    whir_ctx* c = new whir_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail("hipStreamCreateWithFlags", e);
+    }
+    c->stream = c->own;
+    *out = c;
+    return 0;
+}
+
+int whir_ctx_destroy(whir_ctx* ctx) {
+    if (!ctx) return 0;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (DevBuf* b : {&ctx->scratch, &ctx->partials, &ctx->zpow, &ctx->hin, &ctx->hout, &ctx->hnodes, &ctx->hidx,
+                      &ctx->hcomp})
+        if (b->p) (void)hipFree(b->p);
+    for (auto& kv : ctx->tw) {
+        (void)hipFree(kv.second.lo);
+        (void)hipFree(kv.second.hi);
+    }
+    for (auto* p : ctx->t1024)
+        if (p) (void)hipFree(p);
+    if (ctx->own) (void)hipStreamDestroy(ctx->own);
+    if (ctx == g_default) g_default = nullptr;
+    delete ctx;
+    return 0;
+}
+
+int whir_ctx_set_stream(whir_ctx* ctx, void* s) {
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    ctx->stream = s ? (hipStream_t)s : ctx->own;
+    return 0;
+}
+void* whir_ctx_stream(whir_ctx* ctx) { return (void*)ctx->stream; }
+int whir_ctx_synchronize(whir_ctx* ctx) {
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int whir_dev_malloc(whir_ctx* ctx, size_t bytes, void** out) {
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipMalloc(out, std::max<size_t>(bytes, 1)));
+    return 0;
+}
+int whir_dev_free(whir_ctx* ctx, void* p) {
+    (void)ctx;
+    if (p) HIPCHK(hipFree(p));
+    return 0;
+}
+int whir_memcpy_h2d(whir_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!bytes) return 0;
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+int whir_memcpy_d2h(whir_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!bytes) return 0;
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// ------------------------------------------------------------ device forms
+
+int whir_dev_rs_encode(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_polys, size_t poly_size, size_t cw,
+                       size_t depth, uint64_t* d_out) {
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return rs_encode_any(ctx, d_coeffs, num_polys, poly_size, cw, depth, d_out, 0);
+}
+
+int whir_dev_rs_encode_ext3(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_polys, size_t poly_size, size_t cw,
+                            size_t depth, uint64_t* d_out) {
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return rs_encode_any(ctx, d_coeffs, num_polys, poly_size, cw, depth, d_out, 1);
+}
+
+int whir_dev_ntt_batch(whir_ctx* ctx, uint64_t* d_values, size_t total, size_t size, int inverse) {
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (size == 0 || total % size) return bad_arg("total must be a multiple of size");
+    if (total == 0 || size == 1) return 0;
+    if (!is_pow2(size) || size > (1ull << 20)) return bad_arg("NTT size must be a power of two <= 2^20");
+    LdeShape s;
+    s.C = total / size;
+    s.m = size;
+    s.n = size;
+    s.row_major = false;
+    s.inverse = inverse != 0;
+    return run_lde(ctx, s, d_values, d_values);
+}
+
+int whir_dev_commit_leaves(whir_ctx* ctx, const uint64_t* d_matrix, size_t rows, size_t row_u64, uint8_t* d_leaves) {
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    leaf_hash_launch(d_matrix, rows, (uint32_t)row_u64, d_leaves, ctx->stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int whir_dev_blake3_hash_many(whir_ctx* ctx, size_t size, const uint8_t* d_in, size_t count, uint8_t* d_out) {
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    hash_many_launch(d_in, size, count, d_out, ctx->stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+size_t whir_merkle_num_nodes(size_t n) {
+    size_t w = 1;
+    while (w < n) w <<= 1;
+    return 2 * w - 1;
+}
+
+int whir_dev_merkle_build(whir_ctx* ctx, uint8_t* d_nodes, size_t num_leaves) {
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    merkle_build_launch(d_nodes, num_leaves, ctx->stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int whir_dev_gather_hashes(whir_ctx* ctx, const uint8_t* d_nodes, const uint64_t* d_idx, size_t count,
+                           uint8_t* d_out) {
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    gather_hashes_launch(d_nodes, d_idx, count, d_out, ctx->stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+static int commit_any(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_polys, size_t poly_size, size_t cw,
+                      size_t depth, uint64_t* d_matrix, uint8_t* d_nodes, uint8_t* out_root, int ext) {
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    int rc = rs_encode_any(ctx, d_coeffs, num_polys, poly_size, cw, depth, d_matrix, ext);
+    if (rc) return rc;
+    size_t row_u64 = num_polys * depth * (ext ? 3 : 1);
+    leaf_hash_launch(d_matrix, cw, (uint32_t)row_u64, d_nodes, ctx->stream);
+    merkle_build_launch(d_nodes, cw, ctx->stream);
+    HIPCHK(hipGetLastError());
+    if (out_root) {
+        size_t nn = whir_merkle_num_nodes(cw);
+        HIPCHK(hipMemcpyAsync(out_root, d_nodes + 32 * (nn - 1), 32, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    return 0;
+}
+
+int whir_dev_commit_base(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_polys, size_t poly_size, size_t cw,
+                         size_t depth, uint64_t* d_matrix, uint8_t* d_nodes, uint8_t* out_root) {
+    return commit_any(ctx, d_coeffs, num_polys, poly_size, cw, depth, d_matrix, d_nodes, out_root, 0);
+}
+
+int whir_dev_commit_ext3(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_polys, size_t poly_size, size_t cw,
+                         size_t depth, uint64_t* d_matrix, uint8_t* d_nodes, uint8_t* out_root) {
+    return commit_any(ctx, d_coeffs, num_polys, poly_size, cw, depth, d_matrix, d_nodes, out_root, 1);
+}
+
+int whir_dev_sumcheck_compute(whir_ctx* ctx, const uint64_t* d_a, int a_is_base, const uint64_t* d_b, size_t n,
+                              uint64_t* out_c0, uint64_t* out_c2) {
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (n % 2) return bad_arg("sumcheck length must be even");
+    int rc = ctx->ensure(ctx->partials, 6 * SUMCHECK_MAX_BLOCKS * 8);
+    if (rc) return rc;
+    uint64_t* part = (uint64_t*)ctx->partials.p;
+    sumcheck_compute_launch(d_a, a_is_base != 0, d_b, n, part, ctx->stream);
+    HIPCHK(hipGetLastError());
+    uint64_t h[6];
+    HIPCHK(hipMemcpyAsync(h, part, 48, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    std::memcpy(out_c0, h, 24);
+    std::memcpy(out_c2, h + 3, 24);
+    return 0;
+}
+
+int whir_dev_sumcheck_fold(whir_ctx* ctx, const uint64_t* d_v, int v_is_base, size_t n, const uint64_t* r,
+                           uint64_t* d_out) {
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (n % 2) return bad_arg("fold length must be even");
+    if (v_is_base && (const void*)d_v == (const void*)d_out) return bad_arg("base-field fold cannot run in place");
+    sumcheck_fold_launch(d_v, v_is_base != 0, n, r, d_out, ctx->stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int whir_dev_horner(whir_ctx* ctx, const uint64_t* d_f, int f_is_ext3, size_t n, const uint64_t* z, uint64_t* out) {
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (n == 0) {
+        out[0] = out[1] = out[2] = 0;
+        return 0;
+    }
+    int rc = ctx->ensure(ctx->zpow, 192 * 8 + 3 * 8);
+    if (rc) return rc;
+    rc = ctx->ensure(ctx->partials, 6 * SUMCHECK_MAX_BLOCKS * 8);
+    if (rc) return rc;
+    uint64_t h[192];
+    gl::e3 p = {z[0], z[1], z[2]};
+    for (int l = 0; l < 64; ++l) {
+        h[3 * l] = p.c0;
+        h[3 * l + 1] = p.c1;
+        h[3 * l + 2] = p.c2;
+        p = gl::e3_mul(p, p);
+    }
+    uint64_t* dz = (uint64_t*)ctx->zpow.p;
+    HIPCHK(hipMemcpyAsync(dz, h, sizeof h, hipMemcpyHostToDevice, ctx->stream));
+    horner_launch(d_f, f_is_ext3 != 0, n, dz, (uint64_t*)ctx->partials.p, dz + 192, ctx->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, dz + 192, 24, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+// ------------------------------------------------------ host-pointer drop-ins
+
+static int ntt_host(uint64_t* values, size_t total, size_t size, int inverse) {
+    {
+        std::lock_guard<std::mutex> g(g_flags_mu);
+        if (total < g_threshold) {
+            g_err = "below the GPU NTT threshold";
+            return 1;
+        }
+    }
+    return with_default([&](whir_ctx* ctx) {
+        int rc = ctx->ensure(ctx->hin, total * 8);
+        if (rc) return rc;
+        uint64_t* d = (uint64_t*)ctx->hin.p;
+        HIPCHK(hipMemcpyAsync(d, values, total * 8, hipMemcpyHostToDevice, ctx->stream));
+        rc = whir_dev_ntt_batch(ctx, d, total, size, inverse);
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(values, d, total * 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        return 0;
+    });
+}
+
+int whir_gpu_ntt_batch(uint64_t* values, size_t total, size_t size) { return ntt_host(values, total, size, 0); }
+int whir_gpu_intt_batch(uint64_t* values, size_t total, size_t size) { return ntt_host(values, total, size, 1); }
+
+static int rs_host(const uint64_t* coeffs, uint64_t* out_matrix, uint8_t* out_hashes, size_t num_polys,
+                   size_t poly_size, size_t cw, size_t depth) {
+    return with_default([&](whir_ctx* ctx) {
+        LdeShape s;
+        int rc = rs_shape(num_polys, poly_size, cw, depth, s);
+        if (rc) return rc;
+        size_t nin = num_polys * poly_size, nout = cw * num_polys * depth;
+        if ((rc = ctx->ensure(ctx->hin, nin * 8))) return rc;
+        if ((rc = ctx->ensure(ctx->hout, nout * 8))) return rc;
+        if (out_hashes && (rc = ctx->ensure(ctx->hnodes, cw * 32))) return rc;
+        uint64_t* din = (uint64_t*)ctx->hin.p;
+        uint64_t* dout = (uint64_t*)ctx->hout.p;
+        HIPCHK(hipMemcpyAsync(din, coeffs, nin * 8, hipMemcpyHostToDevice, ctx->stream));
+        if ((rc = run_lde(ctx, s, din, dout))) return rc;
+        if (out_hashes) leaf_hash_launch(dout, cw, (uint32_t)(num_polys * depth), (uint8_t*)ctx->hnodes.p, ctx->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out_matrix, dout, nout * 8, hipMemcpyDeviceToHost, ctx->stream));
+        if (out_hashes) HIPCHK(hipMemcpyAsync(out_hashes, ctx->hnodes.p, cw * 32, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        return 0;
+    });
+}
+
+int whir_gpu_interleaved_rs_encode(const uint64_t* coeffs, uint64_t* out, size_t num_polys, size_t poly_size,
+                                   size_t cw, size_t depth) {
+    return rs_host(coeffs, out, nullptr, num_polys, poly_size, cw, depth);
+}
+
+int whir_gpu_interleaved_rs_encode_blake3_matrix_leaves(const uint64_t* coeffs, uint64_t* out_matrix,
+                                                        uint8_t* out_hashes, size_t num_polys, size_t poly_size,
+                                                        size_t cw, size_t depth) {
+    return rs_host(coeffs, out_matrix, out_hashes, num_polys, poly_size, cw, depth);
+}
+
+int whir_gpu_interleaved_rs_encode_blake3_ext3_matrix_leaves(const uint64_t* c0, const uint64_t* c1,
+                                                             const uint64_t* c2, uint64_t* o0, uint64_t* o1,
+                                                             uint64_t* o2, uint8_t* out_hashes, size_t num_polys,
+                                                             size_t poly_size, size_t cw, size_t depth) {
+    return with_default([&](whir_ctx* ctx) {
+        LdeShape s;
+        int rc = rs_shape(num_polys, poly_size, cw, depth, s);
+        if (rc) return rc;
+        size_t nin = num_polys * poly_size, nout = cw * num_polys * depth;
+        if ((rc = ctx->ensure(ctx->hin, 3 * nin * 8))) return rc;
+        if ((rc = ctx->ensure(ctx->hout, 3 * nout * 8))) return rc;
+        if ((rc = ctx->ensure(ctx->hnodes, cw * 32))) return rc;
+        uint64_t* din = (uint64_t*)ctx->hin.p;
+        uint64_t* dout = (uint64_t*)ctx->hout.p;
+        const uint64_t* src[3] = {c0, c1, c2};
+        for (int c = 0; c < 3; ++c)
+            HIPCHK(hipMemcpyAsync(din + c * nin, src[c], nin * 8, hipMemcpyHostToDevice, ctx->stream));
+        s.out_el = 3;
+        for (uint32_t c = 0; c < 3; ++c) {
+            s.out_comp = c;
+            if ((rc = run_lde(ctx, s, din + c * nin, dout))) return rc;
+        }
+        leaf_hash_launch(dout, cw, (uint32_t)(3 * num_polys * depth), (uint8_t*)ctx->hnodes.p, ctx->stream);
+        HIPCHK(hipGetLastError());
+        std::vector<uint64_t> aos(3 * nout);
+        HIPCHK(hipMemcpyAsync(aos.data(), dout, 3 * nout * 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(out_hashes, ctx->hnodes.p, cw * 32, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        for (size_t i = 0; i < nout; ++i) {
+            o0[i] = aos[3 * i];
+            o1[i] = aos[3 * i + 1];
+            o2[i] = aos[3 * i + 2];
+        }
+        return 0;
+    });
+}
+
+int whir_gpu_blake3_merkle_tree(const uint8_t* leaves, size_t n, uint8_t* out_root, uint8_t* out_nodes) {
+    return with_default([&](whir_ctx* ctx) {
+        size_t nn = whir_merkle_num_nodes(n);
+        int rc = ctx->ensure(ctx->hnodes, nn * 32);
+        if (rc) return rc;
+        uint8_t* d = (uint8_t*)ctx->hnodes.p;
+        if (n) HIPCHK(hipMemcpyAsync(d, leaves, n * 32, hipMemcpyHostToDevice, ctx->stream));
+        merkle_build_launch(d, n, ctx->stream);
+        HIPCHK(hipGetLastError());
+        if (out_nodes) HIPCHK(hipMemcpyAsync(out_nodes, d, nn * 32, hipMemcpyDeviceToHost, ctx->stream));
+        if (out_root) HIPCHK(hipMemcpyAsync(out_root, d + 32 * (nn - 1), 32, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        return 0;
+    });
+}
+
+size_t whir_merkle_hint_node_indices(size_t num_leaves, const uint64_t* indices, size_t k, uint64_t* out) {
+    // the sibling order of open_path, merkle_tree.hpp:165-211
+    std::vector<uint64_t> cur(indices, indices + k);
+    std::sort(cur.begin(), cur.end());
+    cur.erase(std::unique(cur.begin(), cur.end()), cur.end());
+    size_t width = 1;
+    while (width < num_leaves) width <<= 1;
+    size_t off = 0, nh = 0;
+    while (width > 1) {
+        std::vector<uint64_t> next;
+        next.reserve(cur.size());
+        for (size_t i = 0; i < cur.size();) {
+            uint64_t a = cur[i];
+            if (i + 1 < cur.size() && cur[i + 1] == (a ^ 1)) {
+                i += 2;
+            } else {
+                if (out) out[nh] = off + (a ^ 1);
+                ++nh;
+                i += 1;
+            }
+            next.push_back(a >> 1);
+        }
+        cur.swap(next);
+        off += width;
+        width >>= 1;
+    }
+    return nh;
+}
+
+int whir_gpu_blake3_merkle_open_path(const uint8_t* leaves, size_t n, const uint64_t* indices, size_t k,
+                                     uint8_t* out_root, uint8_t* out_hints) {
+    for (size_t i = 0; i < k; ++i)
+        if (indices[i] >= n) return bad_arg("opening index out of range");
+    return with_default([&](whir_ctx* ctx) {
+        size_t nn = whir_merkle_num_nodes(n);
+        size_t nh = whir_merkle_hint_node_indices(n, indices, k, nullptr);
+        std::vector<uint64_t> idx(nh);
+        whir_merkle_hint_node_indices(n, indices, k, idx.data());
+        int rc;
+        if ((rc = ctx->ensure(ctx->hnodes, nn * 32))) return rc;
+        if ((rc = ctx->ensure(ctx->hidx, nh * 8))) return rc;
+        if ((rc = ctx->ensure(ctx->hout, nh * 32))) return rc;
+        uint8_t* d = (uint8_t*)ctx->hnodes.p;
+        if (n) HIPCHK(hipMemcpyAsync(d, leaves, n * 32, hipMemcpyHostToDevice, ctx->stream));
+        merkle_build_launch(d, n, ctx->stream);
+        if (nh) {
+            HIPCHK(hipMemcpyAsync(ctx->hidx.p, idx.data(), nh * 8, hipMemcpyHostToDevice, ctx->stream));
+            gather_hashes_launch(d, (uint64_t*)ctx->hidx.p, nh, (uint8_t*)ctx->hout.p, ctx->stream);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out_root, d + 32 * (nn - 1), 32, hipMemcpyDeviceToHost, ctx->stream));
+        if (nh) HIPCHK(hipMemcpyAsync(out_hints, ctx->hout.p, nh * 32, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        return 0;
+    });
+}
+
+static int ood_host(const uint64_t* const* comps, int ext, const uint64_t* points, uint64_t* out, size_t nv,
+                    size_t vs, size_t np) {
+    if (nv == 0 || vs == 0 || np == 0) {
+        g_err = "empty OOD evaluation";
+        return 1;  // the reference returns false here (cuda_integration.hpp:1109-1111)
+    }
+    return with_default([&](whir_ctx* ctx) {
+        size_t cnt = nv * vs;
+        int rc = ctx->ensure(ctx->hin, cnt * 8 * (ext ? 3 : 1));
+        if (rc) return rc;
+        uint64_t* d = (uint64_t*)ctx->hin.p;
+        if (ext) {
+            std::vector<uint64_t> aos(3 * cnt);
+            for (size_t i = 0; i < cnt; ++i)
+                for (int c = 0; c < 3; ++c) aos[3 * i + c] = comps[c][i];
+            HIPCHK(hipMemcpyAsync(d, aos.data(), 3 * cnt * 8, hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+        } else {
+            HIPCHK(hipMemcpyAsync(d, comps[0], cnt * 8, hipMemcpyHostToDevice, ctx->stream));
+        }
+        for (size_t p = 0; p < np; ++p)
+            for (size_t v = 0; v < nv; ++v) {
+                rc = whir_dev_horner(ctx, d + v * vs * (ext ? 3 : 1), ext, vs, points + 3 * p, out + 3 * (p * nv + v));
+                if (rc) return rc;
+            }
+        return 0;
+    });
+}
+
+int whir_gpu_ood_evaluate_ext3_from_base(const uint64_t* coeff0, const uint64_t* points, uint64_t* out, size_t nv,
+                                         size_t vs, size_t np) {
+    const uint64_t* c[3] = {coeff0, nullptr, nullptr};
+    return ood_host(c, 0, points, out, nv, vs, np);
+}
+
+int whir_gpu_ood_evaluate_ext3_from_ext3(const uint64_t* c0, const uint64_t* c1, const uint64_t* c2,
+                                         const uint64_t* points, uint64_t* out, size_t nv, size_t vs, size_t np) {
+    const uint64_t* c[3] = {c0, c1, c2};
+    return ood_host(c, 1, points, out, nv, vs, np);
+}
+
+int whir_gpu_encode_to_bytes(const uint64_t* values, uint8_t* out, size_t count) {
+    // canonical limbs are stored as 8 little-endian bytes on the device already
+    return with_default([&](whir_ctx* ctx) {
+        int rc = ctx->ensure(ctx->hin, count * 8);
+        if (rc) return rc;
+        if (!count) return 0;
+        HIPCHK(hipMemcpyAsync(ctx->hin.p, values, count * 8, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(out, ctx->hin.p, count * 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        return 0;
+    });
+}
+
+int whir_gpu_blake3_hash_many(size_t size, const uint8_t* input, size_t count, uint8_t* output) {
+    return with_default([&](whir_ctx* ctx) {
+        int rc;
+        if ((rc = ctx->ensure(ctx->hin, size * count + 16))) return rc;
+        if ((rc = ctx->ensure(ctx->hout, count * 32))) return rc;
+        if (size * count) HIPCHK(hipMemcpyAsync(ctx->hin.p, input, size * count, hipMemcpyHostToDevice, ctx->stream));
+        hash_many_launch((uint8_t*)ctx->hin.p, size, count, (uint8_t*)ctx->hout.p, ctx->stream);
+        HIPCHK(hipGetLastError());
+        if (count) HIPCHK(hipMemcpyAsync(output, ctx->hout.p, count * 32, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        return 0;
+    });
+}
+
+int whir_gpu_sumcheck_compute(const uint64_t* a, int a_is_base, const uint64_t* b, size_t n, uint64_t* c0,
+                              uint64_t* c2) {
+    return with_default([&](whir_ctx* ctx) {
+        size_t na = n * (a_is_base ? 1 : 3);
+        int rc;
+        if ((rc = ctx->ensure(ctx->hin, na * 8))) return rc;
+        if ((rc = ctx->ensure(ctx->hout, 3 * n * 8))) return rc;
+        HIPCHK(hipMemcpyAsync(ctx->hin.p, a, na * 8, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->hout.p, b, 3 * n * 8, hipMemcpyHostToDevice, ctx->stream));
+        return whir_dev_sumcheck_compute(ctx, (uint64_t*)ctx->hin.p, a_is_base, (uint64_t*)ctx->hout.p, n, c0, c2);
+    });
+}
+
+int whir_gpu_sumcheck_fold(const uint64_t* v, int v_is_base, size_t n, const uint64_t* r, uint64_t* out) {
+    return with_default([&](whir_ctx* ctx) {
+        size_t nv = n * (v_is_base ? 1 : 3);
+        int rc;
+        if ((rc = ctx->ensure(ctx->hin, nv * 8))) return rc;
+        if ((rc = ctx->ensure(ctx->hout, 3 * (n / 2) * 8 + 8))) return rc;
+        HIPCHK(hipMemcpyAsync(ctx->hin.p, v, nv * 8, hipMemcpyHostToDevice, ctx->stream));
+        rc = whir_dev_sumcheck_fold(ctx, (uint64_t*)ctx->hin.p, v_is_base, n, r, (uint64_t*)ctx->hout.p);
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(out, ctx->hout.p, 3 * (n / 2) * 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        return 0;
+    });
+}
+
+}  // extern "C"
