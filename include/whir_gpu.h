@@ -73,6 +73,9 @@ int whir_dev_malloc(whir_ctx* ctx, size_t bytes, void** out);
 int whir_dev_free(whir_ctx* ctx, void* p);
 int whir_memcpy_h2d(whir_ctx* ctx, void* dst, const void* src, size_t bytes);
 int whir_memcpy_d2h(whir_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Device-to-device copy by a streaming kernel on the context's stream
+ * (bytes a multiple of 16, 16-byte aligned pointers); asynchronous. */
+int whir_dev_copy(whir_ctx* ctx, void* dst, const void* src, size_t bytes);
 
 /* ------------------------------------------- host-pointer drop-in functions */
 
@@ -189,6 +192,23 @@ int whir_dev_sumcheck_fold(whir_ctx* ctx, const uint64_t* d_v, int v_is_base, si
                            uint64_t* d_out);
 /* sum_i f_i z^i, f base (f_is_ext3 = 0) or Ext3 AoS; z and out on the host. */
 int whir_dev_horner(whir_ctx* ctx, const uint64_t* d_f, int f_is_ext3, size_t n, const uint64_t* z, uint64_t* out);
+/* One prover round of sumcheck::Config::prove (sumcheck_protocol.hpp:108-140)
+ * with the folding challenge r given by the caller instead of squeezed:
+ * (c0, c2) of (a, b), then a' = fold(a, r) and b' = fold(b, r) into d_a_out,
+ * d_b_out (n/2 Ext3 each).  c0, c2 are copied to the host. */
+int whir_dev_sumcheck_round(whir_ctx* ctx, const uint64_t* d_a, int a_is_base, const uint64_t* d_b, size_t n,
+                            const uint64_t* r, uint64_t* d_a_out, uint64_t* d_b_out, uint64_t* out_c0,
+                            uint64_t* out_c2);
+
+/* --------------------------------------------------------- stage timing */
+
+/* Stage timing with HIP events on the context's stream (the stream the
+ * kernels run on).  Stages: 0 RS encode (LDE), 1 row leaves, 2 Merkle tree,
+ * 3 sumcheck round.  Totals accumulate until reset. */
+int whir_ctx_enable_timing(whir_ctx* ctx, int on);
+int whir_ctx_reset_timing(whir_ctx* ctx);
+/* out_ms[4]: summed milliseconds per stage; out_counts[4]: timed launches. */
+int whir_ctx_stage_times(whir_ctx* ctx, double* out_ms, uint64_t* out_counts);
 
 #ifdef __cplusplus
 }

@@ -136,7 +136,7 @@ def test_leaves_row_lengths(orc):
 
 # ---------------------------------------------------------------- Merkle
 
-@pytest.mark.parametrize("n", [0, 1, 2, 3, 17, 1024, 4097, 65536])
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 17, 1024, 4097, 65536, (1 << 18) + 5])
 def test_merkle_tree(orc, n):
     leaves = np.random.default_rng(n).integers(0, 256, 32 * n, dtype=np.uint8)
     root, nodes = W.merkle_tree(leaves, n)
@@ -226,6 +226,28 @@ def test_sumcheck_compute_fold(orc, n):
     w0, w2 = orc.sumcheck_compute_ext3(lift(f), b)
     assert np.array_equal(c0, w0) and np.array_equal(c2, w2)
     assert np.array_equal(W.sumcheck_fold(f, r, v_is_base=True), orc.sumcheck_fold_ext3(lift(f), r))
+
+
+@pytest.mark.parametrize("n,a_base,inplace", [(2, True, False), (16, False, False), (1000, False, True),
+                                              (1 << 16, True, False), (1 << 16, False, True), (1 << 20, True, False)])
+def test_sumcheck_round_fused(orc, n, a_base, inplace):
+    """compute + fold(a) + fold(b) in one sweep equals the three reference steps"""
+    a = rand(3 * n + 1, n if a_base else 3 * n)
+    b = rand(3 * n + 2, 3 * n)
+    r = rand(3 * n + 3, 3)
+    la = lift(a) if a_base else a
+    w0, w2 = orc.sumcheck_compute_ext3(la, b)
+    wa, wb = orc.sumcheck_fold_ext3(la, r), orc.sumcheck_fold_ext3(b, r)
+    with W.Context() as ctx:
+        da, db = ctx.upload(a), ctx.upload(b)
+        if inplace:
+            dfa, dfb = da, db
+        else:
+            dfa, dfb = ctx.alloc(24 * (n // 2)), ctx.alloc(24 * (n // 2))
+        c0, c2 = ctx.sumcheck_round(da, a_base, db, n, r, dfa, dfb)
+        fa, fb = ctx.download(dfa, 3 * (n // 2)), ctx.download(dfb, 3 * (n // 2))
+    assert np.array_equal(c0, w0) and np.array_equal(c2, w2)
+    assert np.array_equal(fa, wa) and np.array_equal(fb, wb)
 
 
 def test_sumcheck_golden(fixtures):

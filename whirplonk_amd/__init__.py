@@ -57,6 +57,7 @@ _SIGS = [
     ("whir_dev_free", _int, [_vp, _vp]),
     ("whir_memcpy_h2d", _int, [_vp, _vp, _vp, _sz]),
     ("whir_memcpy_d2h", _int, [_vp, _vp, _vp, _sz]),
+    ("whir_dev_copy", _int, [_vp, _vp, _vp, _sz]),
     ("whir_gpu_ntt_batch", _int, [_u64p, _sz, _sz]),
     ("whir_gpu_intt_batch", _int, [_u64p, _sz, _sz]),
     ("whir_gpu_interleaved_rs_encode", _int, [_u64p, _u64p, _sz, _sz, _sz, _sz]),
@@ -85,6 +86,10 @@ _SIGS = [
     ("whir_dev_sumcheck_compute", _int, [_vp, _vp, _int, _vp, _sz, _u64p, _u64p]),
     ("whir_dev_sumcheck_fold", _int, [_vp, _vp, _int, _sz, _u64p, _vp]),
     ("whir_dev_horner", _int, [_vp, _vp, _int, _sz, _u64p, _u64p]),
+    ("whir_dev_sumcheck_round", _int, [_vp, _vp, _int, _vp, _sz, _u64p, _vp, _vp, _u64p, _u64p]),
+    ("whir_ctx_enable_timing", _int, [_vp, _int]),
+    ("whir_ctx_reset_timing", _int, [_vp]),
+    ("whir_ctx_stage_times", _int, [_vp, ctypes.POINTER(ctypes.c_double), _u64p]),
 ]
 
 _lib = None
@@ -287,6 +292,9 @@ class Context:
         _check(load().whir_memcpy_d2h(self.h, out.ctypes.data, ptr, out.nbytes))
         return out
 
+    def copy(self, dst: int, src: int, nbytes: int):
+        _check(load().whir_dev_copy(self.h, dst, src, nbytes))
+
     def sync(self):
         _check(load().whir_ctx_synchronize(self.h))
 
@@ -321,6 +329,27 @@ class Context:
         rr = _u64(r)
         _check(load().whir_dev_sumcheck_fold(self.h, d_v, int(v_is_base), n, _p64(rr), d_out))
 
+    def sumcheck_round(self, d_a, a_is_base, d_b, n, r, d_a_out, d_b_out):
+        rr = _u64(r)
+        c0, c2 = np.empty(3, dtype=np.uint64), np.empty(3, dtype=np.uint64)
+        _check(load().whir_dev_sumcheck_round(self.h, d_a, int(a_is_base), d_b, n, _p64(rr), d_a_out, d_b_out,
+                                              _p64(c0), _p64(c2)))
+        return c0, c2
+
+    def enable_timing(self, on: bool = True):
+        _check(load().whir_ctx_enable_timing(self.h, int(on)))
+
+    def reset_timing(self):
+        _check(load().whir_ctx_reset_timing(self.h))
+
+    STAGES = ("rs_encode", "leaves", "merkle", "sumcheck_round")
+
+    def stage_times(self) -> dict:
+        ms = (ctypes.c_double * 4)()
+        cnt = np.zeros(4, dtype=np.uint64)
+        _check(load().whir_ctx_stage_times(self.h, ms, _p64(cnt)))
+        return {k: (ms[i], int(cnt[i])) for i, k in enumerate(self.STAGES)}
+
     def horner(self, d_f, f_is_ext3, n, z) -> np.ndarray:
         zz = _u64(z)
         out = np.empty(3, dtype=np.uint64)
@@ -330,3 +359,40 @@ class Context:
 
 def merkle_num_nodes(n: int) -> int:
     return load().whir_merkle_num_nodes(n)
+
+
+# ------------------------------------------------------------ seeded inputs
+
+_LCG_A = 6364136223846793005
+_LCG_C = 1442695040888963407
+
+
+def lcg_stream(seed: int, n: int) -> np.ndarray:
+    """The first n states of the LCG x <- x*A + C (mod 2^64) used by the
+    reference's tests and dumpers (e.g. cpp/tests/dump_whir.cpp:43-50),
+    vectorised with jump-ahead: block [2^j, 2^(j+1)) = A^(2^j) * block[0, 2^j) + C_(2^j)."""
+    out = np.empty(max(n, 1), dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        out[0] = np.uint64((seed * _LCG_A + _LCG_C) % 2**64)
+        have, a, c = 1, _LCG_A, _LCG_C  # (a, c) maps x_k -> x_{k+have}
+        while have < n:
+            take = min(have, n - have)
+            out[have:have + take] = out[:take] * np.uint64(a) + np.uint64(c)
+            a, c = (a * a) % 2**64, (a * c + c) % 2**64
+            have += take
+    return out[:n]
+
+
+def lcg_uniform(seed: int, n: int) -> tuple[np.ndarray, int]:
+    """n values uniform on [0, p): the LCG stream with draws >= p rejected.
+    Returns (values, state) so streams can be chained like the reference's Lcg."""
+    want = n + 64
+    while True:
+        s = lcg_stream(seed, want)
+        keep = s[s < np.uint64(P)]
+        if keep.size >= n:
+            # the state after the n-th accepted draw
+            last = int(np.nonzero(s < np.uint64(P))[0][n - 1]) if n else -1
+            state = int(s[last]) if n else seed
+            return keep[:n].copy(), state
+        want += 1024

@@ -16,22 +16,24 @@ __host__ __device__ constexpr int w64_exp(int k, bool inv) {
     return inv ? (192 - e) % 192 : e;
 }
 
-// x * 2^e mod p for an exponent that is a constant after unrolling.
-__device__ __forceinline__ uint64_t mul_pow2_rt(uint64_t x, int e) {
-    bool negate = e >= 96;
-    if (negate) e -= 96;
-    uint64_t r;
-    if (e == 0) {
-        r = x;
-    } else if (e < 64) {
-        r = gl::reduce128(x << e, x >> (64 - e));
-    } else {
-        uint64_t mid = (x << (e - 64)) & 0xFFFFFFFFull;
-        uint64_t hi = x >> (96 - e);
-        uint64_t t = (mid << 32) - mid;
-        r = (t >= hi) ? t - hi : t - hi + GL_P;
-    }
-    return negate ? gl::neg(r) : r;
+// x * 2^e mod p for 0 <= e < 96, e a constant after unrolling (2^96 = -1 is
+// folded into the butterfly by swapping its operands instead).
+__device__ __forceinline__ uint64_t mul_pow2_lt96(uint64_t x, int e) {
+    if (e == 0) return x;
+    if (e < 64) return gl::reduce128(x << e, x >> (64 - e));
+    // bits [64, 96) of x * 2^e weigh eps, bits [96, 160) weigh -1 (and are < 2^63 < p)
+    const int k = e - 64;
+    uint32_t mid = (uint32_t)x << k;
+    uint64_t hi = x >> (32 - k);
+    uint32_t b1, b2;
+    uint32_t t0 = __builtin_subc(0u, mid, 0u, &b1);
+    uint32_t t1 = __builtin_subc(mid, 0u, b1, &b2);
+    return gl::sub(((uint64_t)t1 << 32) | t0, hi);
+}
+
+// (a - b) * 2^e for any e in [0, 192): a factor 2^96 = -1 becomes (b - a).
+__device__ __forceinline__ uint64_t sub_mul_pow2(uint64_t a, uint64_t b, int e) {
+    return e >= 96 ? mul_pow2_lt96(gl::sub(b, a), e - 96) : mul_pow2_lt96(gl::sub(a, b), e);
 }
 
 constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
@@ -59,7 +61,7 @@ __device__ __forceinline__ void dft_reg(uint64_t* v) {
                 for (int j = 0; j < len / 2; ++j) {
                     uint64_t a = v[g + j], b = v[g + j + len / 2];
                     v[g + j] = gl::add(a, b);
-                    v[g + j + len / 2] = mul_pow2_rt(gl::sub(a, b), w64_exp(j * (64 / len), INV));
+                    v[g + j + len / 2] = sub_mul_pow2(a, b, w64_exp(j * (64 / len), INV));
                 }
             }
         }

@@ -31,8 +31,10 @@ struct LdeShape {
 // Scratch needed by lde_launch (elements of u64).
 size_t lde_scratch_elems(const LdeShape& s);
 // Returns 0, or >0 when the shape is not supported (not a power of two, m > 2^20).
+// With leaves != null and a row-major 16-column codeword the BLAKE3 row
+// leaves are hashed inside the last pass; *fused tells whether that happened.
 int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* scratch,
-               const Twiddles& tw, hipStream_t st);
+               const Twiddles& tw, hipStream_t st, uint8_t* leaves = nullptr, bool* fused = nullptr);
 
 // leaves[i] = BLAKE3(row i), row = row_u64 little-endian u64 words.
 void leaf_hash_launch(const uint64_t* matrix, uint64_t rows, uint32_t row_u64, uint8_t* leaves, hipStream_t st);
@@ -43,6 +45,8 @@ void merkle_build_launch(uint8_t* nodes, uint64_t n_leaves, hipStream_t st);
 // out[i] = nodes[idx[i]] (32 bytes each)
 void gather_hashes_launch(const uint8_t* nodes, const uint64_t* idx, uint64_t count, uint8_t* out, hipStream_t st);
 void fill_zero_launch(uint8_t* p, uint64_t bytes, hipStream_t st);
+// dst[0..bytes) = src[0..bytes) with a streaming kernel (bytes % 16 == 0, 16-byte aligned)
+void copy_launch(void* dst, const void* src, uint64_t bytes, hipStream_t st);
 
 // sumcheck over Ext3 (a may be base-field, lifted on the fly)
 // partials must hold 2*3*SUMCHECK_MAX_BLOCKS u64; result c0[3], c2[3] left in
@@ -50,6 +54,9 @@ void fill_zero_launch(uint8_t* p, uint64_t bytes, hipStream_t st);
 constexpr int SUMCHECK_MAX_BLOCKS = 2048;
 void sumcheck_compute_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint64_t n,
                              uint64_t* partials, hipStream_t st);
+// compute + both folds in one sweep; an Ext3 input may be folded in place
+void sumcheck_round_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint64_t n, const uint64_t r[3],
+                           uint64_t* a_out, uint64_t* b_out, uint64_t* partials, hipStream_t st);
 void sumcheck_fold_launch(const uint64_t* in, bool in_base, uint64_t n, const uint64_t r[3],
                           uint64_t* out, hipStream_t st);
 

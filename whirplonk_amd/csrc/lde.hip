@@ -19,8 +19,16 @@
 // of a row together (128-byte rows for C = 16).
 // Only the m nonzero inputs are ever read; the scratch Z_s holds m*C values
 // per coset in flight.
+// Twiddles shared by a whole workgroup live in LDS: pass A's coset twist
+// w_L^{s j1} depends on j1 only (shared by its TA columns), and when a pass-B
+// workgroup owns a single i1 the inter-pass twiddle w_n^{i1 j2} is applied
+// there (shared by all C columns) instead of at the end of pass A.
+// With C = 16 columns the pass-B workgroup holds complete 128-byte rows, so
+// the BLAKE3 row leaves (matrix_commit.hpp:169-195) are hashed right there,
+// from LDS, before the rows are written out.
 #include <algorithm>
 
+#include "blake3_dev.h"
 #include "dft.h"
 #include "kernels.h"
 
@@ -33,13 +41,21 @@ struct LdeArgs {
     const uint64_t* in;
     uint64_t* out;
     uint64_t* z;
+    uint8_t* leaves;   // fused row leaves (pass B), or null
     const uint64_t* tw_lo;
     const uint64_t* tw_hi;
     const uint64_t* t1024;
     uint64_t C, m, n, M1, M2, E, L, zstride;
     uint32_t in_el, in_comp, out_el, out_comp;
     uint32_t TA, TB, CB, row_major;
+    uint32_t tw_in_b;  // inter-pass twiddle applied in pass B (TB == 1)
 };
+
+constexpr int FUSE_COLS = 16;  // row width (u64) of the fused leaf hashing
+
+// swizzled position of (row, col) in the 16-wide LDS row stage: rows are
+// read one per lane, so columns rotate with the row to spread the banks
+__device__ __forceinline__ int stage_idx(int row, int col) { return row * FUSE_COLS + (col ^ (row & 15)); }
 
 // w_n^e for e < n from the two-level table
 __device__ __forceinline__ uint64_t twid(const LdeArgs& a, uint64_t e) {
@@ -60,11 +76,16 @@ __global__ __launch_bounds__(THREADS) void lde_pass_a(LdeArgs a, uint64_t s0) {
     const uint64_t s = s0 + blockIdx.y;
     uint64_t* z = a.z + blockIdx.y * a.zstride;
     const uint64_t* col = a.in + (c * a.m) * a.in_el + a.in_comp;
+    uint64_t* twist = lds + TA * PAD;  // w_L^{s j1} = w_n^{s j1 M2}, shared by the TA columns
 
+    if (s != 0) {
+        for (uint32_t j1 = threadIdx.x; j1 < M1; j1 += blockDim.x) twist[j1] = twid(a, (s * j1 * a.M2) & (a.n - 1));
+        __syncthreads();
+    }
     for (uint32_t q = threadIdx.x; q < TA * M1; q += blockDim.x) {
         uint32_t t = q % TA, j1 = q / TA;
         uint64_t x = col[(J + t + a.M2 * j1) * a.in_el];
-        if (s != 0 && j1 != 0) x = gl::mul(x, twid(a, (s * j1 * a.M2) & (a.n - 1)));
+        if (s != 0) x = gl::mul(x, twist[j1]);
         lds[t * PAD + dft::pidx<M1>(j1)] = x;
     }
     __syncthreads();
@@ -81,9 +102,11 @@ __global__ __launch_bounds__(THREADS) void lde_pass_a(LdeArgs a, uint64_t s0) {
 #pragma unroll
         for (int k2 = 0; k2 < R2; ++k2) {
             uint64_t k = k1 + (uint64_t)R1 * k2;
-            uint64_t e = ((s + a.E * k) * j2) & (a.n - 1);
             uint64_t zv = v[k2];
-            if (e) zv = gl::mul(zv, twid(a, e));
+            if (!a.tw_in_b) {
+                uint64_t e = ((s + a.E * k) * j2) & (a.n - 1);
+                if (e) zv = gl::mul(zv, twid(a, e));
+            }
             z[(k * a.C + c) * a.M2 + j2] = zv;
         }
     }
@@ -93,6 +116,7 @@ template <int M2, bool INV>
 __global__ __launch_bounds__(THREADS) void lde_pass_b(LdeArgs a, uint64_t s0) {
     using S = dft::Split<M2>;
     constexpr int R1 = S::R1, R2 = S::R2, PAD = S::PADDED + 1;
+    constexpr int DFT_WORDS = 16 * PAD, STAGE_WORDS = FUSE_COLS * M2;
     extern __shared__ uint64_t lds[];
     const uint32_t TB = a.TB, CB = a.CB, DB = TB * CB;
     const uint64_t nk = a.M1 / TB;
@@ -100,12 +124,22 @@ __global__ __launch_bounds__(THREADS) void lde_pass_b(LdeArgs a, uint64_t s0) {
     const uint64_t C0 = (blockIdx.x / nk) * CB;
     const uint64_t s = s0 + blockIdx.y;
     const uint64_t* z = a.z + blockIdx.y * a.zstride;
+    const bool fuse = a.leaves != nullptr;
+    // inter-pass twiddles w_n^{i1 j2} of this workgroup's single i1
+    uint64_t* twt = lds + (fuse ? (DFT_WORDS > STAGE_WORDS ? DFT_WORDS : STAGE_WORDS) : DB * PAD);
 
+    if (a.tw_in_b) {
+        const uint64_t i1 = s + a.E * K0;
+        for (uint32_t j2 = threadIdx.x; j2 < M2; j2 += blockDim.x) twt[j2] = twid(a, (i1 * j2) & (a.n - 1));
+        __syncthreads();
+    }
     for (uint32_t q = threadIdx.x; q < DB * M2; q += blockDim.x) {
         uint32_t j2 = q % M2, d = q / M2;
         uint32_t tb = a.row_major ? d / CB : d % TB;
         uint32_t cb = a.row_major ? d % CB : d / TB;
-        lds[d * PAD + dft::pidx<M2>(j2)] = z[((K0 + tb) * a.C + C0 + cb) * a.M2 + j2];
+        uint64_t v = z[((K0 + tb) * a.C + C0 + cb) * a.M2 + j2];
+        if (a.tw_in_b) v = gl::mul(v, twt[j2]);
+        lds[d * PAD + dft::pidx<M2>(j2)] = v;
     }
     __syncthreads();
     for (uint32_t w = threadIdx.x; w < DB * R2; w += blockDim.x) {
@@ -113,20 +147,71 @@ __global__ __launch_bounds__(THREADS) void lde_pass_b(LdeArgs a, uint64_t s0) {
         dft::lds_dft_step1<M2, INV>(lds + d * PAD, n2, a.t1024);
     }
     __syncthreads();
-    for (uint32_t w = threadIdx.x; w < DB * R1; w += blockDim.x) {
-        uint32_t d = w % DB, k1 = w / DB;
-        uint32_t tb = a.row_major ? d / CB : d % TB;
-        uint32_t cb = a.row_major ? d % CB : d / TB;
-        uint64_t v[R2];
-        dft::lds_dft_step2<M2, INV>(lds + d * PAD, k1, v);
-        const uint64_t i1 = s + a.E * (K0 + tb);
-        const uint64_t c = C0 + cb;
+    if (!fuse) {
+        for (uint32_t w = threadIdx.x; w < DB * R1; w += blockDim.x) {
+            uint32_t d = w % DB, k1 = w / DB;
+            uint32_t tb = a.row_major ? d / CB : d % TB;
+            uint32_t cb = a.row_major ? d % CB : d / TB;
+            uint64_t v[R2];
+            dft::lds_dft_step2<M2, INV>(lds + d * PAD, k1, v);
+            const uint64_t i1 = s + a.E * (K0 + tb);
+            const uint64_t c = C0 + cb;
 #pragma unroll
-        for (int k2 = 0; k2 < R2; ++k2) {
-            uint64_t i = i1 + a.L * (k1 + (uint64_t)R1 * k2);
-            uint64_t idx = a.row_major ? (i * a.C + c) : (c * a.n + i);
-            a.out[idx * a.out_el + a.out_comp] = v[k2];
+            for (int k2 = 0; k2 < R2; ++k2) {
+                uint64_t i = i1 + a.L * (k1 + (uint64_t)R1 * k2);
+                uint64_t idx = a.row_major ? (i * a.C + c) : (c * a.n + i);
+                a.out[idx * a.out_el + a.out_comp] = v[k2];
+            }
         }
+        return;
+    }
+    // Fused leaves: this workgroup owns rows i = i1 + L*i2 (i2 < M2) with all
+    // 16 columns (TB = 1, CB = C = 16).  Stage them row-major in LDS, hash one
+    // row per lane, then write the rows out coalesced.
+    constexpr int NW = (16 * R1 + THREADS - 1) / THREADS;
+    uint64_t v[NW][R2];
+#pragma unroll
+    for (int it = 0; it < NW; ++it) {
+        uint32_t w = threadIdx.x + it * THREADS;
+        if (w < 16 * R1) dft::lds_dft_step2<M2, INV>(lds + (w % 16) * PAD, w / 16, v[it]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < NW; ++it) {
+        uint32_t w = threadIdx.x + it * THREADS;
+        if (w < 16 * R1) {
+            const int cb = w % 16, k1 = w / 16;
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) {
+                int row = k1 + R1 * k2;
+                lds[stage_idx(row, cb)] = v[it][k2];
+            }
+        }
+    }
+    __syncthreads();
+    const uint64_t i1 = s + a.E * K0;
+    for (uint32_t r = threadIdx.x; r < M2; r += blockDim.x) {
+        uint32_t cv[8], m[16];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) cv[i] = b3::iv(i);
+#pragma unroll
+        for (int blk = 0; blk < 2; ++blk) {
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                uint64_t x = lds[stage_idx(r, 8 * blk + c)];
+                m[2 * c] = (uint32_t)x;
+                m[2 * c + 1] = (uint32_t)(x >> 32);
+            }
+            b3::compress(cv, m, 0, 64, blk == 0 ? b3::CHUNK_START : (b3::CHUNK_END | b3::ROOT), cv);
+        }
+        uint64_t i = i1 + a.L * r;
+        uint4* dst = reinterpret_cast<uint4*>(a.leaves + 32 * i);
+        dst[0] = make_uint4(cv[0], cv[1], cv[2], cv[3]);
+        dst[1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
+    }
+    for (uint32_t q = threadIdx.x; q < 16 * M2; q += blockDim.x) {
+        uint32_t r = q / 16, c = q % 16;
+        a.out[(i1 + a.L * r) * 16 + c] = lds[stage_idx(r, c)];
     }
 }
 
@@ -224,14 +309,19 @@ size_t lde_scratch_elems(const LdeShape& s) {
 }
 
 int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* scratch,
-               const Twiddles& tw, hipStream_t st) {
+               const Twiddles& tw, hipStream_t st, uint8_t* leaves, bool* fused) {
     Plan p;
+    if (fused) *fused = false;
     if (!make_plan(s, p)) return 1;
     if (tw.log_n != (uint32_t)ilog2u(s.n)) return 2;
     LdeArgs a;
     a.in = in;
     a.out = out;
     a.z = scratch;
+    a.tw_in_b = p.TB == 1 ? 1 : 0;
+    bool fuse = leaves && s.row_major && s.C == FUSE_COLS && p.TB == 1 && p.CB == FUSE_COLS && s.out_el == 1;
+    a.leaves = fuse ? leaves : nullptr;
+    if (fused) *fused = fuse;
     a.tw_lo = tw.lo;
     a.tw_hi = tw.hi;
     a.t1024 = tw.t1024;
@@ -254,8 +344,9 @@ int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* s
 
     KernelFn ka = s.inverse ? pick_a<true>(p.M1) : pick_a<false>(p.M1);
     KernelFn kb = s.inverse ? pick_b<true>(p.M2) : pick_b<false>(p.M2);
-    size_t lds_a = (size_t)p.TA * padded(p.M1) * 8;
-    size_t lds_b = (size_t)p.TB * p.CB * padded(p.M2) * 8;
+    size_t lds_a = ((size_t)p.TA * padded(p.M1) + p.M1) * 8;
+    size_t words_b = fuse ? std::max<size_t>(16 * padded(p.M2), FUSE_COLS * p.M2) : (size_t)p.TB * p.CB * padded(p.M2);
+    size_t lds_b = (words_b + (a.tw_in_b ? p.M2 : 0)) * 8;
     static bool attr_done[2][2][11] = {};
     int ia = ilog2u(p.M1), ib = ilog2u(p.M2), inv = s.inverse ? 1 : 0;
     if (!attr_done[inv][0][ia]) {

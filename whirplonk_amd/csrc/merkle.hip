@@ -107,6 +107,29 @@ __global__ __launch_bounds__(256) void merkle_level_kernel(uint8_t* __restrict__
     store_hash(nodes + 32 * (parent_off + i), h);
 }
 
+// Two levels per launch: lane i hashes children 4i..4i+3 into parents 2i, 2i+1
+// (written out) and their parent i, so the middle level is never re-read.
+__global__ __launch_bounds__(256) void merkle_2level_kernel(uint8_t* __restrict__ nodes, uint64_t child_off,
+                                                            uint64_t child_width) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= child_width / 4) return;
+    uint32_t m[16], h0[8], h1[8], g[8];
+    load64(nodes + 32 * (child_off + 4 * i), m);
+    b3::hash64(m, h0);
+    load64(nodes + 32 * (child_off + 4 * i + 2), m);
+    b3::hash64(m, h1);
+    const uint64_t parent_off = child_off + child_width;
+    store_hash(nodes + 32 * (parent_off + 2 * i), h0);
+    store_hash(nodes + 32 * (parent_off + 2 * i + 1), h1);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        m[k] = h0[k];
+        m[8 + k] = h1[k];
+    }
+    b3::hash64(m, g);
+    store_hash(nodes + 32 * (parent_off + child_width / 2 + i), g);
+}
+
 // The top of the tree: levels of width <= TOP_WIDTH/2 parents, one workgroup.
 __global__ __launch_bounds__(1024) void merkle_top_kernel(uint8_t* __restrict__ nodes, uint64_t child_off,
                                                           uint64_t child_width) {
@@ -135,6 +158,12 @@ __global__ __launch_bounds__(256) void gather_hashes_kernel(const uint8_t* __res
     d[1] = s[1];
 }
 
+__global__ __launch_bounds__(256) void copy16_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src,
+                                                     uint64_t n16) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
 __global__ __launch_bounds__(256) void fill_zero_kernel(uint4* p, uint64_t n16) {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n16) p[i] = make_uint4(0, 0, 0, 0);
@@ -160,6 +189,15 @@ void fill_zero_launch(uint8_t* p, uint64_t bytes, hipStream_t st) {
     hipLaunchKernelGGL(fill_zero_kernel, dim3(blocks_for(n16, 256)), dim3(256), 0, st, reinterpret_cast<uint4*>(p), n16);
 }
 
+void copy_launch(void* dst, const void* src, uint64_t bytes, hipStream_t st) {
+    uint64_t n16 = bytes / 16;
+    if (!n16) return;
+    unsigned g = blocks_for(n16, 256);
+    if (g > 65536) g = 65536;
+    hipLaunchKernelGGL(copy16_kernel, dim3(g), dim3(256), 0, st, reinterpret_cast<uint4*>(dst),
+                       reinterpret_cast<const uint4*>(src), n16);
+}
+
 void merkle_build_launch(uint8_t* nodes, uint64_t n_leaves, hipStream_t st) {
     uint64_t width = 1;
     while (width < n_leaves) width <<= 1;
@@ -169,6 +207,11 @@ void merkle_build_launch(uint8_t* nodes, uint64_t n_leaves, hipStream_t st) {
         return;
     }
     uint64_t off = 0;
+    while (width >= 4 * TOP_WIDTH) {
+        hipLaunchKernelGGL(merkle_2level_kernel, dim3(blocks_for(width / 4, 256)), dim3(256), 0, st, nodes, off, width);
+        off += width + width / 2;
+        width /= 4;
+    }
     while (width > TOP_WIDTH) {
         uint64_t parents = width / 2;
         hipLaunchKernelGGL(merkle_level_kernel, dim3(blocks_for(parents, 256)), dim3(256), 0, st, nodes, off,

@@ -86,6 +86,46 @@ __global__ __launch_bounds__(256) void sumcheck_compute_kernel(const uint64_t* _
     }
 }
 
+// One sumcheck round in a single sweep: (c0, c2) partials of (a, b) and both
+// folds with the round's challenge r (sumcheck_protocol.hpp:108-134, where the
+// reference computes, then folds a, then folds b: three passes over the data).
+// Here r is known up front (given by the caller), so each (i, i + n/2) pair is
+// read once.
+// Lane i reads indices i and i + n/2 and writes index i only, so the folded
+// vectors may overwrite the lower halves of Ext3 inputs (no __restrict__).
+__global__ __launch_bounds__(256) void sumcheck_round_kernel(const uint64_t* a, int a_base, const uint64_t* b,
+                                                             uint64_t half, e3 r, uint64_t* a_out, uint64_t* b_out,
+                                                             uint64_t* __restrict__ partials) {
+    e3 acc[2] = {{0, 0, 0}, {0, 0, 0}};
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < half; i += stride) {
+        e3 b0 = load_e3(b, i), b1 = load_e3(b, i + half);
+        e3 db = gl::e3_sub(b1, b0);
+        e3 fa;
+        if (a_base) {
+            uint64_t a0 = a[i], a1 = a[i + half];
+            uint64_t da = gl::sub(a1, a0);
+            acc[0] = gl::e3_add(acc[0], gl::e3_mul_base(b0, a0));
+            acc[1] = gl::e3_add(acc[1], gl::e3_mul_base(db, da));
+            fa = gl::e3_mul_base(r, da);
+            fa.c0 = gl::add(fa.c0, a0);
+        } else {
+            e3 a0 = load_e3(a, i), a1 = load_e3(a, i + half);
+            e3 da = gl::e3_sub(a1, a0);
+            acc[0] = gl::e3_add(acc[0], gl::e3_mul(a0, b0));
+            acc[1] = gl::e3_add(acc[1], gl::e3_mul(da, db));
+            fa = gl::e3_add(a0, gl::e3_mul(da, r));
+        }
+        store_e3(a_out, i, fa);
+        store_e3(b_out, i, gl::e3_add(b0, gl::e3_mul(db, r)));
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        store_e3(partials, 2 * blockIdx.x, acc[0]);
+        store_e3(partials, 2 * blockIdx.x + 1, acc[1]);
+    }
+}
+
 // partials[2*blk], partials[2*blk+1] for blk < nblocks -> partials[0], partials[1]
 __global__ __launch_bounds__(256) void reduce_pairs_kernel(uint64_t* partials, uint32_t nblocks) {
     e3 acc[2] = {{0, 0, 0}, {0, 0, 0}};
@@ -163,6 +203,15 @@ void sumcheck_compute_launch(const uint64_t* a, bool a_base, const uint64_t* b, 
                              hipStream_t st) {
     unsigned g = grid_for(n / 2, SUMCHECK_MAX_BLOCKS);
     hipLaunchKernelGGL(sumcheck_compute_kernel, dim3(g), dim3(256), 0, st, a, a_base ? 1 : 0, b, n / 2, partials);
+    hipLaunchKernelGGL(reduce_pairs_kernel, dim3(1), dim3(256), 0, st, partials, g);
+}
+
+void sumcheck_round_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint64_t n, const uint64_t r[3],
+                           uint64_t* a_out, uint64_t* b_out, uint64_t* partials, hipStream_t st) {
+    unsigned g = grid_for(n / 2, SUMCHECK_MAX_BLOCKS);
+    e3 rr = {r[0], r[1], r[2]};
+    hipLaunchKernelGGL(sumcheck_round_kernel, dim3(g), dim3(256), 0, st, a, a_base ? 1 : 0, b, n / 2, rr, a_out, b_out,
+                       partials);
     hipLaunchKernelGGL(reduce_pairs_kernel, dim3(1), dim3(256), 0, st, partials, g);
 }
 

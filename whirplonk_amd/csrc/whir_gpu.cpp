@@ -76,6 +76,28 @@ struct whir_ctx {
     DevBuf scratch, partials, zpow, hin, hout, hnodes, hidx, hcomp;
     std::map<std::pair<uint32_t, bool>, Twiddles> tw;
     uint64_t* t1024[2] = {nullptr, nullptr};
+    // stage timing (HIP events on `stream`)
+    bool timing = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double acc_ms[4] = {0, 0, 0, 0};
+    uint64_t cnt[4] = {0, 0, 0, 0};
+
+    int mark(int k) {
+        if (!timing) return 0;
+        if (!ev[k]) HIPCHK(hipEventCreate(&ev[k]));
+        HIPCHK(hipEventRecord(ev[k], stream));
+        return 0;
+    }
+    // adds the time between marks a and b to stage st (waits for mark b)
+    int collect(int a, int b, int st) {
+        if (!timing) return 0;
+        HIPCHK(hipEventSynchronize(ev[b]));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ev[a], ev[b]));
+        acc_ms[st] += ms;
+        cnt[st] += 1;
+        return 0;
+    }
 
     int ensure(DevBuf& b, size_t bytes) {
         if (b.bytes >= bytes && b.p) return 0;
@@ -176,24 +198,28 @@ int rs_shape(size_t num_polys, size_t poly_size, size_t cw, size_t depth, LdeSha
     return 0;
 }
 
-int run_lde(whir_ctx* ctx, LdeShape s, const uint64_t* in, uint64_t* out) {
+int run_lde(whir_ctx* ctx, LdeShape s, const uint64_t* in, uint64_t* out, uint8_t* leaves = nullptr,
+            bool* fused = nullptr) {
     Twiddles tw;
     int rc = ctx->twiddles(log2u(s.n), s.inverse, tw);
     if (rc) return rc;
     rc = ctx->ensure(ctx->scratch, lde_scratch_elems(s) * 8);
     if (rc) return rc;
-    rc = lde_launch(s, in, out, (uint64_t*)ctx->scratch.p, tw, ctx->stream);
+    rc = lde_launch(s, in, out, (uint64_t*)ctx->scratch.p, tw, ctx->stream, leaves, fused);
     if (rc > 0) return bad_arg("LDE shape not supported");
     HIPCHK(hipGetLastError());
     return 0;
 }
 
+// RS encode; for base-field input, leaves (when non-null) may be filled by the
+// fused last pass, reported through *fused.
 int rs_encode_any(whir_ctx* ctx, const uint64_t* d_in, size_t num_polys, size_t poly_size, size_t cw, size_t depth,
-                  uint64_t* d_out, int ext) {
+                  uint64_t* d_out, int ext, uint8_t* leaves = nullptr, bool* fused = nullptr) {
     LdeShape s;
+    if (fused) *fused = false;
     int rc = rs_shape(num_polys, poly_size, cw, depth, s);
     if (rc) return rc;
-    if (!ext) return run_lde(ctx, s, d_in, d_out);
+    if (!ext) return run_lde(ctx, s, d_in, d_out, leaves, fused);
     s.in_el = 3;
     s.out_el = 3;
     for (uint32_t c = 0; c < 3; ++c) {
@@ -284,6 +310,8 @@ int whir_ctx_destroy(whir_ctx* ctx) {
     }
     for (auto* p : ctx->t1024)
         if (p) (void)hipFree(p);
+    for (auto* e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
     if (ctx == g_default) g_default = nullptr;
     delete ctx;
@@ -321,6 +349,14 @@ int whir_memcpy_d2h(whir_ctx* ctx, void* dst, const void* src, size_t bytes) {
     if (!bytes) return 0;
     HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int whir_dev_copy(whir_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16) return bad_arg("copy needs 16-byte multiples");
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    copy_launch(dst, src, bytes, ctx->stream);
+    HIPCHK(hipGetLastError());
     return 0;
 }
 
@@ -390,17 +426,23 @@ int whir_dev_gather_hashes(whir_ctx* ctx, const uint8_t* d_nodes, const uint64_t
 static int commit_any(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_polys, size_t poly_size, size_t cw,
                       size_t depth, uint64_t* d_matrix, uint8_t* d_nodes, uint8_t* out_root, int ext) {
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
-    int rc = rs_encode_any(ctx, d_coeffs, num_polys, poly_size, cw, depth, d_matrix, ext);
-    if (rc) return rc;
+    int rc;
+    bool fused = false;
+    if ((rc = ctx->mark(0))) return rc;
+    if ((rc = rs_encode_any(ctx, d_coeffs, num_polys, poly_size, cw, depth, d_matrix, ext, d_nodes, &fused))) return rc;
+    if ((rc = ctx->mark(1))) return rc;
     size_t row_u64 = num_polys * depth * (ext ? 3 : 1);
-    leaf_hash_launch(d_matrix, cw, (uint32_t)row_u64, d_nodes, ctx->stream);
+    if (!fused) leaf_hash_launch(d_matrix, cw, (uint32_t)row_u64, d_nodes, ctx->stream);
+    if ((rc = ctx->mark(2))) return rc;
     merkle_build_launch(d_nodes, cw, ctx->stream);
     HIPCHK(hipGetLastError());
+    if ((rc = ctx->mark(3))) return rc;
     if (out_root) {
         size_t nn = whir_merkle_num_nodes(cw);
         HIPCHK(hipMemcpyAsync(out_root, d_nodes + 32 * (nn - 1), 32, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
     }
+    if ((rc = ctx->collect(0, 1, 0)) || (rc = ctx->collect(1, 2, 1)) || (rc = ctx->collect(2, 3, 2))) return rc;
     return 0;
 }
 
@@ -468,6 +510,53 @@ int whir_dev_horner(whir_ctx* ctx, const uint64_t* d_f, int f_is_ext3, size_t n,
     return 0;
 }
 
+int whir_dev_sumcheck_round(whir_ctx* ctx, const uint64_t* d_a, int a_is_base, const uint64_t* d_b, size_t n,
+                            const uint64_t* r, uint64_t* d_a_out, uint64_t* d_b_out, uint64_t* out_c0,
+                            uint64_t* out_c2) {
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (n % 2) return bad_arg("sumcheck length must be even");
+    if (a_is_base && (const void*)d_a == (const void*)d_a_out) return bad_arg("base-field fold cannot run in place");
+    int rc = ctx->ensure(ctx->partials, 6 * SUMCHECK_MAX_BLOCKS * 8);
+    if (rc) return rc;
+    uint64_t* part = (uint64_t*)ctx->partials.p;
+    if ((rc = ctx->mark(0))) return rc;
+    // Each output index i < n/2 reads only inputs i and i + n/2, so writing
+    // the folds over the inputs' lower halves in place is also safe.
+    sumcheck_round_launch(d_a, a_is_base != 0, d_b, n, r, d_a_out, d_b_out, part, ctx->stream);
+    HIPCHK(hipGetLastError());
+    if ((rc = ctx->mark(1))) return rc;
+    uint64_t h[6];
+    HIPCHK(hipMemcpyAsync(h, part, 48, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    std::memcpy(out_c0, h, 24);
+    std::memcpy(out_c2, h + 3, 24);
+    return ctx->collect(0, 1, 3);
+}
+
+int whir_ctx_enable_timing(whir_ctx* ctx, int on) {
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    ctx->timing = on != 0;
+    return 0;
+}
+
+int whir_ctx_reset_timing(whir_ctx* ctx) {
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    for (int i = 0; i < 4; ++i) {
+        ctx->acc_ms[i] = 0;
+        ctx->cnt[i] = 0;
+    }
+    return 0;
+}
+
+int whir_ctx_stage_times(whir_ctx* ctx, double* out_ms, uint64_t* out_counts) {
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    for (int i = 0; i < 4; ++i) {
+        if (out_ms) out_ms[i] = ctx->acc_ms[i];
+        if (out_counts) out_counts[i] = ctx->cnt[i];
+    }
+    return 0;
+}
+
 // ------------------------------------------------------ host-pointer drop-ins
 
 static int ntt_host(uint64_t* values, size_t total, size_t size, int inverse) {
@@ -506,9 +595,11 @@ static int rs_host(const uint64_t* coeffs, uint64_t* out_matrix, uint8_t* out_ha
         if (out_hashes && (rc = ctx->ensure(ctx->hnodes, cw * 32))) return rc;
         uint64_t* din = (uint64_t*)ctx->hin.p;
         uint64_t* dout = (uint64_t*)ctx->hout.p;
+        uint8_t* dleaves = out_hashes ? (uint8_t*)ctx->hnodes.p : nullptr;
+        bool fused = false;
         HIPCHK(hipMemcpyAsync(din, coeffs, nin * 8, hipMemcpyHostToDevice, ctx->stream));
-        if ((rc = run_lde(ctx, s, din, dout))) return rc;
-        if (out_hashes) leaf_hash_launch(dout, cw, (uint32_t)(num_polys * depth), (uint8_t*)ctx->hnodes.p, ctx->stream);
+        if ((rc = run_lde(ctx, s, din, dout, dleaves, &fused))) return rc;
+        if (out_hashes && !fused) leaf_hash_launch(dout, cw, (uint32_t)(num_polys * depth), dleaves, ctx->stream);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(out_matrix, dout, nout * 8, hipMemcpyDeviceToHost, ctx->stream));
         if (out_hashes) HIPCHK(hipMemcpyAsync(out_hashes, ctx->hnodes.p, cw * 32, hipMemcpyDeviceToHost, ctx->stream));
