@@ -119,6 +119,7 @@ def ref():
         R.ref_sumcheck_fold_ext3.argtypes = [_u64p, _sz, _u64p]
         R.ref_horner_base_ext3.argtypes = [_u64p, _sz, _u64p, _u64p]
         R.ref_commit_base.argtypes = [_u64p, _sz, _sz, _sz, _u64p, _u8p]
+        R.ref_blake3_many.argtypes = [_sz, _u8p, _sz, _u8p]
         _ref = R
     return _ref
 

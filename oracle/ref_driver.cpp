@@ -16,6 +16,7 @@
 //   ref_horner_base_ext3   algebra/utilities.hpp:195-246 mixed_univariate_evaluate
 //   ref_commit_base        the irs_commit.hpp:344-460 sequence (encode, leaves, tree)
 //                          without the transcript
+//   ref_blake3_many        hash/blake3_engine.hpp:90-199 Blake3::hash_many (both branches)
 #include "whir/algebra/embedding.hpp"
 #include "whir/algebra/goldilocks.hpp"
 #include "whir/algebra/goldilocks_ext3.hpp"
@@ -170,6 +171,12 @@ void ref_horner_base_ext3(const uint64_t* coeffs, size_t n, const uint64_t z[3],
     auto r = whir::algebra::mixed_univariate_evaluate<whir::algebra::Basefield<GoldilocksExt3>>(
         emb, std::span<const Goldilocks>{reinterpret_cast<const Goldilocks*>(coeffs), n}, p);
     from_e3(r, out);
+}
+
+void ref_blake3_many(size_t size, const uint8_t* input, size_t count, uint8_t* out) {
+    std::vector<Hash> h(count);
+    blake3().hash_many(size, std::span<const uint8_t>{input, size * count}, std::span<Hash>{h});
+    if (count) std::memcpy(out, h.data(), count * 32);
 }
 
 // One plain commit of a single base-field polynomial: RS encode, row leaves,

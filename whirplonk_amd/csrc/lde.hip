@@ -49,6 +49,8 @@ struct LdeArgs {
     uint32_t in_el, in_comp, out_el, out_comp;
     uint32_t TA, TB, CB, row_major;
     uint32_t tw_in_b;  // inter-pass twiddle applied in pass B (TB == 1)
+    // log2 of the power-of-two tile sizes: every index split is a shift/mask
+    uint32_t lgTA, lgTB, lgCB, lgNJ, lgNK;
 };
 
 constexpr int FUSE_COLS = 16;  // row width (u64) of the fused leaf hashing
@@ -69,13 +71,13 @@ __global__ __launch_bounds__(THREADS) void lde_pass_a(LdeArgs a, uint64_t s0) {
     using S = dft::Split<M1>;
     constexpr int R1 = S::R1, R2 = S::R2, PAD = S::PADDED + 1;
     extern __shared__ uint64_t lds[];
-    const uint32_t TA = a.TA;
-    const uint64_t nj = a.M2 / TA;
-    const uint64_t c = blockIdx.x / nj;
-    const uint64_t J = (blockIdx.x % nj) * TA;
+    const uint32_t TA = a.TA, lgTA = a.lgTA;
+    const uint32_t c = blockIdx.x >> a.lgNJ;
+    const uint32_t J = (blockIdx.x & ((1u << a.lgNJ) - 1)) << lgTA;
     const uint64_t s = s0 + blockIdx.y;
-    uint64_t* z = a.z + blockIdx.y * a.zstride;
-    const uint64_t* col = a.in + (c * a.m) * a.in_el + a.in_comp;
+    uint64_t* z = a.z + blockIdx.y * a.zstride + (uint64_t)c * a.M2 + J;   // Z[k][c][J + t] = z[k*C*M2 + t]
+    const uint64_t* col = a.in + ((uint64_t)c * a.m + J) * a.in_el + a.in_comp;
+    const uint32_t in_el = a.in_el, M2 = (uint32_t)a.M2, zrow = (uint32_t)(a.C * a.M2);
     uint64_t* twist = lds + TA * PAD;  // w_L^{s j1} = w_n^{s j1 M2}, shared by the TA columns
 
     if (s != 0) {
@@ -83,8 +85,8 @@ __global__ __launch_bounds__(THREADS) void lde_pass_a(LdeArgs a, uint64_t s0) {
         __syncthreads();
     }
     for (uint32_t q = threadIdx.x; q < TA * M1; q += blockDim.x) {
-        uint32_t t = q % TA, j1 = q / TA;
-        uint64_t x = col[(J + t + a.M2 * j1) * a.in_el];
+        uint32_t t = q & (TA - 1), j1 = q >> lgTA;
+        uint64_t x = col[(t + M2 * j1) * in_el];
         if (s != 0) x = gl::mul(x, twist[j1]);
         lds[t * PAD + dft::pidx<M1>(j1)] = x;
     }
@@ -95,19 +97,19 @@ __global__ __launch_bounds__(THREADS) void lde_pass_a(LdeArgs a, uint64_t s0) {
     }
     __syncthreads();
     for (uint32_t w = threadIdx.x; w < TA * R1; w += blockDim.x) {
-        uint32_t t = w % TA, k1 = w / TA;
+        uint32_t t = w & (TA - 1), k1 = w >> lgTA;
         uint64_t v[R2];
         dft::lds_dft_step2<M1, INV>(lds + t * PAD, k1, v);
         const uint64_t j2 = J + t;
 #pragma unroll
         for (int k2 = 0; k2 < R2; ++k2) {
-            uint64_t k = k1 + (uint64_t)R1 * k2;
+            uint32_t k = k1 + R1 * k2;
             uint64_t zv = v[k2];
             if (!a.tw_in_b) {
                 uint64_t e = ((s + a.E * k) * j2) & (a.n - 1);
                 if (e) zv = gl::mul(zv, twid(a, e));
             }
-            z[(k * a.C + c) * a.M2 + j2] = zv;
+            z[k * zrow + t] = zv;
         }
     }
 }
@@ -118,12 +120,12 @@ __global__ __launch_bounds__(THREADS) void lde_pass_b(LdeArgs a, uint64_t s0) {
     constexpr int R1 = S::R1, R2 = S::R2, PAD = S::PADDED + 1;
     constexpr int DFT_WORDS = 16 * PAD, STAGE_WORDS = FUSE_COLS * M2;
     extern __shared__ uint64_t lds[];
-    const uint32_t TB = a.TB, CB = a.CB, DB = TB * CB;
-    const uint64_t nk = a.M1 / TB;
-    const uint64_t K0 = (blockIdx.x % nk) * TB;
-    const uint64_t C0 = (blockIdx.x / nk) * CB;
+    const uint32_t TB = a.TB, CB = a.CB, DB = TB * CB, lgTB = a.lgTB, lgCB = a.lgCB;
+    const uint32_t K0 = (blockIdx.x & ((1u << a.lgNK) - 1)) << lgTB;
+    const uint32_t C0 = (blockIdx.x >> a.lgNK) << lgCB;
     const uint64_t s = s0 + blockIdx.y;
-    const uint64_t* z = a.z + blockIdx.y * a.zstride;
+    const uint32_t CM2 = (uint32_t)(a.C * M2);
+    const uint64_t* z = a.z + blockIdx.y * a.zstride + (uint64_t)K0 * CM2 + (uint64_t)C0 * M2;
     const bool fuse = a.leaves != nullptr;
     // inter-pass twiddles w_n^{i1 j2} of this workgroup's single i1
     uint64_t* twt = lds + (fuse ? (DFT_WORDS > STAGE_WORDS ? DFT_WORDS : STAGE_WORDS) : DB * PAD);
@@ -135,9 +137,9 @@ __global__ __launch_bounds__(THREADS) void lde_pass_b(LdeArgs a, uint64_t s0) {
     }
     for (uint32_t q = threadIdx.x; q < DB * M2; q += blockDim.x) {
         uint32_t j2 = q % M2, d = q / M2;
-        uint32_t tb = a.row_major ? d / CB : d % TB;
-        uint32_t cb = a.row_major ? d % CB : d / TB;
-        uint64_t v = z[((K0 + tb) * a.C + C0 + cb) * a.M2 + j2];
+        uint32_t tb = a.row_major ? d >> lgCB : d & (TB - 1);
+        uint32_t cb = a.row_major ? d & (CB - 1) : d >> lgTB;
+        uint64_t v = z[tb * CM2 + cb * M2 + j2];
         if (a.tw_in_b) v = gl::mul(v, twt[j2]);
         lds[d * PAD + dft::pidx<M2>(j2)] = v;
     }
@@ -149,9 +151,9 @@ __global__ __launch_bounds__(THREADS) void lde_pass_b(LdeArgs a, uint64_t s0) {
     __syncthreads();
     if (!fuse) {
         for (uint32_t w = threadIdx.x; w < DB * R1; w += blockDim.x) {
-            uint32_t d = w % DB, k1 = w / DB;
-            uint32_t tb = a.row_major ? d / CB : d % TB;
-            uint32_t cb = a.row_major ? d % CB : d / TB;
+            uint32_t d = w & (DB - 1), k1 = w >> (lgTB + lgCB);
+            uint32_t tb = a.row_major ? d >> lgCB : d & (TB - 1);
+            uint32_t cb = a.row_major ? d & (CB - 1) : d >> lgTB;
             uint64_t v[R2];
             dft::lds_dft_step2<M2, INV>(lds + d * PAD, k1, v);
             const uint64_t i1 = s + a.E * (K0 + tb);
@@ -284,6 +286,7 @@ bool make_plan(const LdeShape& s, Plan& p) {
     p.M2 = std::min<uint64_t>(s.m, 1024);
     p.M1 = s.m / p.M2;
     if (p.M1 > 1024) return false;
+    if (s.m * s.C * s.in_el >= (1ull << 31)) return false;  // per-launch offsets are 32-bit
     p.E = s.n / s.m;
     p.L = p.E * p.M1;
     p.TA = (uint32_t)std::min<uint64_t>(p.M2, THREADS / split_r2(p.M1));
@@ -340,6 +343,11 @@ int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* s
     a.TA = p.TA;
     a.TB = p.TB;
     a.CB = p.CB;
+    a.lgTA = (uint32_t)ilog2u(p.TA);
+    a.lgTB = (uint32_t)ilog2u(p.TB);
+    a.lgCB = (uint32_t)ilog2u(p.CB);
+    a.lgNJ = (uint32_t)ilog2u(p.M2 / p.TA);
+    a.lgNK = (uint32_t)ilog2u(p.M1 / p.TB);
     a.row_major = s.row_major ? 1 : 0;
 
     KernelFn ka = s.inverse ? pick_a<true>(p.M1) : pick_a<false>(p.M1);
