@@ -17,6 +17,9 @@
 //   ref_commit_base        the irs_commit.hpp:344-460 sequence (encode, leaves, tree)
 //                          without the transcript
 //   ref_blake3_many        hash/blake3_engine.hpp:90-199 Blake3::hash_many (both branches)
+//   ref_whir_config        protocols/whir/whir.hpp:118-224 Config::from_params, flattened
+//   ref_whir_prove         the main.cpp:292-424 PCS flow: from_params, the CLI's domain
+//                          separator (main.cpp:176-215), commit, MLE constraints, prove
 #include "whir/algebra/embedding.hpp"
 #include "whir/algebra/goldilocks.hpp"
 #include "whir/algebra/goldilocks_ext3.hpp"
@@ -27,6 +30,8 @@
 #include "whir/hash/blake3_engine.hpp"
 #include "whir/protocols/matrix_commit.hpp"
 #include "whir/protocols/merkle_tree.hpp"
+#include "whir/protocols/whir/whir.hpp"
+#include "whir/transcript/transcript.hpp"
 
 #include <cstdint>
 #include <cstring>
@@ -198,6 +203,155 @@ void ref_commit_base(const uint64_t* coeffs, size_t poly_size, size_t cw, size_t
                             [](whir::EngineId) -> const whir::hash::HashEngine& { return blake3(); });
     std::memcpy(out_root, w.nodes.back().data(), 32);
     if (out_matrix) std::memcpy(out_matrix, matrix.data(), matrix.size() * 8);
+}
+
+}  // extern "C"
+
+namespace {
+namespace whirp = whir::protocols::whir;
+using Emb = whir::algebra::Basefield<GoldilocksExt3>;
+
+whir::ProtocolParameters cli_params(uint32_t log_inv_rate, uint32_t folding_factor, uint32_t security_level,
+                                    uint32_t pow_bits) {
+    // main.cpp:315-323 (PCS, ConjectureList, batch 1, BLAKE3)
+    whir::ProtocolParameters p;
+    p.security_level = security_level;
+    p.pow_bits = pow_bits;
+    p.initial_folding_factor = folding_factor;
+    p.folding_factor = folding_factor;
+    p.unique_decoding = false;
+    p.starting_log_inv_rate = log_inv_rate;
+    p.batch_size = 1;
+    p.hash_id = whir::hash::ENGINE_ID_BLAKE3;
+    return p;
+}
+
+uint64_t dbits(double x) {
+    uint64_t u;
+    std::memcpy(&u, &x, 8);
+    return u;
+}
+
+template <typename C>
+void put_irs(std::vector<uint64_t>& o, const C& c) {
+    o.push_back(c.num_vectors);
+    o.push_back(c.vector_size);
+    o.push_back(c.codeword_length);
+    o.push_back(c.interleaving_depth);
+    o.push_back(c.in_domain_samples);
+    o.push_back(c.out_domain_samples);
+    o.push_back(dbits(c.johnson_slack));
+}
+
+// The CLI's domain separator: SHA3-512 of the CBOR map of the parameters and
+// SHA3-256 of the CBOR session text (restated from main.cpp:176-215).
+whir::transcript::DomainSeparator cli_ds(const whir::ProtocolParameters& p) {
+    namespace tx = whir::transcript;
+    std::vector<uint8_t> m;
+    m.push_back(0xA0 + 8);
+    tx::cbor_write_text(m, "unique_decoding");
+    m.push_back(p.unique_decoding ? 0xF5 : 0xF4);
+    tx::cbor_write_text(m, "starting_log_inv_rate");
+    tx::cbor_write_uint(m, p.starting_log_inv_rate);
+    tx::cbor_write_text(m, "initial_folding_factor");
+    tx::cbor_write_uint(m, p.initial_folding_factor);
+    tx::cbor_write_text(m, "folding_factor");
+    tx::cbor_write_uint(m, p.folding_factor);
+    tx::cbor_write_text(m, "security_level");
+    tx::cbor_write_uint(m, p.security_level);
+    tx::cbor_write_text(m, "pow_bits");
+    tx::cbor_write_uint(m, p.pow_bits);
+    tx::cbor_write_text(m, "batch_size");
+    tx::cbor_write_uint(m, p.batch_size);
+    tx::cbor_write_text(m, "hash_id");
+    tx::cbor_array_header(m, p.hash_id.bytes().size());
+    for (auto b : p.hash_id.bytes()) tx::cbor_write_uint(m, b);
+    tx::DomainSeparator ds;
+    sha3_512_hash(m.data(), m.size(), ds.protocol_id.data());
+    std::vector<uint8_t> sess;
+    tx::cbor_write_text(sess, "Example at src/bin/main.rs:133");
+    sha3_256_hash(sess.data(), sess.size(), ds.session_id.data());
+    return ds;
+}
+}  // namespace
+
+extern "C" {
+
+// Flattened Config::from_params: [n_rounds | initial irs (7) | initial sumcheck size, rounds,
+// pow | skip pow | per round: irs (7), sumcheck size, rounds, pow, round pow | final sumcheck
+// size, rounds, pow | final pow].  Returns the number of words written (out may be null).
+size_t ref_whir_config(uint32_t num_variables, uint32_t log_inv_rate, uint32_t folding_factor,
+                       uint32_t security_level, uint32_t pow_bits, uint64_t* out, size_t cap) {
+    auto p = cli_params(log_inv_rate, folding_factor, security_level, pow_bits);
+    auto c = whirp::Config<Emb>::from_params(std::size_t{1} << num_variables, p);
+    std::vector<uint64_t> o;
+    o.push_back(c.round_configs.size());
+    put_irs(o, c.initial_committer);
+    o.push_back(c.initial_sumcheck.initial_size);
+    o.push_back(c.initial_sumcheck.num_rounds);
+    o.push_back(c.initial_sumcheck.round_pow.threshold_val);
+    o.push_back(c.initial_skip_pow.threshold_val);
+    for (const auto& r : c.round_configs) {
+        put_irs(o, r.irs_committer);
+        o.push_back(r.sumcheck.initial_size);
+        o.push_back(r.sumcheck.num_rounds);
+        o.push_back(r.sumcheck.round_pow.threshold_val);
+        o.push_back(r.pow.threshold_val);
+    }
+    o.push_back(c.final_sumcheck.initial_size);
+    o.push_back(c.final_sumcheck.num_rounds);
+    o.push_back(c.final_sumcheck.round_pow.threshold_val);
+    o.push_back(c.final_pow.threshold_val);
+    if (out) std::memcpy(out, o.data(), std::min(cap, o.size()) * 8);
+    return o.size();
+}
+
+// One PCS run as the CLI does it (main.cpp:292-424) with caller-given base-field
+// coefficients and Ext3 MLE constraint points (num_constraints x num_variables x 3).
+// Writes the constraint evaluations (num_constraints x 3) and the proof bytes.
+// Returns 0, or 1 when a buffer is too small (the lengths are set either way).
+int ref_whir_prove(uint32_t num_variables, uint32_t log_inv_rate, uint32_t folding_factor,
+                   uint32_t security_level, uint32_t pow_bits, const uint64_t* coeffs,
+                   size_t num_constraints, const uint64_t* points, uint64_t* out_evals, uint8_t* narg,
+                   size_t narg_cap, size_t* narg_len, uint8_t* hints, size_t hints_cap, size_t* hints_len) {
+    namespace tx = whir::transcript;
+    namespace alg = whir::algebra;
+    auto p = cli_params(log_inv_rate, folding_factor, security_level, pow_bits);
+    const std::size_t size = std::size_t{1} << num_variables;
+    auto config = whirp::Config<Emb>::from_params(size, p);
+    std::vector<Goldilocks> vector(size);
+    for (std::size_t i = 0; i < size; ++i) vector[i] = Goldilocks::from_u64(coeffs[i]);
+    std::vector<std::span<const Goldilocks>> spans{std::span<const Goldilocks>{vector}};
+    Emb emb{};
+    std::vector<std::unique_ptr<alg::LinearForm<GoldilocksExt3>>> forms;
+    std::vector<GoldilocksExt3> evaluations;
+    for (std::size_t e = 0; e < num_constraints; ++e) {
+        std::vector<GoldilocksExt3> pt(num_variables);
+        for (std::size_t k = 0; k < num_variables; ++k) {
+            const uint64_t* q = points + 3 * (e * num_variables + k);
+            pt[k] = GoldilocksExt3(Goldilocks::from_u64(q[0]), Goldilocks::from_u64(q[1]),
+                                   Goldilocks::from_u64(q[2]));
+        }
+        auto lf = std::make_unique<alg::MultilinearExtension<GoldilocksExt3>>(pt);
+        evaluations.push_back(lf->template evaluate<Emb>(emb, std::span<const Goldilocks>{vector}));
+        from_e3(evaluations.back(), out_evals + 3 * e);
+        forms.push_back(std::move(lf));
+    }
+    auto ds = cli_ds(p);
+    tx::Empty instance;
+    auto ps = tx::ProverState::from_ds(ds, instance);
+    auto witness = config.commit(ps, spans);
+    std::vector<whir::protocols::irs_commit::Witness<Goldilocks, GoldilocksExt3>> wits;
+    wits.push_back(std::move(witness));
+    config.prove(ps, spans, std::span<const whir::protocols::irs_commit::Witness<Goldilocks, GoldilocksExt3>>{wits},
+                 std::move(forms), std::span<const GoldilocksExt3>{evaluations});
+    auto proof = std::move(ps).proof();
+    *narg_len = proof.narg_string.size();
+    *hints_len = proof.hints.size();
+    if (proof.narg_string.size() > narg_cap || proof.hints.size() > hints_cap) return 1;
+    std::memcpy(narg, proof.narg_string.data(), proof.narg_string.size());
+    std::memcpy(hints, proof.hints.data(), proof.hints.size());
+    return 0;
 }
 
 }  // extern "C"
