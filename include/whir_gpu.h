@@ -200,6 +200,118 @@ int whir_dev_sumcheck_round(whir_ctx* ctx, const uint64_t* d_a, int a_is_base, c
                             const uint64_t* r, uint64_t* d_a_out, uint64_t* d_b_out, uint64_t* out_c0,
                             uint64_t* out_c2);
 
+/* ------------------------------------------------------------ prove loop */
+
+/* ProtocolParameters (parameters.hpp:25-34).  hash_id selects the Merkle and
+ * PoW hash engine; only WHIR_HASH_BLAKE3 (hash/blake3_engine.hpp:50) is
+ * served. */
+#define WHIR_HASH_BLAKE3 1u
+typedef struct whir_params {
+    int unique_decoding;
+    uint32_t starting_log_inv_rate;
+    uint32_t initial_folding_factor;
+    uint32_t folding_factor;
+    uint32_t security_level;
+    uint32_t pow_bits;
+    uint32_t batch_size;
+    uint32_t hash_id;
+} whir_params;
+
+/* The CLI's defaults (main.cpp Args: rate 2^-1, folding factor 4, 128-bit
+ * security, 20 PoW bits, one vector, BLAKE3, list decoding). */
+void whir_params_default(whir_params* params);
+
+/* whir::Config::from_params (protocols/whir/whir.hpp:118-224) for vectors of
+ * `size` coefficients over the Goldilocks cubic extension, flattened:
+ *   [n_rounds | initial irs (7) | initial sumcheck size, rounds, PoW threshold |
+ *    skip PoW threshold | per round: irs (7), sumcheck size, rounds, PoW
+ *    threshold, round PoW threshold | final sumcheck size, rounds, PoW
+ *    threshold | final PoW threshold]
+ * irs = num_vectors, vector_size, codeword_length, interleaving_depth,
+ * in_domain_samples, out_domain_samples, johnson_slack (IEEE-754 bits).
+ * Thresholds are UINT64_MAX for no PoW.  Returns the number of words (0 on
+ * invalid parameters); writes min(cap, words) words when out is non-null. */
+size_t whir_config_flat(uint64_t size, const whir_params* params, uint64_t* out, size_t cap);
+
+/* The CLI's domain separator (main.cpp:176-215): protocol_id[64] =
+ * SHA3-512(CBOR map of the parameters), session_id[32] = SHA3-256(CBOR text
+ * `session`, or the CLI's own text when session is NULL). */
+int whir_domain_separator_cli(const whir_params* params, const char* session, uint8_t* protocol_id,
+                              uint8_t* session_id);
+
+/* A prover for one proof: ProverState::from_ds (transcript.hpp:606-611) with
+ * the domain separator and instance bytes, plus the derived configuration.
+ * Work runs on ctx's stream; calls on one prover (or one ctx) are serialised. */
+typedef struct whir_prover whir_prover;
+int whir_prover_create(whir_ctx* ctx, const whir_params* params, uint64_t size, const uint8_t* protocol_id,
+                       const uint8_t* session_id, const uint8_t* instance, size_t instance_len,
+                       whir_prover** out);
+int whir_prover_destroy(whir_prover* prover);
+/* whir::Config::commit (protocols/whir/whir.hpp:304-309 ->
+ * irs_commit.hpp:328-553): d_vectors holds batch_size base-field vectors of
+ * `size` coefficients back to back on the device; it must stay valid until
+ * whir_prover_prove returns. */
+int whir_prover_commit(whir_prover* prover, const uint64_t* d_vectors, size_t num_vectors);
+/* whir::Config::prove (protocols/whir/whir_prover.hpp:42-400) with
+ * num_forms multilinear-evaluation constraints (MultilinearExtension,
+ * algebra/linear_form.hpp): points = num_forms x log2(size) Ext3 coordinates
+ * (host, AoS), evaluations = num_forms x batch_size Ext3 claimed values (host).
+ * Optional outputs: the final claim's evaluation point (log2(size) Ext3) and
+ * the constraint RLC coefficients (num_forms Ext3). */
+int whir_prover_prove(whir_prover* prover, size_t num_forms, const uint64_t* points, const uint64_t* evaluations,
+                      uint64_t* out_eval_point, uint64_t* out_forms_rlc);
+/* ProverState::proof: the narg string and the hints.  Lengths are always
+ * written; the bytes when the buffers are non-null and large enough. */
+int whir_prover_proof(whir_prover* prover, uint8_t* narg, size_t narg_cap, size_t* narg_len, uint8_t* hints,
+                      size_t hints_cap, size_t* hints_len);
+/* Number of PoW candidates the prover's searches scanned. */
+size_t whir_prover_pow_hashes(const whir_prover* prover);
+
+/* The prover transcript on its own (ProverState, transcript.hpp:595-662),
+ * host only: SHAKE-128 duplex sponge, narg string and hints.  With null ids
+ * the sponge starts empty (Shake128DuplexSponge, transcript.hpp:80-139);
+ * otherwise protocol_id[64], session_id[32] and the instance bytes are
+ * absorbed (DomainSeparator::absorb_into, transcript.hpp:531-538). */
+typedef struct whir_transcript whir_transcript;
+int whir_transcript_create(const uint8_t* protocol_id, const uint8_t* session_id, const uint8_t* instance,
+                           size_t instance_len, whir_transcript** out);
+int whir_transcript_destroy(whir_transcript* t);
+/* public_message: absorb only */
+int whir_transcript_absorb(whir_transcript* t, const uint8_t* data, size_t len);
+/* prover_message of a fixed-size value: absorb + append to narg */
+int whir_transcript_message(whir_transcript* t, const uint8_t* data, size_t len);
+/* prover_hint of a fixed-size value (raw bytes) or of a vector (8-byte LE
+ * element count, then the elements) */
+int whir_transcript_hint(whir_transcript* t, const uint8_t* data, size_t len);
+int whir_transcript_hint_vector(whir_transcript* t, const uint8_t* data, size_t count, size_t elem_bytes);
+/* raw squeeze (verifier_message of a byte array) */
+int whir_transcript_squeeze(whir_transcript* t, uint8_t* out, size_t len);
+/* verifier_message<Goldilocks>: 40 bytes, LE integer mod p */
+int whir_transcript_challenge_gl(whir_transcript* t, uint64_t* out);
+/* challenge_indices (protocols/challenge_indices.hpp:96-119); returns the
+ * number of indices written (num_leaves must be a power of two). */
+size_t whir_transcript_indices(whir_transcript* t, size_t num_leaves, size_t count, int dedup, uint64_t* out);
+int whir_transcript_proof(whir_transcript* t, uint8_t* narg, size_t narg_cap, size_t* narg_len, uint8_t* hints,
+                          size_t hints_cap, size_t* hints_len);
+
+/* find_nonce (proof_of_work.hpp:95-193): the smallest nonce whose
+ * BLAKE3(challenge[32] || LE64(nonce) || 0^24), read as LE64, is <= threshold
+ * (0 when threshold is UINT64_MAX). */
+int whir_dev_pow_search(whir_ctx* ctx, const uint8_t* challenge, uint64_t threshold, uint64_t* out_nonce);
+/* eval_eq (algebra/multilinear.hpp:120-133): d_acc[i] (+)= scalar * eq(point, i)
+ * for i < 2^num_variables, point[0] pairing with the top index bit; Ext3 AoS.
+ * accumulate = 0 overwrites. */
+int whir_dev_eq_accumulate(whir_ctx* ctx, uint64_t* d_acc, size_t num_variables, const uint64_t* point,
+                           const uint64_t* scalar, int accumulate);
+/* geometric_accumulate (algebra/utilities.hpp:260-307): d_acc[i] +=
+ * sum_j scalars[j] * points[j]^i, i < n (a power of two); Ext3 AoS. */
+int whir_dev_geometric_accumulate(whir_ctx* ctx, uint64_t* d_acc, size_t n, const uint64_t* points,
+                                  const uint64_t* scalars, size_t count);
+/* mixed_multilinear_extend (algebra/multilinear.hpp:41-103): the multilinear
+ * extension of d_f (2^num_variables entries, base or Ext3) at point. */
+int whir_dev_mle_evaluate(whir_ctx* ctx, const uint64_t* d_f, int f_is_base, size_t num_variables,
+                          const uint64_t* point, uint64_t* out);
+
 /* --------------------------------------------------------- stage timing */
 
 /* Stage timing with HIP events on the context's stream (the stream the

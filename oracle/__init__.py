@@ -94,6 +94,11 @@ def lib():
         L.or_horner_base.restype = _u64
         L.or_horner_base.argtypes = [_u64p, _sz, _u64]
         L.or_lcg_fill.argtypes = [_u64p, _u64p, _sz, ctypes.c_int]
+        L.or_eval_eq_ext3.argtypes = [_u64p, _u64p, _sz, _u64p]
+        L.or_geometric_accumulate_ext3.argtypes = [_u64p, _sz, _u64p, _u64p, _sz]
+        L.or_mle_evaluate.argtypes = [_u64p, ctypes.c_int, _u64p, _sz, _u64p]
+        L.or_pow_find.restype = _u64
+        L.or_pow_find.argtypes = [_u8p, _u64]
         L.or_sha3_256.argtypes = [_u8p, _sz, _u8p]
         L.or_sha3_512.argtypes = [_u8p, _sz, _u8p]
         L.or_shake128.argtypes = [_u8p, _sz, _u8p, _sz]
@@ -120,6 +125,13 @@ def ref():
         R.ref_horner_base_ext3.argtypes = [_u64p, _sz, _u64p, _u64p]
         R.ref_commit_base.argtypes = [_u64p, _sz, _sz, _sz, _u64p, _u8p]
         R.ref_blake3_many.argtypes = [_sz, _u8p, _sz, _u8p]
+        R.ref_whir_config.restype = _sz
+        R.ref_whir_config.argtypes = [ctypes.c_uint32] * 5 + [_u64p, _sz]
+        R.ref_whir_prove_params.restype = ctypes.c_int
+        R.ref_whir_prove_params.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, _u64p, _sz, _u64p,
+                                            _u64p, _u8p, _sz, ctypes.POINTER(_sz), _u8p, _sz, ctypes.POINTER(_sz)]
+        R.ref_cli_ds.restype = None
+        R.ref_cli_ds.argtypes = [ctypes.POINTER(ctypes.c_uint32), _u8p, _u8p]
         _ref = R
     return _ref
 
@@ -327,3 +339,74 @@ def shake128(data: bytes, n: int) -> bytes:
     out = np.zeros(max(n, 1), dtype=np.uint8)
     lib().or_shake128(_ptr8(buf), len(data), _ptr8(out), n)
     return out[:n].tobytes()
+
+
+# ------------------------------------------------------------ prove path
+
+def eval_eq_ext3(acc, point, scalar) -> np.ndarray:
+    a = u64(acc).copy()
+    pt = u64(point).reshape(-1)
+    sc = u64(scalar)
+    lib().or_eval_eq_ext3(_ptr64(a), _ptr64(pt), pt.size // 3, _ptr64(sc))
+    return a
+
+
+def geometric_accumulate_ext3(acc, points, scalars) -> np.ndarray:
+    a = u64(acc).copy()
+    pts, scs = u64(points).reshape(-1), u64(scalars).reshape(-1)
+    lib().or_geometric_accumulate_ext3(_ptr64(a), a.size // 3, _ptr64(pts), _ptr64(scs), pts.size // 3)
+    return a
+
+
+def mle_evaluate(evals, point, is_base: bool) -> np.ndarray:
+    e = u64(evals)
+    pt = u64(point).reshape(-1)
+    out = np.zeros(3, dtype=np.uint64)
+    lib().or_mle_evaluate(_ptr64(e), int(is_base), _ptr64(pt), pt.size // 3, _ptr64(out))
+    return out
+
+
+def pow_find(challenge: bytes, threshold: int) -> int:
+    ch = np.frombuffer(bytes(challenge), dtype=np.uint8).copy()
+    return int(lib().or_pow_find(_ptr8(ch), threshold))
+
+
+def _prm(unique=0, rate=1, iff=4, ff=4, sec=128, pow_bits=20, batch=1):
+    return (ctypes.c_uint32 * 7)(unique, rate, iff, ff, sec, pow_bits, batch)
+
+
+def ref_config(nv: int, rate: int, ff: int, sec: int, pow_bits: int) -> np.ndarray:
+    R = ref()
+    n = R.ref_whir_config(nv, rate, ff, sec, pow_bits, None, 0)
+    out = np.zeros(n, dtype=np.uint64)
+    R.ref_whir_config(nv, rate, ff, sec, pow_bits, _ptr64(out), n)
+    return out
+
+
+def ref_cli_ds(**prm) -> tuple[bytes, bytes]:
+    pid = np.zeros(64, dtype=np.uint8)
+    sid = np.zeros(32, dtype=np.uint8)
+    ref().ref_cli_ds(_prm(**prm), _ptr8(pid), _ptr8(sid))
+    return pid.tobytes(), sid.tobytes()
+
+
+def ref_prove(nv: int, coeffs, points, **prm):
+    """The reference's own prover (compiled from its sources): the CLI's PCS
+    flow with caller-given coefficients (batch x 2^nv) and MLE points
+    (num_constraints x nv x 3).  Returns (evaluations, narg, hints)."""
+    R = ref()
+    c = u64(coeffs)
+    pts = u64(points).reshape(-1)
+    nf = pts.size // (3 * nv) if nv else 0
+    batch = prm.get("batch", 1)
+    ev = np.zeros(3 * max(nf * batch, 1), dtype=np.uint64)
+    cap_n, cap_h = 1 << 16, 1 << 24
+    while True:
+        narg = np.zeros(cap_n, dtype=np.uint8)
+        hints = np.zeros(cap_h, dtype=np.uint8)
+        nl, hl = _sz(), _sz()
+        rc = R.ref_whir_prove_params(_prm(**prm), nv, _ptr64(c), nf, _ptr64(pts) if nf else None, _ptr64(ev),
+                                     _ptr8(narg), cap_n, ctypes.byref(nl), _ptr8(hints), cap_h, ctypes.byref(hl))
+        if rc == 0:
+            return ev[:3 * nf * batch].reshape(-1, 3), narg[:nl.value].tobytes(), hints[:hl.value].tobytes()
+        cap_n, cap_h = max(cap_n, nl.value), max(cap_h, hl.value)

@@ -18,7 +18,7 @@
 //                          without the transcript
 //   ref_blake3_many        hash/blake3_engine.hpp:90-199 Blake3::hash_many (both branches)
 //   ref_whir_config        protocols/whir/whir.hpp:118-224 Config::from_params, flattened
-//   ref_whir_prove         the main.cpp:292-424 PCS flow: from_params, the CLI's domain
+//   ref_whir_prove(_params) the main.cpp:292-424 PCS flow: from_params, the CLI's domain
 //                          separator (main.cpp:176-215), commit, MLE constraints, prove
 #include "whir/algebra/embedding.hpp"
 #include "whir/algebra/goldilocks.hpp"
@@ -310,18 +310,34 @@ size_t ref_whir_config(uint32_t num_variables, uint32_t log_inv_rate, uint32_t f
 // coefficients and Ext3 MLE constraint points (num_constraints x num_variables x 3).
 // Writes the constraint evaluations (num_constraints x 3) and the proof bytes.
 // Returns 0, or 1 when a buffer is too small (the lengths are set either way).
-int ref_whir_prove(uint32_t num_variables, uint32_t log_inv_rate, uint32_t folding_factor,
-                   uint32_t security_level, uint32_t pow_bits, const uint64_t* coeffs,
-                   size_t num_constraints, const uint64_t* points, uint64_t* out_evals, uint8_t* narg,
-                   size_t narg_cap, size_t* narg_len, uint8_t* hints, size_t hints_cap, size_t* hints_len) {
+// General form: prm = {unique_decoding, starting_log_inv_rate, initial_folding_factor,
+// folding_factor, security_level, pow_bits, batch_size}; coeffs holds batch_size vectors
+// of 2^num_variables base-field coefficients; out_evals gets num_constraints x batch_size
+// Ext3 values (constraint-major).  The domain separator is the CLI's for these parameters.
+int ref_whir_prove_params(const uint32_t* prm, uint32_t num_variables, const uint64_t* coeffs,
+                          size_t num_constraints, const uint64_t* points, uint64_t* out_evals, uint8_t* narg,
+                          size_t narg_cap, size_t* narg_len, uint8_t* hints, size_t hints_cap,
+                          size_t* hints_len) {
     namespace tx = whir::transcript;
     namespace alg = whir::algebra;
-    auto p = cli_params(log_inv_rate, folding_factor, security_level, pow_bits);
+    whir::ProtocolParameters p;
+    p.unique_decoding = prm[0] != 0;
+    p.starting_log_inv_rate = prm[1];
+    p.initial_folding_factor = prm[2];
+    p.folding_factor = prm[3];
+    p.security_level = prm[4];
+    p.pow_bits = prm[5];
+    p.batch_size = prm[6];
+    p.hash_id = whir::hash::ENGINE_ID_BLAKE3;
     const std::size_t size = std::size_t{1} << num_variables;
+    const std::size_t nvec = p.batch_size;
     auto config = whirp::Config<Emb>::from_params(size, p);
-    std::vector<Goldilocks> vector(size);
-    for (std::size_t i = 0; i < size; ++i) vector[i] = Goldilocks::from_u64(coeffs[i]);
-    std::vector<std::span<const Goldilocks>> spans{std::span<const Goldilocks>{vector}};
+    std::vector<std::vector<Goldilocks>> vecs(nvec, std::vector<Goldilocks>(size));
+    std::vector<std::span<const Goldilocks>> spans;
+    for (std::size_t v = 0; v < nvec; ++v) {
+        for (std::size_t i = 0; i < size; ++i) vecs[v][i] = Goldilocks::from_u64(coeffs[v * size + i]);
+        spans.push_back(std::span<const Goldilocks>{vecs[v]});
+    }
     Emb emb{};
     std::vector<std::unique_ptr<alg::LinearForm<GoldilocksExt3>>> forms;
     std::vector<GoldilocksExt3> evaluations;
@@ -333,8 +349,10 @@ int ref_whir_prove(uint32_t num_variables, uint32_t log_inv_rate, uint32_t foldi
                                    Goldilocks::from_u64(q[2]));
         }
         auto lf = std::make_unique<alg::MultilinearExtension<GoldilocksExt3>>(pt);
-        evaluations.push_back(lf->template evaluate<Emb>(emb, std::span<const Goldilocks>{vector}));
-        from_e3(evaluations.back(), out_evals + 3 * e);
+        for (std::size_t v = 0; v < nvec; ++v) {
+            evaluations.push_back(lf->template evaluate<Emb>(emb, spans[v]));
+            from_e3(evaluations.back(), out_evals + 3 * (e * nvec + v));
+        }
         forms.push_back(std::move(lf));
     }
     auto ds = cli_ds(p);
@@ -352,6 +370,31 @@ int ref_whir_prove(uint32_t num_variables, uint32_t log_inv_rate, uint32_t foldi
     std::memcpy(narg, proof.narg_string.data(), proof.narg_string.size());
     std::memcpy(hints, proof.hints.data(), proof.hints.size());
     return 0;
+}
+
+// The CLI domain separator for prm (layout as above): protocol_id[64], session_id[32].
+void ref_cli_ds(const uint32_t* prm, uint8_t* protocol_id, uint8_t* session_id) {
+    whir::ProtocolParameters p;
+    p.unique_decoding = prm[0] != 0;
+    p.starting_log_inv_rate = prm[1];
+    p.initial_folding_factor = prm[2];
+    p.folding_factor = prm[3];
+    p.security_level = prm[4];
+    p.pow_bits = prm[5];
+    p.batch_size = prm[6];
+    p.hash_id = whir::hash::ENGINE_ID_BLAKE3;
+    auto ds = cli_ds(p);
+    std::memcpy(protocol_id, ds.protocol_id.data(), 64);
+    std::memcpy(session_id, ds.session_id.data(), 32);
+}
+
+int ref_whir_prove(uint32_t num_variables, uint32_t log_inv_rate, uint32_t folding_factor,
+                   uint32_t security_level, uint32_t pow_bits, const uint64_t* coeffs,
+                   size_t num_constraints, const uint64_t* points, uint64_t* out_evals, uint8_t* narg,
+                   size_t narg_cap, size_t* narg_len, uint8_t* hints, size_t hints_cap, size_t* hints_len) {
+    uint32_t prm[7] = {0, log_inv_rate, folding_factor, folding_factor, security_level, pow_bits, 1};
+    return ref_whir_prove_params(prm, num_variables, coeffs, num_constraints, points, out_evals, narg, narg_cap,
+                                 narg_len, hints, hints_cap, hints_len);
 }
 
 }  // extern "C"

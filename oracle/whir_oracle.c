@@ -510,6 +510,70 @@ uint64_t or_horner_base(const uint64_t* c, size_t n, uint64_t z) {
     return acc;
 }
 
+/* ------------------------------------------------------ prove-path algebra */
+
+void or_eval_eq_ext3(uint64_t* acc, const uint64_t* point, size_t k, const uint64_t scalar[3]) {
+    /* multilinear.hpp:120-133 */
+    if (k == 0) {
+        or_e3_add(acc, scalar, acc);
+        return;
+    }
+    uint64_t s1[3], s0[3];
+    or_e3_mul(scalar, point, s1);
+    or_e3_sub(scalar, s1, s0);
+    size_t half = (size_t)1 << (k - 1);
+    or_eval_eq_ext3(acc, point + 3, k - 1, s0);
+    or_eval_eq_ext3(acc + 3 * half, point + 3, k - 1, s1);
+}
+
+void or_geometric_accumulate_ext3(uint64_t* acc, size_t n, const uint64_t* points, const uint64_t* scalars,
+                                  size_t count) {
+    /* utilities.hpp:300-306: entry += s_j, then s_j *= p_j, for each entry in order */
+    uint64_t* s = (uint64_t*)malloc(3 * (count ? count : 1) * sizeof(uint64_t));
+    memcpy(s, scalars, 3 * count * sizeof(uint64_t));
+    for (size_t i = 0; i < n; ++i)
+        for (size_t j = 0; j < count; ++j) {
+            or_e3_add(acc + 3 * i, s + 3 * j, acc + 3 * i);
+            or_e3_mul(s + 3 * j, points + 3 * j, s + 3 * j);
+        }
+    free(s);
+}
+
+void or_mle_evaluate(const uint64_t* evals, int is_base, const uint64_t* point, size_t k, uint64_t out[3]) {
+    /* multilinear.hpp:93-101: f0 + (f1 - f0) * point[0] over the two halves */
+    if (k == 0) {
+        if (is_base) {
+            out[0] = evals[0];
+            out[1] = out[2] = 0;
+        } else {
+            memcpy(out, evals, 3 * sizeof(uint64_t));
+        }
+        return;
+    }
+    size_t half = (size_t)1 << (k - 1);
+    uint64_t f0[3], f1[3], d[3];
+    or_mle_evaluate(evals, is_base, point + 3, k - 1, f0);
+    or_mle_evaluate(evals + (is_base ? 1 : 3) * half, is_base, point + 3, k - 1, f1);
+    or_e3_sub(f1, f0, d);
+    or_e3_mul(d, point, d);
+    or_e3_add(f0, d, out);
+}
+
+uint64_t or_pow_find(const uint8_t ch[32], uint64_t threshold) {
+    /* proof_of_work.hpp:95-193 */
+    if (threshold == UINT64_MAX) return 0;
+    uint8_t in[64], h[32];
+    memset(in, 0, sizeof in);
+    memcpy(in, ch, 32);
+    for (uint64_t n = 0;; ++n) {
+        for (int b = 0; b < 8; ++b) in[32 + b] = (uint8_t)(n >> (8 * b));
+        or_blake3(in, 64, h);
+        uint64_t v = 0;
+        for (int b = 7; b >= 0; --b) v = (v << 8) | h[b];
+        if (v <= threshold) return n;
+    }
+}
+
 /* ------------------------------------------------------------- seeded LCG */
 
 void or_lcg_fill(uint64_t* state, uint64_t* out, size_t n, int mode) {

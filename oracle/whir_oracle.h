@@ -113,6 +113,21 @@ void or_horner_base_ext3(const uint64_t* coeffs, size_t n, const uint64_t point[
 void or_horner_ext3_ext3(const uint64_t* coeffs, size_t n, const uint64_t point[3], uint64_t out[3]);
 uint64_t or_horner_base(const uint64_t* coeffs, size_t n, uint64_t point);
 
+/* ---- prove-path algebra over Ext3 (AoS) ---- */
+/* eval_eq, algebra/multilinear.hpp:120-133 (recursive halving: point[0] splits
+ * the accumulator's top half); acc has 2^k entries */
+void or_eval_eq_ext3(uint64_t* acc, const uint64_t* point, size_t k, const uint64_t scalar[3]);
+/* geometric_accumulate, algebra/utilities.hpp:260-307 (serial branch :300-306):
+ * acc[i] += sum_j scalars[j] * points[j]^i */
+void or_geometric_accumulate_ext3(uint64_t* acc, size_t n, const uint64_t* points, const uint64_t* scalars,
+                                  size_t count);
+/* mixed_multilinear_extend, algebra/multilinear.hpp:41-103 (the recursive
+ * default branch for every k): evals base (is_base) or Ext3, 2^k entries */
+void or_mle_evaluate(const uint64_t* evals, int is_base, const uint64_t* point, size_t k, uint64_t out[3]);
+/* find_nonce, protocols/proof_of_work.hpp:95-193 (serial scan): the first n
+ * with LE64(BLAKE3(ch || LE64(n) || 0^24)[0..8]) <= threshold; UINT64_MAX -> 0 */
+uint64_t or_pow_find(const uint8_t ch[32], uint64_t threshold);
+
 /* ---- seeded inputs: the LCG of cpp/tests/dump_*.cpp (e.g. dump_whir.cpp:43-50) ----
  * x <- x*6364136223846793005 + 1442695040888963407.  mode 0: from_u64 (x>=p -> x-p,
  * as the reference's dumpers); mode 1: reject x >= p (exactly uniform). */

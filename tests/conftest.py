@@ -25,3 +25,10 @@ def orc():
 
     oracle.lib()
     return oracle
+
+
+@pytest.fixture(scope="session")
+def W():
+    import whirplonk_amd
+    whirplonk_amd.load()
+    return whirplonk_amd

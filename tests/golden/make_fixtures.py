@@ -160,6 +160,29 @@ def utils(ref: str) -> dict:
     x = rng.gl()
     uni = kv(cases(txt, "utilities")[3])
     out = {"univariate_evaluate": {"coeffs": coeffs, "x": x, "result": int(uni["result"][0])}}
+    [rng.gl() for _ in range(17)]              # scalar_mul_add acc, w, vec
+    acc2 = [rng.gl() for _ in range(8)]
+    scal = [rng.gl() for _ in range(3)]
+    pts = [rng.gl() for _ in range(3)]
+    ga = cases(txt, "utilities")[5]
+    out["geometric_accumulate"] = {"acc": acc2, "scalars": scal, "points": pts,
+                                   "result": [int(v) for v in ga[2:]]}
+
+    # SECTION multilinear (LCG 0x2222...): multilinear_extend k=3,4,5, eval_eq k=3,4
+    rng = Lcg(0x2222222222222222)
+    ml = cases(txt, "multilinear")
+    ext = []
+    for i, k in enumerate((3, 4, 5)):
+        ev = [rng.gl() for _ in range(1 << k)]
+        pt = [rng.gl() for _ in range(k)]
+        ext.append({"evals": ev, "point": pt, "result": int(kv(ml[i])["result"][0])})
+    eqs = []
+    for i, k in ((3, 3), (4, 4)):
+        pt = [rng.gl() for _ in range(k)]
+        sc = rng.gl()
+        eqs.append({"point": pt, "scalar": sc, "result": [int(v) for v in ml[i][2:]]})
+    out["multilinear_extend"] = ext
+    out["eval_eq"] = eqs
 
     rng = Lcg(0x3333333333333333)
     sc = cases(txt, "sumcheck")
@@ -196,6 +219,62 @@ def sumcheck(ref: str) -> dict:
             "final_a": int(d["final_a"][0]), "final_b": int(d["final_b"][0])}
 
 
+# -------------------------------------------------------- transcript, PoW
+def _cbor_text(t: str) -> bytes:
+    assert len(t) < 24
+    return bytes([0x60 + len(t)]) + t.encode()
+
+
+def _cbor_test_config() -> bytes:
+    """The CBOR map the dumper hashes in CASE 1: {version: 1, field_size: p,
+    security_bits: 100}."""
+    return (b"\xa3" + _cbor_text("version") + b"\x01" + _cbor_text("field_size") + b"\x1b"
+            + P.to_bytes(8, "big") + _cbor_text("security_bits") + b"\x18\x64")
+
+
+def transcript(ref: str) -> dict:
+    """golden_transcript (UTF-16LE; cpp/tests/dump_transcript.cpp).  CASE 0 and 4:
+    sponge absorb/squeeze of the dumper's ASCII inputs; CASE 1: SHA3-512/256 of the
+    dumper's CBOR bytes; CASE 2: a prover round trip under DomainSeparator
+    protocol(uint32 0xDEADBEEF).session("round_trip") with instance u64 1024.
+    CASE 3 (vector framing) is stale against the current serializer, which writes
+    an 8-byte element count (transcript.hpp:300-307), and is not used."""
+    txt = read(ref, "transcript_output/golden_transcript_cpp.txt", utf16=True)
+    cs = [kv(c) for c in cases(txt, "transcript")]
+    one = lambda d, k: d[k][0]
+    return {
+        "sponge": {"absorb": ["hello", "world"], "squeeze0": one(cs[0], "squeeze0"),
+                   "squeeze1": one(cs[0], "squeeze1"), "squeeze2_16b": one(cs[0], "squeeze2_16b")},
+        "sha3": {"protocol_cbor": _cbor_test_config().hex(),
+                 "protocol_id": one(cs[1], "protocol_id_h1") + one(cs[1], "protocol_id_h2"),
+                 "session_cbor": _cbor_text("test_session_42").hex(), "session_id": one(cs[1], "session_id")},
+        "round_trip": {"protocol_cbor": "1adeadbeef", "session": "round_trip", "instance_u64": 1024,
+                       "challenges": [one(cs[2], f"challenge{i}") for i in range(3)],
+                       "narg": one(cs[2], "proof_narg"), "hints": one(cs[2], "proof_hints")},
+        "xof": {"absorb": "test_xof_streaming", "once_40": one(cs[4], "squeeze_40b_once"),
+                "first_20": one(cs[4], "squeeze_20b_first"), "second_20": one(cs[4], "squeeze_20b_second")},
+    }
+
+
+def pow_and_indices(ref: str) -> dict:
+    """golden_pow (cpp/tests/dump_pow.cpp: challenge = 32 bytes of the case's fill
+    byte, threshold 2^60) and golden_indices (cpp/tests/dump_challenge_indices.cpp:
+    the dumper's entropy bytes per case)."""
+    txt = read(ref, "pow_output/golden_pow_cpp.txt")
+    pc = cases(txt, "pow")
+    b3 = kv(pc[0])
+    pw = {"fill": 0xAA, "threshold": 1 << 60, "nonce": int(b3["nonce"][0]), "hash": b3["hash"][0]}
+    txt = read(ref, "indices_output/golden_indices_cpp.txt")
+    ent = [(128, 5, True, "0123456789"), (128, 5, False, "0123456789"),
+           (8192, 5, True, "0123456789abcdef1234"), (1 << 20, 4, True, "123456789abcdef0112233 44".replace(" ", "")),
+           (128, 5, True, "2040206040"), (1, 3, True, ""), (1, 3, False, ""), (8, 0, True, "")]
+    idx = []
+    for (n, k, d, e), c in zip(ent, cases(txt, "challenge_indices")):
+        got = c[1].split()[1:] if len(c) > 1 else []
+        idx.append({"num_leaves": n, "count": k, "dedup": d, "entropy": e, "indices": [int(v) for v in got]})
+    return {"pow": pw, "indices": idx}
+
+
 # -------------------------------------------------------------------- whir
 def whir(ref: str) -> list:
     """golden_whir (cpp/tests/dump_whir.cpp): LCG(0xCAFE) shared across cases
@@ -227,6 +306,8 @@ def main() -> None:
         "utils": utils(ref),
         "sumcheck": sumcheck(ref),
         "whir": whir(ref),
+        "transcript": transcript(ref),
+        "pow_indices": pow_and_indices(ref),
     }
     path = os.path.join(HERE, "fixtures.json")
     with open(path, "w") as f:

@@ -87,6 +87,29 @@ _SIGS = [
     ("whir_dev_sumcheck_fold", _int, [_vp, _vp, _int, _sz, _u64p, _vp]),
     ("whir_dev_horner", _int, [_vp, _vp, _int, _sz, _u64p, _u64p]),
     ("whir_dev_sumcheck_round", _int, [_vp, _vp, _int, _vp, _sz, _u64p, _vp, _vp, _u64p, _u64p]),
+    ("whir_params_default", None, [_vp]),
+    ("whir_config_flat", _sz, [ctypes.c_uint64, _vp, _u64p, _sz]),
+    ("whir_domain_separator_cli", _int, [_vp, ctypes.c_char_p, _u8p, _u8p]),
+    ("whir_prover_create", _int, [_vp, _vp, ctypes.c_uint64, _u8p, _u8p, _u8p, _sz, ctypes.POINTER(_vp)]),
+    ("whir_prover_destroy", _int, [_vp]),
+    ("whir_prover_commit", _int, [_vp, _vp, _sz]),
+    ("whir_prover_prove", _int, [_vp, _sz, _u64p, _u64p, _u64p, _u64p]),
+    ("whir_prover_proof", _int, [_vp, _u8p, _sz, ctypes.POINTER(_sz), _u8p, _sz, ctypes.POINTER(_sz)]),
+    ("whir_prover_pow_hashes", _sz, [_vp]),
+    ("whir_transcript_create", _int, [_u8p, _u8p, _u8p, _sz, ctypes.POINTER(_vp)]),
+    ("whir_transcript_destroy", _int, [_vp]),
+    ("whir_transcript_absorb", _int, [_vp, _u8p, _sz]),
+    ("whir_transcript_message", _int, [_vp, _u8p, _sz]),
+    ("whir_transcript_hint", _int, [_vp, _u8p, _sz]),
+    ("whir_transcript_hint_vector", _int, [_vp, _u8p, _sz, _sz]),
+    ("whir_transcript_squeeze", _int, [_vp, _u8p, _sz]),
+    ("whir_transcript_challenge_gl", _int, [_vp, _u64p]),
+    ("whir_transcript_indices", _sz, [_vp, _sz, _sz, _int, _u64p]),
+    ("whir_transcript_proof", _int, [_vp, _u8p, _sz, ctypes.POINTER(_sz), _u8p, _sz, ctypes.POINTER(_sz)]),
+    ("whir_dev_pow_search", _int, [_vp, _u8p, ctypes.c_uint64, _u64p]),
+    ("whir_dev_eq_accumulate", _int, [_vp, _vp, _sz, _u64p, _u64p, _int]),
+    ("whir_dev_geometric_accumulate", _int, [_vp, _vp, _sz, _u64p, _u64p, _sz]),
+    ("whir_dev_mle_evaluate", _int, [_vp, _vp, _int, _sz, _u64p, _u64p]),
     ("whir_ctx_enable_timing", _int, [_vp, _int]),
     ("whir_ctx_reset_timing", _int, [_vp]),
     ("whir_ctx_stage_times", _int, [_vp, ctypes.POINTER(ctypes.c_double), _u64p]),
@@ -355,6 +378,187 @@ class Context:
         out = np.empty(3, dtype=np.uint64)
         _check(load().whir_dev_horner(self.h, d_f, int(f_is_ext3), n, _p64(zz), _p64(out)))
         return out
+
+
+    # ---- prove-path building blocks
+    def pow_search(self, challenge: bytes, threshold: int) -> int:
+        ch = np.frombuffer(bytes(challenge), dtype=np.uint8).copy()
+        out = np.zeros(1, dtype=np.uint64)
+        _check(load().whir_dev_pow_search(self.h, _p8(ch), threshold, _p64(out)))
+        return int(out[0])
+
+    def eq_accumulate(self, d_acc, point, scalar=(1, 0, 0), accumulate=True):
+        pt = _u64(point).reshape(-1)
+        sc = _u64(scalar)
+        _check(load().whir_dev_eq_accumulate(self.h, d_acc, pt.size // 3, _p64(pt), _p64(sc), int(accumulate)))
+
+    def geometric_accumulate(self, d_acc, n, points, scalars):
+        pts, scs = _u64(points).reshape(-1), _u64(scalars).reshape(-1)
+        _check(load().whir_dev_geometric_accumulate(self.h, d_acc, n, _p64(pts), _p64(scs), pts.size // 3))
+
+    def mle_evaluate(self, d_f, f_is_base, point) -> np.ndarray:
+        pt = _u64(point).reshape(-1)
+        out = np.empty(3, dtype=np.uint64)
+        _check(load().whir_dev_mle_evaluate(self.h, d_f, int(f_is_base), pt.size // 3, _p64(pt), _p64(out)))
+        return out
+
+
+# ------------------------------------------------------------ prove loop
+
+class Params(ctypes.Structure):
+    """whir_params (ProtocolParameters, parameters.hpp:25-34)."""
+    _fields_ = [
+        ("unique_decoding", ctypes.c_int),
+        ("starting_log_inv_rate", ctypes.c_uint32),
+        ("initial_folding_factor", ctypes.c_uint32),
+        ("folding_factor", ctypes.c_uint32),
+        ("security_level", ctypes.c_uint32),
+        ("pow_bits", ctypes.c_uint32),
+        ("batch_size", ctypes.c_uint32),
+        ("hash_id", ctypes.c_uint32),
+    ]
+
+    @classmethod
+    def default(cls, **kw) -> "Params":
+        p = cls()
+        load().whir_params_default(ctypes.byref(p))
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+
+def config_flat(size: int, params: Params) -> np.ndarray:
+    """whir::Config::from_params flattened (layout in include/whir_gpu.h)."""
+    n = load().whir_config_flat(size, ctypes.byref(params), None, 0)
+    if n == 0:
+        raise WhirGpuError(3, "invalid parameters")
+    out = np.zeros(n, dtype=np.uint64)
+    load().whir_config_flat(size, ctypes.byref(params), _p64(out), n)
+    return out
+
+
+def domain_separator_cli(params: Params, session: str | None = None) -> tuple[bytes, bytes]:
+    pid = np.zeros(64, dtype=np.uint8)
+    sid = np.zeros(32, dtype=np.uint8)
+    _check(load().whir_domain_separator_cli(ctypes.byref(params), session.encode() if session else None,
+                                            _p8(pid), _p8(sid)))
+    return pid.tobytes(), sid.tobytes()
+
+
+def _buf(b: bytes) -> np.ndarray:
+    return np.frombuffer(bytes(b), dtype=np.uint8).copy() if b else np.zeros(1, dtype=np.uint8)
+
+
+class Transcript:
+    """The host prover transcript (whir_transcript_*), for host-side checks."""
+
+    def __init__(self, protocol_id: bytes | None = None, session_id: bytes | None = None, instance: bytes = b""):
+        h = _vp()
+        pid = _buf(protocol_id) if protocol_id is not None else None
+        sid = _buf(session_id) if session_id is not None else None
+        ins = _buf(instance)
+        _check(load().whir_transcript_create(_p8(pid) if pid is not None else None,
+                                             _p8(sid) if sid is not None else None, _p8(ins), len(instance),
+                                             ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().whir_transcript_destroy(self.h)
+            self.h = None
+
+    def absorb(self, data: bytes):
+        b = _buf(data)
+        _check(load().whir_transcript_absorb(self.h, _p8(b), len(data)))
+
+    def message(self, data: bytes):
+        b = _buf(data)
+        _check(load().whir_transcript_message(self.h, _p8(b), len(data)))
+
+    def hint(self, data: bytes):
+        b = _buf(data)
+        _check(load().whir_transcript_hint(self.h, _p8(b), len(data)))
+
+    def hint_vector(self, data: bytes, count: int, elem_bytes: int):
+        b = _buf(data)
+        _check(load().whir_transcript_hint_vector(self.h, _p8(b), count, elem_bytes))
+
+    def squeeze(self, n: int) -> bytes:
+        out = np.zeros(max(n, 1), dtype=np.uint8)
+        _check(load().whir_transcript_squeeze(self.h, _p8(out), n))
+        return out[:n].tobytes()
+
+    def challenge_gl(self) -> int:
+        out = np.zeros(1, dtype=np.uint64)
+        _check(load().whir_transcript_challenge_gl(self.h, _p64(out)))
+        return int(out[0])
+
+    def indices(self, num_leaves: int, count: int, dedup: bool) -> list[int]:
+        out = np.zeros(max(count, 1), dtype=np.uint64)
+        k = load().whir_transcript_indices(self.h, num_leaves, count, int(dedup), _p64(out))
+        return [int(x) for x in out[:k]]
+
+    def proof(self) -> tuple[bytes, bytes]:
+        na, nh = _sz(), _sz()
+        _check(load().whir_transcript_proof(self.h, None, 0, ctypes.byref(na), None, 0, ctypes.byref(nh)))
+        a = np.zeros(max(na.value, 1), dtype=np.uint8)
+        b = np.zeros(max(nh.value, 1), dtype=np.uint8)
+        _check(load().whir_transcript_proof(self.h, _p8(a), a.size, ctypes.byref(na), _p8(b), b.size,
+                                            ctypes.byref(nh)))
+        return a[:na.value].tobytes(), b[:nh.value].tobytes()
+
+
+class Prover:
+    """One proof: whir_prover_create / commit / prove / proof."""
+
+    def __init__(self, ctx: Context, params: Params, size: int, protocol_id: bytes, session_id: bytes,
+                 instance: bytes = b""):
+        self.ctx = ctx
+        self.size = size
+        pid = np.frombuffer(protocol_id, dtype=np.uint8).copy()
+        sid = np.frombuffer(session_id, dtype=np.uint8).copy()
+        ins = np.frombuffer(instance, dtype=np.uint8).copy() if instance else np.zeros(1, dtype=np.uint8)
+        h = _vp()
+        _check(load().whir_prover_create(ctx.h, ctypes.byref(params), size, _p8(pid), _p8(sid), _p8(ins),
+                                         len(instance), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            load().whir_prover_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def commit(self, d_vectors: int, num_vectors: int = 1):
+        _check(load().whir_prover_commit(self.h, d_vectors, num_vectors))
+
+    def prove(self, points, evaluations):
+        """points: (num_forms, num_variables, 3); evaluations: (num_forms * num_vectors, 3)."""
+        pts = _u64(points).reshape(-1)
+        ev = _u64(evaluations).reshape(-1)
+        nv = self.size.bit_length() - 1
+        nf = pts.size // (3 * nv) if nv else 0
+        ep = np.zeros(3 * nv, dtype=np.uint64)
+        rl = np.zeros(3 * max(nf, 1), dtype=np.uint64)
+        _check(load().whir_prover_prove(self.h, nf, _p64(pts) if nf else None, _p64(ev) if nf else None,
+                                        _p64(ep), _p64(rl)))
+        return ep.reshape(-1, 3), rl[:3 * nf].reshape(-1, 3)
+
+    def proof(self) -> tuple[bytes, bytes]:
+        na, nh = _sz(), _sz()
+        _check(load().whir_prover_proof(self.h, None, 0, ctypes.byref(na), None, 0, ctypes.byref(nh)))
+        a = np.zeros(max(na.value, 1), dtype=np.uint8)
+        b = np.zeros(max(nh.value, 1), dtype=np.uint8)
+        _check(load().whir_prover_proof(self.h, _p8(a), a.size, ctypes.byref(na), _p8(b), b.size, ctypes.byref(nh)))
+        return a[:na.value].tobytes(), b[:nh.value].tobytes()
+
+    def pow_hashes(self) -> int:
+        return load().whir_prover_pow_hashes(self.h)
 
 
 def merkle_num_nodes(n: int) -> int:

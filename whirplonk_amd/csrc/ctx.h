@@ -1,0 +1,152 @@
+// ctx.h — the library's context object and error helpers (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gl.h"
+#include "kernels.h"
+
+namespace whir_gpu {
+
+std::string& last_error();  // per thread
+
+inline int fail(const char* what, hipError_t e) {
+    last_error() = std::string(what) + ": " + hipGetErrorString(e);
+    return -1;
+}
+
+inline int bad_arg(const char* what) {
+    last_error() = what;
+    return 3;
+}
+
+inline bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
+inline uint32_t log2u(uint64_t x) {
+    uint32_t k = 0;
+    while ((1ull << k) < x) ++k;
+    return k;
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace whir_gpu
+
+#define HIPCHK(expr)                                              \
+    do {                                                          \
+        hipError_t e_ = (expr);                                   \
+        if (e_ != hipSuccess) return ::whir_gpu::fail(#expr, e_); \
+    } while (0)
+
+struct whir_ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    std::recursive_mutex mu;
+    whir_gpu::DevBuf scratch, partials, zpow, hin, hout, hnodes, hidx, hcomp;
+    // prove loop: PoW words, term descriptors and tables, opened rows, vector, covector
+    whir_gpu::DevBuf powbuf, tdesc, ttab, oidx, orows, pvec, pcov;
+    std::map<std::pair<uint32_t, bool>, whir_gpu::Twiddles> tw;
+    uint64_t* t1024[2] = {nullptr, nullptr};
+    // stage timing (HIP events on `stream`)
+    bool timing = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double acc_ms[4] = {0, 0, 0, 0};
+    uint64_t cnt[4] = {0, 0, 0, 0};
+
+    int mark(int k) {
+        if (!timing) return 0;
+        if (!ev[k]) HIPCHK(hipEventCreate(&ev[k]));
+        HIPCHK(hipEventRecord(ev[k], stream));
+        return 0;
+    }
+    // adds the time between marks a and b to stage st (waits for mark b)
+    int collect(int a, int b, int st) {
+        if (!timing) return 0;
+        HIPCHK(hipEventSynchronize(ev[b]));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ev[a], ev[b]));
+        acc_ms[st] += ms;
+        cnt[st] += 1;
+        return 0;
+    }
+
+    int ensure(whir_gpu::DevBuf& b, size_t bytes) {
+        if (b.bytes >= bytes && b.p) return 0;
+        if (b.p) {
+            HIPCHK(hipStreamSynchronize(stream));
+            HIPCHK(hipFree(b.p));
+            b.p = nullptr;
+            b.bytes = 0;
+        }
+        size_t want = std::max<size_t>(bytes, 256);
+        HIPCHK(hipMalloc(&b.p, want));
+        b.bytes = want;
+        return 0;
+    }
+
+    int twiddles(uint32_t log_n, bool inverse, whir_gpu::Twiddles& out) {
+        auto key = std::make_pair(log_n, inverse);
+        auto it = tw.find(key);
+        if (it != tw.end()) {
+            out = it->second;
+            return 0;
+        }
+        int inv = inverse ? 1 : 0;
+        if (!t1024[inv]) {
+            uint64_t w = gl::root_of_unity(10);
+            if (inverse) w = gl::inv(w);
+            std::vector<uint64_t> h(1024);
+            uint64_t acc = 1;
+            for (int i = 0; i < 1024; ++i) {
+                h[i] = acc;
+                acc = gl::mul(acc, w);
+            }
+            HIPCHK(hipMalloc(&t1024[inv], 1024 * 8));
+            HIPCHK(hipMemcpy(t1024[inv], h.data(), 1024 * 8, hipMemcpyHostToDevice));
+        }
+        uint64_t n = 1ull << log_n;
+        uint64_t w = gl::root_of_unity(log_n);
+        if (inverse) w = gl::inv(w);
+        std::vector<uint64_t> lo(2048), hi(std::max<uint64_t>(1, n / 2048));
+        uint64_t acc = 1;
+        for (int i = 0; i < 2048; ++i) {
+            lo[i] = acc;
+            acc = gl::mul(acc, w);
+        }
+        uint64_t w2048 = gl::pow(w, 2048);
+        acc = 1;
+        for (size_t i = 0; i < hi.size(); ++i) {
+            hi[i] = acc;
+            acc = gl::mul(acc, w2048);
+        }
+        whir_gpu::Twiddles t;
+        t.log_n = log_n;
+        HIPCHK(hipMalloc(&t.lo, lo.size() * 8));
+        HIPCHK(hipMalloc(&t.hi, hi.size() * 8));
+        HIPCHK(hipMemcpy(t.lo, lo.data(), lo.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(t.hi, hi.data(), hi.size() * 8, hipMemcpyHostToDevice));
+        t.t1024 = t1024[inv];
+        tw[key] = t;
+        out = t;
+        return 0;
+    }
+};
+
+namespace whir_gpu {
+
+// RS encode + leaves + Merkle tree of num_polys vectors already on the device;
+// the root is copied to out_root (host) when non-null.
+int commit_device(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_polys, size_t poly_size, size_t cw,
+                  size_t depth, uint64_t* d_matrix, uint8_t* d_nodes, uint8_t* out_root, int ext);
+
+}  // namespace whir_gpu

@@ -66,4 +66,39 @@ constexpr int HORNER_MAX_BLOCKS = 1024;
 void horner_launch(const uint64_t* f, bool f_ext3, uint64_t n, const uint64_t* zpow,
                    uint64_t* partials, uint64_t* out, hipStream_t st);
 
+// ------------------------------------------------------------- prove loop
+// Scan nonces base + k, k < blocks*256*per_thread, for the smallest one whose
+// BLAKE3(ch || LE64(nonce) || 0^24) read as LE64 is <= threshold; *best
+// (initialised to ~0 by the caller) takes the minimum hit.
+void pow_search_launch(const uint32_t* d_ch, uint64_t threshold, uint64_t base, uint32_t blocks,
+                       uint32_t per_thread, unsigned long long* d_best, hipStream_t st);
+void set_u64_launch(uint64_t* p, uint64_t v, hipStream_t st);
+
+// Weight terms sum_t s_t w_t(i) over i < 2^nv, factored as hi_t[i >> L] * lo_t[i & (2^L-1)].
+// Descriptor of a term, TERM_WORDS u64: kind (0 eq, 1 power), s (Ext3),
+// z (Ext3, power terms), z^(2^L) (Ext3, power terms), offset of the eq point
+// in the coordinate array (Ext3 elements, nv coordinates, first = top bit).
+constexpr uint32_t TERM_WORDS = 11;
+// hi: nterms * 2^(nv-L) Ext3; lo: nterms * 2^L Ext3, or u64 when lo_base
+// (power terms with z in the base field).
+void term_tables_launch(const uint64_t* d_desc, const uint64_t* d_coords, uint32_t nterms, uint32_t nv, uint32_t L,
+                        uint64_t* d_hi, uint64_t* d_lo, bool lo_base, hipStream_t st);
+// acc[i] = (accumulate ? acc[i] : 0) + sum over both term groups
+void term_accumulate_launch(uint64_t* d_acc, uint64_t n, uint32_t L, const uint64_t* hi_e, const uint64_t* lo_e,
+                            uint32_t te, const uint64_t* hi_b, const uint64_t* lo_b, uint32_t tb, bool accumulate,
+                            hipStream_t st);
+// sum_i f_i hi[i >> L] lo[i & mask] into partials[0..3) (device)
+void mle_eval_launch(const uint64_t* f, bool f_base, uint64_t n, uint32_t L, const uint64_t* hi, const uint64_t* lo,
+                     uint64_t* partials, hipStream_t st);
+// fold (a, b) of length n (>= 4) by r to length n/2, and (c0, c2) of the folded
+// pair into partials[0..6) (device); Ext3 inputs may be folded in place
+void fold_compute_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint64_t n, const uint64_t r[3],
+                         uint64_t* a_out, uint64_t* b_out, uint64_t* partials, hipStream_t st);
+// out[k*row_u64 + c] = matrix[idx[k]*row_u64 + c]
+void gather_rows_launch(const uint64_t* matrix, uint64_t row_u64, const uint64_t* d_idx, uint64_t count,
+                        uint64_t* out, hipStream_t st);
+// out = f_0 + sum_{k>=1} coef_k f_k (Ext3 out, base f_k of length n, coef Ext3 on device)
+void rlc_combine_launch(const uint64_t* f, uint64_t n, uint32_t nvec, const uint64_t* d_coef, uint64_t* out,
+                        hipStream_t st);
+
 }  // namespace whir_gpu

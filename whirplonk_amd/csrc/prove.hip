@@ -1,0 +1,315 @@
+// prove.hip — kernels of the WHIR prove loop beyond commit and plain sumcheck.
+//
+//  * pow_search: the smallest nonce n with LE64(BLAKE3(ch || LE64(n) || 0^24))
+//    <= threshold (protocols/proof_of_work.hpp:95-193).  Each launch scans one
+//    window of nonces in full and keeps the minimum hit, so windows scanned in
+//    increasing order return the global minimum, as the reference's scan does.
+//  * term tables + term_accumulate: acc[i] (+)= sum_t s_t * w_t(i) for terms
+//      eq terms   w(i) = prod_k (bit_k(i) ? x_k : 1 - x_k), x_0 the top bit
+//                 (eval_eq, algebra/multilinear.hpp:120-133)
+//      power terms w(i) = z^i (UnivariateEvaluation::accumulate /
+//                 geometric_accumulate, algebra/utilities.hpp:260-307)
+//    Every term factors over i = h * 2^L + l as hi_t[h] * lo_t[l]; a small
+//    kernel writes the two tables and the sweep does one multiply-add per
+//    term and entry (an Ext3 x base multiply when z is in the base field).
+//  * mle_evaluate: sum_i f_i eq(x, i) from the same tables
+//    (mixed_multilinear_extend, algebra/multilinear.hpp:41-103).
+//  * fold_compute: one sumcheck fold of (a, b) by r fused with the next round
+//    polynomial of the folded pair (sumcheck.hpp:43-116).
+//  * gather_rows: opened codeword rows (irs_commit.hpp:656-668).
+//  * rlc_combine: vector = f_0 + sum_k c_k f_k lifted to Ext3
+//    (lift / mixed_scalar_mul_add, utilities.hpp:152-181).
+#include "blake3_dev.h"
+#include "gl.h"
+#include "kernels.h"
+
+#include <algorithm>
+
+namespace whir_gpu {
+namespace {
+
+using gl::e3;
+
+__device__ __forceinline__ e3 ld3(const uint64_t* p, uint64_t i) { return {p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+__device__ __forceinline__ void st3(uint64_t* p, uint64_t i, e3 v) {
+    p[3 * i] = v.c0;
+    p[3 * i + 1] = v.c1;
+    p[3 * i + 2] = v.c2;
+}
+
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+    uint32_t lo = __shfl_xor((uint32_t)v, m, 64);
+    uint32_t hi = __shfl_xor((uint32_t)(v >> 32), m, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Sum of NV Ext3 values over a 256-thread workgroup; thread 0 gets the totals.
+template <int NV>
+__device__ __forceinline__ void block_sum256(e3* v) {
+    __shared__ e3 sh[NV][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) {
+            v[k].c0 = gl::add(v[k].c0, shfl_xor64(v[k].c0, m));
+            v[k].c1 = gl::add(v[k].c1, shfl_xor64(v[k].c1, m));
+            v[k].c2 = gl::add(v[k].c2, shfl_xor64(v[k].c2, m));
+        }
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) sh[k][wave] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+            v[k] = gl::e3_add(gl::e3_add(sh[k][0], sh[k][1]), gl::e3_add(sh[k][2], sh[k][3]));
+    }
+}
+
+// ------------------------------------------------------------------ PoW
+__global__ __launch_bounds__(256) void pow_search_kernel(const uint32_t* __restrict__ ch, uint64_t threshold,
+                                                         uint64_t base, uint32_t per_thread,
+                                                         unsigned long long* __restrict__ best) {
+    uint32_t m[16];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) m[i] = ch[i];
+#pragma unroll
+    for (int i = 10; i < 16; ++i) m[i] = 0;
+    uint32_t cv[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) cv[i] = b3::iv(i);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t nonce = base + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long found = ~0ull;
+    for (uint32_t k = 0; k < per_thread; ++k, nonce += stride) {
+        m[8] = (uint32_t)nonce;
+        m[9] = (uint32_t)(nonce >> 32);
+        uint32_t out[8];
+        b3::compress(cv, m, 0, 64, b3::CHUNK_START | b3::CHUNK_END | b3::ROOT, out);
+        uint64_t h = ((uint64_t)out[1] << 32) | out[0];
+        if (h <= threshold && nonce < found) found = nonce;
+    }
+    if (found != ~0ull) atomicMin(best, found);
+}
+
+// ------------------------------------------------------------ term tables
+// Term descriptor (TERM_WORDS u64): kind, s (3), z or unused (3), Z = z^(2^L)
+// (3), offset of the eq coordinates in `coords` (in Ext3 elements).
+enum : uint64_t { TERM_EQ = 0, TERM_POW = 1 };
+
+// hi tables: H entries per term (Ext3); lo tables: 2^L entries per term, Ext3
+// for eq terms and Ext3 powers, u64 for base-field powers (lo_base != 0).
+__global__ __launch_bounds__(256) void term_tables_kernel(const uint64_t* __restrict__ desc,
+                                                          const uint64_t* __restrict__ coords, uint32_t nv,
+                                                          uint32_t L, uint64_t* __restrict__ hi,
+                                                          uint64_t* __restrict__ lo, int lo_base) {
+    const uint32_t t = blockIdx.y;
+    const uint64_t* d = desc + (uint64_t)TERM_WORDS * t;
+    const uint64_t H = 1ull << (nv - L), LO = 1ull << L;
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const e3 s = {d[1], d[2], d[3]};
+    const e3 one = {1, 0, 0};
+    if (d[0] == TERM_EQ) {
+        const uint64_t* x = coords + 3 * d[10];
+        if (j < H) {
+            // top nv - L coordinates; coordinate k pairs with bit (nv - L - 1 - k) of h
+            e3 acc = s;
+            for (uint32_t k = 0; k < nv - L; ++k) {
+                e3 xk = ld3(x, k);
+                bool bit = (j >> (nv - L - 1 - k)) & 1;
+                acc = gl::e3_mul(acc, bit ? xk : gl::e3_sub(one, xk));
+            }
+            st3(hi, (uint64_t)t * H + j, acc);
+        }
+        if (j < LO) {
+            e3 acc = one;
+            for (uint32_t k = 0; k < L; ++k) {
+                e3 xk = ld3(x, nv - L + k);
+                bool bit = (j >> (L - 1 - k)) & 1;
+                acc = gl::e3_mul(acc, bit ? xk : gl::e3_sub(one, xk));
+            }
+            st3(lo, (uint64_t)t * LO + j, acc);
+        }
+    } else {
+        const e3 z = {d[4], d[5], d[6]}, Z = {d[7], d[8], d[9]};
+        if (j < H) st3(hi, (uint64_t)t * H + j, gl::e3_mul(s, gl::e3_pow(Z, j)));
+        if (j < LO) {
+            if (lo_base) {
+                lo[(uint64_t)t * LO + j] = gl::pow(z.c0, j);
+            } else {
+                st3(lo, (uint64_t)t * LO + j, gl::e3_pow(z, j));
+            }
+        }
+    }
+}
+
+// acc[i] (+)= sum over Ext3-lo terms of hi*lo + sum over base-lo terms of hi*lo
+__global__ __launch_bounds__(256) void term_accumulate_kernel(uint64_t* __restrict__ acc, uint64_t n, uint32_t L,
+                                                              const uint64_t* __restrict__ hi_e,
+                                                              const uint64_t* __restrict__ lo_e, uint32_t te,
+                                                              const uint64_t* __restrict__ hi_b,
+                                                              const uint64_t* __restrict__ lo_b, uint32_t tb,
+                                                              int accumulate) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t H = n >> L, LO = 1ull << L;
+    const uint64_t h = i >> L, l = i & (LO - 1);
+    e3 sum = accumulate ? ld3(acc, i) : e3{0, 0, 0};
+    for (uint32_t t = 0; t < te; ++t)
+        sum = gl::e3_add(sum, gl::e3_mul(ld3(hi_e, t * H + h), ld3(lo_e, t * LO + l)));
+    for (uint32_t t = 0; t < tb; ++t)
+        sum = gl::e3_add(sum, gl::e3_mul_base(ld3(hi_b, t * H + h), lo_b[t * LO + l]));
+    st3(acc, i, sum);
+}
+
+// partial sums of f_i * hi[i >> L] * lo[i & mask] (one eq term)
+__global__ __launch_bounds__(256) void mle_eval_kernel(const uint64_t* __restrict__ f, int f_base, uint64_t n,
+                                                       uint32_t L, const uint64_t* __restrict__ hi,
+                                                       const uint64_t* __restrict__ lo,
+                                                       uint64_t* __restrict__ partials) {
+    e3 acc[1] = {{0, 0, 0}};
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x, mask = (1ull << L) - 1;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        e3 w = gl::e3_mul(ld3(hi, i >> L), ld3(lo, i & mask));
+        e3 v = f_base ? gl::e3_mul_base(w, f[i]) : gl::e3_mul(w, ld3(f, i));
+        acc[0] = gl::e3_add(acc[0], v);
+    }
+    block_sum256<1>(acc);
+    if (threadIdx.x == 0) st3(partials, blockIdx.x, acc[0]);
+}
+
+__global__ __launch_bounds__(256) void sum_partials_kernel(uint64_t* partials, uint32_t count, uint32_t nv) {
+    e3 acc[2] = {{0, 0, 0}, {0, 0, 0}};
+    for (uint32_t k = threadIdx.x; k < count; k += blockDim.x)
+        for (uint32_t v = 0; v < nv; ++v) acc[v] = gl::e3_add(acc[v], ld3(partials, (uint64_t)nv * k + v));
+    block_sum256<2>(acc);
+    if (threadIdx.x == 0)
+        for (uint32_t v = 0; v < nv; ++v) st3(partials, v, acc[v]);
+}
+
+// ------------------------------------------------------------ fold+compute
+// (a, b) of length 4q folded by r into (a', b') of length 2q, plus
+//   c0 = sum_{i<q} a'_i b'_i,  c2 = sum_{i<q} (a'_{i+q} - a'_i)(b'_{i+q} - b'_i).
+// Lane i reads indices i, i+q, i+2q, i+3q and writes i, i+q only, so the
+// outputs may overwrite Ext3 inputs in place.
+__global__ __launch_bounds__(256) void fold_compute_kernel(const uint64_t* a, int a_base, const uint64_t* b,
+                                                           uint64_t q, e3 r, uint64_t* a_out, uint64_t* b_out,
+                                                           uint64_t* __restrict__ partials) {
+    e3 acc[2] = {{0, 0, 0}, {0, 0, 0}};
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < q; i += stride) {
+        e3 fa0, fa1;
+        if (a_base) {
+            uint64_t x0 = a[i], x1 = a[i + q], x2 = a[i + 2 * q], x3 = a[i + 3 * q];
+            fa0 = gl::e3_mul_base(r, gl::sub(x2, x0));
+            fa0.c0 = gl::add(fa0.c0, x0);
+            fa1 = gl::e3_mul_base(r, gl::sub(x3, x1));
+            fa1.c0 = gl::add(fa1.c0, x1);
+        } else {
+            e3 x0 = ld3(a, i), x1 = ld3(a, i + q), x2 = ld3(a, i + 2 * q), x3 = ld3(a, i + 3 * q);
+            fa0 = gl::e3_add(x0, gl::e3_mul(gl::e3_sub(x2, x0), r));
+            fa1 = gl::e3_add(x1, gl::e3_mul(gl::e3_sub(x3, x1), r));
+        }
+        e3 y0 = ld3(b, i), y1 = ld3(b, i + q), y2 = ld3(b, i + 2 * q), y3 = ld3(b, i + 3 * q);
+        e3 fb0 = gl::e3_add(y0, gl::e3_mul(gl::e3_sub(y2, y0), r));
+        e3 fb1 = gl::e3_add(y1, gl::e3_mul(gl::e3_sub(y3, y1), r));
+        st3(a_out, i, fa0);
+        st3(a_out, i + q, fa1);
+        st3(b_out, i, fb0);
+        st3(b_out, i + q, fb1);
+        acc[0] = gl::e3_add(acc[0], gl::e3_mul(fa0, fb0));
+        acc[1] = gl::e3_add(acc[1], gl::e3_mul(gl::e3_sub(fa1, fa0), gl::e3_sub(fb1, fb0)));
+    }
+    block_sum256<2>(acc);
+    if (threadIdx.x == 0) {
+        st3(partials, 2 * blockIdx.x, acc[0]);
+        st3(partials, 2 * blockIdx.x + 1, acc[1]);
+    }
+}
+
+// ------------------------------------------------------------ row gather
+__global__ __launch_bounds__(256) void gather_rows_kernel(const uint64_t* __restrict__ matrix, uint64_t row_u64,
+                                                          const uint64_t* __restrict__ idx, uint64_t count,
+                                                          uint64_t* __restrict__ out) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count * row_u64) return;
+    const uint64_t r = k / row_u64, c = k - r * row_u64;
+    out[k] = matrix[idx[r] * row_u64 + c];
+}
+
+// ------------------------------------------------------------ vector RLC
+// out[i] = f_0[i] + sum_{k>=1} c_k f_k[i], f_k base-field vectors of length n
+__global__ __launch_bounds__(256) void rlc_combine_kernel(const uint64_t* __restrict__ f, uint64_t n, uint32_t nvec,
+                                                          const uint64_t* __restrict__ coef,
+                                                          uint64_t* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    e3 acc = {f[i], 0, 0};
+    for (uint32_t k = 1; k < nvec; ++k) acc = gl::e3_add(acc, gl::e3_mul_base(ld3(coef, k), f[(uint64_t)k * n + i]));
+    st3(out, i, acc);
+}
+
+__global__ void set_u64_kernel(uint64_t* p, uint64_t v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+unsigned blocks_of(uint64_t work) { return (unsigned)((work + 255) / 256); }
+
+}  // namespace
+
+void pow_search_launch(const uint32_t* d_ch, uint64_t threshold, uint64_t base, uint32_t blocks, uint32_t per_thread,
+                       unsigned long long* d_best, hipStream_t st) {
+    hipLaunchKernelGGL(pow_search_kernel, dim3(blocks), dim3(256), 0, st, d_ch, threshold, base, per_thread, d_best);
+}
+
+void set_u64_launch(uint64_t* p, uint64_t v, hipStream_t st) {
+    hipLaunchKernelGGL(set_u64_kernel, dim3(1), dim3(64), 0, st, p, v);
+}
+
+void term_tables_launch(const uint64_t* d_desc, const uint64_t* d_coords, uint32_t nterms, uint32_t nv, uint32_t L,
+                        uint64_t* d_hi, uint64_t* d_lo, bool lo_base, hipStream_t st) {
+    if (!nterms) return;
+    uint64_t span = std::max<uint64_t>(1ull << (nv - L), 1ull << L);
+    hipLaunchKernelGGL(term_tables_kernel, dim3(blocks_of(span), nterms), dim3(256), 0, st, d_desc, d_coords, nv, L,
+                       d_hi, d_lo, lo_base ? 1 : 0);
+}
+
+void term_accumulate_launch(uint64_t* d_acc, uint64_t n, uint32_t L, const uint64_t* hi_e, const uint64_t* lo_e,
+                            uint32_t te, const uint64_t* hi_b, const uint64_t* lo_b, uint32_t tb, bool accumulate,
+                            hipStream_t st) {
+    hipLaunchKernelGGL(term_accumulate_kernel, dim3(blocks_of(n)), dim3(256), 0, st, d_acc, n, L, hi_e, lo_e, te, hi_b,
+                       lo_b, tb, accumulate ? 1 : 0);
+}
+
+void mle_eval_launch(const uint64_t* f, bool f_base, uint64_t n, uint32_t L, const uint64_t* hi, const uint64_t* lo,
+                     uint64_t* partials, hipStream_t st) {
+    unsigned g = std::min<unsigned>(blocks_of(n), SUMCHECK_MAX_BLOCKS);
+    hipLaunchKernelGGL(mle_eval_kernel, dim3(g), dim3(256), 0, st, f, f_base ? 1 : 0, n, L, hi, lo, partials);
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, partials, g, 1u);
+}
+
+void fold_compute_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint64_t n, const uint64_t r[3],
+                         uint64_t* a_out, uint64_t* b_out, uint64_t* partials, hipStream_t st) {
+    unsigned g = std::min<unsigned>(blocks_of(n / 4), SUMCHECK_MAX_BLOCKS);
+    e3 rr = {r[0], r[1], r[2]};
+    hipLaunchKernelGGL(fold_compute_kernel, dim3(g), dim3(256), 0, st, a, a_base ? 1 : 0, b, n / 4, rr, a_out, b_out,
+                       partials);
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, partials, g, 2u);
+}
+
+void gather_rows_launch(const uint64_t* matrix, uint64_t row_u64, const uint64_t* d_idx, uint64_t count, uint64_t* out,
+                        hipStream_t st) {
+    if (!count || !row_u64) return;
+    hipLaunchKernelGGL(gather_rows_kernel, dim3(blocks_of(count * row_u64)), dim3(256), 0, st, matrix, row_u64, d_idx,
+                       count, out);
+}
+
+void rlc_combine_launch(const uint64_t* f, uint64_t n, uint32_t nvec, const uint64_t* d_coef, uint64_t* out,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(rlc_combine_kernel, dim3(blocks_of(n)), dim3(256), 0, st, f, n, nvec, d_coef, out);
+}
+
+}  // namespace whir_gpu

@@ -17,30 +17,20 @@
 #include <string>
 #include <vector>
 
+#include "ctx.h"
 #include "gl.h"
 #include "kernels.h"
 
 using namespace whir_gpu;
 
+namespace whir_gpu {
+std::string& last_error() {
+    thread_local std::string e;
+    return e;
+}
+}  // namespace whir_gpu
+
 namespace {
-
-thread_local std::string g_err;
-
-int fail(const char* what, hipError_t e) {
-    g_err = std::string(what) + ": " + hipGetErrorString(e);
-    return -1;
-}
-
-#define HIPCHK(expr)                                   \
-    do {                                               \
-        hipError_t e_ = (expr);                        \
-        if (e_ != hipSuccess) return fail(#expr, e_);  \
-    } while (0)
-
-int bad_arg(const char* what) {
-    g_err = what;
-    return 3;
-}
 
 bool env_disabled() {
     for (const char* name : {"WHIR_GPU_DISABLE", "WHIR_CUDA_DISABLE"}) {
@@ -54,112 +44,7 @@ std::mutex g_flags_mu;
 bool g_dispatch = !env_disabled();
 size_t g_threshold = 65536;
 
-bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
-uint32_t log2u(uint64_t x) {
-    uint32_t k = 0;
-    while ((1ull << k) < x) ++k;
-    return k;
-}
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-};
-
 }  // namespace
-
-struct whir_ctx {
-    int device = 0;
-    hipStream_t own = nullptr;
-    hipStream_t stream = nullptr;
-    std::recursive_mutex mu;
-    DevBuf scratch, partials, zpow, hin, hout, hnodes, hidx, hcomp;
-    std::map<std::pair<uint32_t, bool>, Twiddles> tw;
-    uint64_t* t1024[2] = {nullptr, nullptr};
-    // stage timing (HIP events on `stream`)
-    bool timing = false;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    double acc_ms[4] = {0, 0, 0, 0};
-    uint64_t cnt[4] = {0, 0, 0, 0};
-
-    int mark(int k) {
-        if (!timing) return 0;
-        if (!ev[k]) HIPCHK(hipEventCreate(&ev[k]));
-        HIPCHK(hipEventRecord(ev[k], stream));
-        return 0;
-    }
-    // adds the time between marks a and b to stage st (waits for mark b)
-    int collect(int a, int b, int st) {
-        if (!timing) return 0;
-        HIPCHK(hipEventSynchronize(ev[b]));
-        float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, ev[a], ev[b]));
-        acc_ms[st] += ms;
-        cnt[st] += 1;
-        return 0;
-    }
-
-    int ensure(DevBuf& b, size_t bytes) {
-        if (b.bytes >= bytes && b.p) return 0;
-        if (b.p) {
-            HIPCHK(hipStreamSynchronize(stream));
-            HIPCHK(hipFree(b.p));
-            b.p = nullptr;
-            b.bytes = 0;
-        }
-        size_t want = std::max<size_t>(bytes, 256);
-        HIPCHK(hipMalloc(&b.p, want));
-        b.bytes = want;
-        return 0;
-    }
-
-    int twiddles(uint32_t log_n, bool inverse, Twiddles& out) {
-        auto key = std::make_pair(log_n, inverse);
-        auto it = tw.find(key);
-        if (it != tw.end()) {
-            out = it->second;
-            return 0;
-        }
-        int inv = inverse ? 1 : 0;
-        if (!t1024[inv]) {
-            uint64_t w = gl::root_of_unity(10);
-            if (inverse) w = gl::inv(w);
-            std::vector<uint64_t> h(1024);
-            uint64_t acc = 1;
-            for (int i = 0; i < 1024; ++i) {
-                h[i] = acc;
-                acc = gl::mul(acc, w);
-            }
-            HIPCHK(hipMalloc(&t1024[inv], 1024 * 8));
-            HIPCHK(hipMemcpy(t1024[inv], h.data(), 1024 * 8, hipMemcpyHostToDevice));
-        }
-        uint64_t n = 1ull << log_n;
-        uint64_t w = gl::root_of_unity(log_n);
-        if (inverse) w = gl::inv(w);
-        std::vector<uint64_t> lo(2048), hi(std::max<uint64_t>(1, n / 2048));
-        uint64_t acc = 1;
-        for (int i = 0; i < 2048; ++i) {
-            lo[i] = acc;
-            acc = gl::mul(acc, w);
-        }
-        uint64_t w2048 = gl::pow(w, 2048);
-        acc = 1;
-        for (size_t i = 0; i < hi.size(); ++i) {
-            hi[i] = acc;
-            acc = gl::mul(acc, w2048);
-        }
-        Twiddles t;
-        t.log_n = log_n;
-        HIPCHK(hipMalloc(&t.lo, lo.size() * 8));
-        HIPCHK(hipMalloc(&t.hi, hi.size() * 8));
-        HIPCHK(hipMemcpy(t.lo, lo.data(), lo.size() * 8, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(t.hi, hi.data(), hi.size() * 8, hipMemcpyHostToDevice));
-        t.t1024 = t1024[inv];
-        tw[key] = t;
-        out = t;
-        return 0;
-    }
-};
 
 namespace {
 
@@ -234,7 +119,7 @@ int rs_encode_any(whir_ctx* ctx, const uint64_t* d_in, size_t num_polys, size_t 
 template <typename F>
 int with_default(F f) {
     if (!dispatch_on()) {
-        g_err = "GPU dispatch disabled";
+        whir_gpu::last_error() = "GPU dispatch disabled";
         return 1;
     }
     whir_ctx* ctx = nullptr;
@@ -246,9 +131,32 @@ int with_default(F f) {
 
 }  // namespace
 
+int whir_gpu::commit_device(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_polys, size_t poly_size, size_t cw,
+                      size_t depth, uint64_t* d_matrix, uint8_t* d_nodes, uint8_t* out_root, int ext) {
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    int rc;
+    bool fused = false;
+    if ((rc = ctx->mark(0))) return rc;
+    if ((rc = rs_encode_any(ctx, d_coeffs, num_polys, poly_size, cw, depth, d_matrix, ext, d_nodes, &fused))) return rc;
+    if ((rc = ctx->mark(1))) return rc;
+    size_t row_u64 = num_polys * depth * (ext ? 3 : 1);
+    if (!fused) leaf_hash_launch(d_matrix, cw, (uint32_t)row_u64, d_nodes, ctx->stream);
+    if ((rc = ctx->mark(2))) return rc;
+    merkle_build_launch(d_nodes, cw, ctx->stream);
+    HIPCHK(hipGetLastError());
+    if ((rc = ctx->mark(3))) return rc;
+    if (out_root) {
+        size_t nn = whir_merkle_num_nodes(cw);
+        HIPCHK(hipMemcpyAsync(out_root, d_nodes + 32 * (nn - 1), 32, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+    }
+    if ((rc = ctx->collect(0, 1, 0)) || (rc = ctx->collect(1, 2, 1)) || (rc = ctx->collect(2, 3, 2))) return rc;
+    return 0;
+}
+
 extern "C" {
 
-const char* whir_last_error(void) { return g_err.c_str(); }
+const char* whir_last_error(void) { return whir_gpu::last_error().c_str(); }
 
 int whir_device_available(void) {
     int n = 0;
@@ -302,7 +210,8 @@ int whir_ctx_destroy(whir_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (DevBuf* b : {&ctx->scratch, &ctx->partials, &ctx->zpow, &ctx->hin, &ctx->hout, &ctx->hnodes, &ctx->hidx,
-                      &ctx->hcomp})
+                      &ctx->hcomp, &ctx->powbuf, &ctx->tdesc, &ctx->ttab, &ctx->oidx, &ctx->orows, &ctx->pvec,
+                      &ctx->pcov})
         if (b->p) (void)hipFree(b->p);
     for (auto& kv : ctx->tw) {
         (void)hipFree(kv.second.lo);
@@ -423,37 +332,14 @@ int whir_dev_gather_hashes(whir_ctx* ctx, const uint8_t* d_nodes, const uint64_t
     return 0;
 }
 
-static int commit_any(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_polys, size_t poly_size, size_t cw,
-                      size_t depth, uint64_t* d_matrix, uint8_t* d_nodes, uint8_t* out_root, int ext) {
-    std::lock_guard<std::recursive_mutex> g(ctx->mu);
-    int rc;
-    bool fused = false;
-    if ((rc = ctx->mark(0))) return rc;
-    if ((rc = rs_encode_any(ctx, d_coeffs, num_polys, poly_size, cw, depth, d_matrix, ext, d_nodes, &fused))) return rc;
-    if ((rc = ctx->mark(1))) return rc;
-    size_t row_u64 = num_polys * depth * (ext ? 3 : 1);
-    if (!fused) leaf_hash_launch(d_matrix, cw, (uint32_t)row_u64, d_nodes, ctx->stream);
-    if ((rc = ctx->mark(2))) return rc;
-    merkle_build_launch(d_nodes, cw, ctx->stream);
-    HIPCHK(hipGetLastError());
-    if ((rc = ctx->mark(3))) return rc;
-    if (out_root) {
-        size_t nn = whir_merkle_num_nodes(cw);
-        HIPCHK(hipMemcpyAsync(out_root, d_nodes + 32 * (nn - 1), 32, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-    }
-    if ((rc = ctx->collect(0, 1, 0)) || (rc = ctx->collect(1, 2, 1)) || (rc = ctx->collect(2, 3, 2))) return rc;
-    return 0;
-}
-
 int whir_dev_commit_base(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_polys, size_t poly_size, size_t cw,
                          size_t depth, uint64_t* d_matrix, uint8_t* d_nodes, uint8_t* out_root) {
-    return commit_any(ctx, d_coeffs, num_polys, poly_size, cw, depth, d_matrix, d_nodes, out_root, 0);
+    return whir_gpu::commit_device(ctx, d_coeffs, num_polys, poly_size, cw, depth, d_matrix, d_nodes, out_root, 0);
 }
 
 int whir_dev_commit_ext3(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_polys, size_t poly_size, size_t cw,
                          size_t depth, uint64_t* d_matrix, uint8_t* d_nodes, uint8_t* out_root) {
-    return commit_any(ctx, d_coeffs, num_polys, poly_size, cw, depth, d_matrix, d_nodes, out_root, 1);
+    return whir_gpu::commit_device(ctx, d_coeffs, num_polys, poly_size, cw, depth, d_matrix, d_nodes, out_root, 1);
 }
 
 int whir_dev_sumcheck_compute(whir_ctx* ctx, const uint64_t* d_a, int a_is_base, const uint64_t* d_b, size_t n,
@@ -563,7 +449,7 @@ static int ntt_host(uint64_t* values, size_t total, size_t size, int inverse) {
     {
         std::lock_guard<std::mutex> g(g_flags_mu);
         if (total < g_threshold) {
-            g_err = "below the GPU NTT threshold";
+            whir_gpu::last_error() = "below the GPU NTT threshold";
             return 1;
         }
     }
@@ -732,7 +618,7 @@ int whir_gpu_blake3_merkle_open_path(const uint8_t* leaves, size_t n, const uint
 static int ood_host(const uint64_t* const* comps, int ext, const uint64_t* points, uint64_t* out, size_t nv,
                     size_t vs, size_t np) {
     if (nv == 0 || vs == 0 || np == 0) {
-        g_err = "empty OOD evaluation";
+        whir_gpu::last_error() = "empty OOD evaluation";
         return 1;  // the reference returns false here (cuda_integration.hpp:1109-1111)
     }
     return with_default([&](whir_ctx* ctx) {
