@@ -246,6 +246,14 @@ typedef struct whir_prover whir_prover;
 int whir_prover_create(whir_ctx* ctx, const whir_params* params, uint64_t size, const uint8_t* protocol_id,
                        const uint8_t* session_id, const uint8_t* instance, size_t instance_len,
                        whir_prover** out);
+/* The same with all of the proof's request-sized device memory (codewords,
+ * trees, folded vector, covector, LDE scratch) carved from a caller-owned,
+ * 256-byte aligned region of at least whir_prover_workspace_bytes bytes
+ * (the serving loop's HBM-budgeted admission uses this). */
+int whir_prover_create_in(whir_ctx* ctx, const whir_params* params, uint64_t size, const uint8_t* protocol_id,
+                          const uint8_t* session_id, const uint8_t* instance, size_t instance_len,
+                          void* d_workspace, size_t workspace_bytes, whir_prover** out);
+size_t whir_prover_workspace_bytes(const whir_params* params, uint64_t size);
 int whir_prover_destroy(whir_prover* prover);
 /* whir::Config::commit (protocols/whir/whir.hpp:304-309 ->
  * irs_commit.hpp:328-553): d_vectors holds batch_size base-field vectors of
@@ -264,8 +272,15 @@ int whir_prover_prove(whir_prover* prover, size_t num_forms, const uint64_t* poi
  * written; the bytes when the buffers are non-null and large enough. */
 int whir_prover_proof(whir_prover* prover, uint8_t* narg, size_t narg_cap, size_t* narg_len, uint8_t* hints,
                       size_t hints_cap, size_t* hints_len);
-/* Number of PoW candidates the prover's searches scanned. */
+/* PoW work of the proof: the sum over its searches of (nonce + 1), the
+ * candidates a serial scan hashes. */
 size_t whir_prover_pow_hashes(const whir_prover* prover);
+/* Host wall time (ms, waits on the device included) per prover phase since
+ * creation: 0 initial commit (encode, leaves, tree), 1 OOD evaluation,
+ * 2 weight vectors (constraints, STIR), 3 sumcheck rounds, 4 PoW searches,
+ * 5 round commits, 6 openings, 7 everything else. */
+#define WHIR_PROVER_PHASES 8
+int whir_prover_phase_times(whir_prover* prover, double* out_ms);
 
 /* The prover transcript on its own (ProverState, transcript.hpp:595-662),
  * host only: SHAKE-128 duplex sponge, narg string and hints.  With null ids
@@ -311,6 +326,38 @@ int whir_dev_geometric_accumulate(whir_ctx* ctx, uint64_t* d_acc, size_t n, cons
  * extension of d_f (2^num_variables entries, base or Ext3) at point. */
 int whir_dev_mle_evaluate(whir_ctx* ctx, const uint64_t* d_f, int f_is_base, size_t num_variables,
                           const uint64_t* point, uint64_t* out);
+
+/* --------------------------------------------------------------- serving */
+
+/* A serving loop for commit+prove requests of any size under one parameter
+ * set and the CLI domain separator: num_slots worker threads, each with its
+ * own context and HIP stream, and one device arena of hbm_budget_bytes from
+ * which every admitted request takes its block (prover workspace +
+ * coefficients) until it completes.  A request waits until a slot is idle and
+ * its block fits; smaller requests may pass a waiting one a bounded number of
+ * times.  Each result is byte-identical to whir_prover_* on the same inputs. */
+typedef struct whir_server whir_server;
+int whir_server_create(int device, const whir_params* params, uint32_t num_slots, size_t hbm_budget_bytes,
+                       whir_server** out);
+/* Finishes every submitted request, then stops the workers. */
+int whir_server_destroy(whir_server* server);
+/* Device bytes a request of `size` coefficients holds while it runs. */
+size_t whir_server_request_bytes(whir_server* server, uint64_t size);
+/* Queue a request: batch_size vectors of `size` base-field coefficients (host,
+ * copied), num_forms MLE points (host, num_forms x log2(size) Ext3) and their
+ * claimed values (num_forms x batch_size Ext3), or evaluations = NULL to have
+ * them computed on the device.  Returns its id. */
+int whir_server_submit(whir_server* server, uint64_t size, const uint64_t* coeffs, size_t num_forms,
+                       const uint64_t* points, const uint64_t* evaluations, uint64_t* out_id);
+/* Block until request `id` is done.  Lengths are always written; with
+ * buffers large enough the proof is copied and the request released (with
+ * null buffers it is kept for a later call).  out_ms (optional, 3 doubles):
+ * queue wait, service time and end-to-end latency in ms. */
+int whir_server_wait(whir_server* server, uint64_t id, uint8_t* narg, size_t narg_cap, size_t* narg_len,
+                     uint8_t* hints, size_t hints_cap, size_t* hints_len, double* out_ms);
+/* out[7]: completed, running, queued, arena bytes in use, peak bytes in use,
+ * budget, slots. */
+int whir_server_stats(whir_server* server, uint64_t* out);
 
 /* --------------------------------------------------------- stage timing */
 

@@ -1,0 +1,84 @@
+"""The serving loop: every proof produced in a mix of concurrent requests of
+different sizes, admitted under an HBM budget, is byte-identical to the same
+request proved alone on the plain path (and to the reference build)."""
+import random
+
+import numpy as np
+import pytest
+
+import oracle
+import whirplonk_amd as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _request(nv, nforms, seed):
+    coeffs, st = W.lcg_uniform(seed, 1 << nv)
+    pts, _ = W.lcg_uniform(st, 3 * nv * nforms)
+    return coeffs, pts.reshape(nforms, nv, 3)
+
+
+def _alone(ctx, params, nv, coeffs, pts):
+    pid, sid = W.domain_separator_cli(params)
+    d = ctx.upload(coeffs)
+    ev = np.array([ctx.mle_evaluate(d, True, pts[f]) for f in range(len(pts))], dtype=np.uint64)
+    with W.Prover(ctx, params, 1 << nv, pid, sid) as pr:
+        pr.commit(d, 1)
+        pr.prove(pts, ev)
+        out = pr.proof()
+    ctx.free(d)
+    return out, ev
+
+
+@pytest.mark.parametrize("slots", [1, 4])
+def test_mix_equals_alone(slots):
+    params = W.Params.default(starting_log_inv_rate=2)
+    rnd = random.Random(slots)
+    sizes = [rnd.choice([8, 10, 12, 14]) for _ in range(24)]
+    reqs = [(nv, rnd.randrange(4)) for nv in sizes]
+    inputs = []
+    for i, (nv, nf) in enumerate(reqs):
+        c, p = _request(nv, nf, 0xC0FFEE00 + i)
+        inputs.append((nv, c, p))
+    ctx = W.Context()
+    alone = [_alone(ctx, params, nv, c, p) for nv, c, p in inputs]
+    ctx.close()
+    # a budget of about two of the largest requests: forces queueing
+    with W.Server(params, num_slots=slots, hbm_budget=1 << 20) as probe:
+        big = probe.request_bytes(1 << 14)
+    budget = 2 * big + (1 << 20)
+    print("serve", slots, "budget", budget, flush=True)
+    with W.Server(params, num_slots=slots, hbm_budget=budget) as srv:
+        ids = []
+        for i, (nv, c, p) in enumerate(inputs):
+            ev = alone[i][1] if i % 2 else None  # half computed on the device
+            ids.append(srv.submit(1 << nv, c, p, ev))
+        for i, rid in enumerate(ids):
+            narg, hints, ms = srv.wait(rid)
+            assert (narg, hints) == alone[i][0], i
+            assert ms[2] >= ms[1] >= 0
+        st = srv.stats()
+        assert st["completed"] == len(inputs)
+        assert st["peak"] <= budget and st["in_use"] == 0
+
+
+def test_served_vs_reference():
+    if oracle.ref() is None:
+        pytest.skip("reference build not available")
+    params = W.Params.default()
+    with W.Server(params, num_slots=3, hbm_budget=1 << 30) as srv:
+        cases = []
+        for i, nv in enumerate([12, 16, 13, 15]):
+            c, p = _request(nv, 3, 0xABCD00 + i)
+            cases.append((nv, c, p, srv.submit(1 << nv, c, p)))
+        for nv, c, p, rid in cases:
+            _, narg, hints = oracle.ref_prove(nv, c, p)
+            assert srv.wait(rid)[:2] == (narg, hints)
+
+
+def test_request_larger_than_budget():
+    params = W.Params.default()
+    with W.Server(params, num_slots=1, hbm_budget=1 << 20) as srv:
+        c, p = _request(16, 1, 1)
+        with pytest.raises(W.WhirGpuError):
+            srv.submit(1 << 16, c, p)

@@ -91,11 +91,15 @@ _SIGS = [
     ("whir_config_flat", _sz, [ctypes.c_uint64, _vp, _u64p, _sz]),
     ("whir_domain_separator_cli", _int, [_vp, ctypes.c_char_p, _u8p, _u8p]),
     ("whir_prover_create", _int, [_vp, _vp, ctypes.c_uint64, _u8p, _u8p, _u8p, _sz, ctypes.POINTER(_vp)]),
+    ("whir_prover_create_in", _int, [_vp, _vp, ctypes.c_uint64, _u8p, _u8p, _u8p, _sz, _vp, _sz,
+                                     ctypes.POINTER(_vp)]),
+    ("whir_prover_workspace_bytes", _sz, [_vp, ctypes.c_uint64]),
     ("whir_prover_destroy", _int, [_vp]),
     ("whir_prover_commit", _int, [_vp, _vp, _sz]),
     ("whir_prover_prove", _int, [_vp, _sz, _u64p, _u64p, _u64p, _u64p]),
     ("whir_prover_proof", _int, [_vp, _u8p, _sz, ctypes.POINTER(_sz), _u8p, _sz, ctypes.POINTER(_sz)]),
     ("whir_prover_pow_hashes", _sz, [_vp]),
+    ("whir_prover_phase_times", _int, [_vp, ctypes.POINTER(ctypes.c_double)]),
     ("whir_transcript_create", _int, [_u8p, _u8p, _u8p, _sz, ctypes.POINTER(_vp)]),
     ("whir_transcript_destroy", _int, [_vp]),
     ("whir_transcript_absorb", _int, [_vp, _u8p, _sz]),
@@ -110,6 +114,13 @@ _SIGS = [
     ("whir_dev_eq_accumulate", _int, [_vp, _vp, _sz, _u64p, _u64p, _int]),
     ("whir_dev_geometric_accumulate", _int, [_vp, _vp, _sz, _u64p, _u64p, _sz]),
     ("whir_dev_mle_evaluate", _int, [_vp, _vp, _int, _sz, _u64p, _u64p]),
+    ("whir_server_create", _int, [_int, _vp, ctypes.c_uint32, _sz, ctypes.POINTER(_vp)]),
+    ("whir_server_destroy", _int, [_vp]),
+    ("whir_server_request_bytes", _sz, [_vp, ctypes.c_uint64]),
+    ("whir_server_submit", _int, [_vp, ctypes.c_uint64, _u64p, _sz, _u64p, _u64p, _u64p]),
+    ("whir_server_wait", _int, [_vp, ctypes.c_uint64, _u8p, _sz, ctypes.POINTER(_sz), _u8p, _sz, ctypes.POINTER(_sz),
+                                ctypes.POINTER(ctypes.c_double)]),
+    ("whir_server_stats", _int, [_vp, _u64p]),
     ("whir_ctx_enable_timing", _int, [_vp, _int]),
     ("whir_ctx_reset_timing", _int, [_vp]),
     ("whir_ctx_stage_times", _int, [_vp, ctypes.POINTER(ctypes.c_double), _u64p]),
@@ -559,6 +570,64 @@ class Prover:
 
     def pow_hashes(self) -> int:
         return load().whir_prover_pow_hashes(self.h)
+
+    PHASES = ("commit", "ood", "weights", "sumcheck", "pow", "round_commit", "open", "other")
+
+    def phase_times(self) -> dict:
+        ms = (ctypes.c_double * len(self.PHASES))()
+        _check(load().whir_prover_phase_times(self.h, ms))
+        return {k: ms[i] for i, k in enumerate(self.PHASES)}
+
+
+class Server:
+    """The serving loop (whir_server_*): submit requests, wait for proofs."""
+
+    def __init__(self, params: Params, num_slots: int = 4, hbm_budget: int = 8 << 30, device: int = 0):
+        h = _vp()
+        _check(load().whir_server_create(device, ctypes.byref(params), num_slots, hbm_budget, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            _check(load().whir_server_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def request_bytes(self, size: int) -> int:
+        return load().whir_server_request_bytes(self.h, size)
+
+    def submit(self, size: int, coeffs, points=None, evaluations=None) -> int:
+        c = _u64(coeffs)
+        pts = _u64(points).reshape(-1) if points is not None and np.size(points) else None
+        nf = pts.size // (3 * (size.bit_length() - 1)) if pts is not None else 0
+        ev = _u64(evaluations).reshape(-1) if evaluations is not None else None
+        out = np.zeros(1, dtype=np.uint64)
+        _check(load().whir_server_submit(self.h, size, _p64(c), nf, _p64(pts) if nf else None,
+                                         _p64(ev) if ev is not None else None, _p64(out)))
+        return int(out[0])
+
+    def wait(self, rid: int):
+        """(narg, hints, (queue_ms, service_ms, latency_ms))"""
+        na, nh = _sz(), _sz()
+        ms = (ctypes.c_double * 3)()
+        _check(load().whir_server_wait(self.h, rid, None, 0, ctypes.byref(na), None, 0, ctypes.byref(nh), ms))
+        a = np.zeros(max(na.value, 1), dtype=np.uint8)
+        b = np.zeros(max(nh.value, 1), dtype=np.uint8)
+        _check(load().whir_server_wait(self.h, rid, _p8(a), a.size, ctypes.byref(na), _p8(b), b.size,
+                                       ctypes.byref(nh), ms))
+        return a[:na.value].tobytes(), b[:nh.value].tobytes(), (ms[0], ms[1], ms[2])
+
+    STATS = ("completed", "running", "queued", "in_use", "peak", "budget", "slots")
+
+    def stats(self) -> dict:
+        out = np.zeros(len(self.STATS), dtype=np.uint64)
+        _check(load().whir_server_stats(self.h, _p64(out)))
+        return {k: int(v) for k, v in zip(self.STATS, out)}
 
 
 def merkle_num_nodes(n: int) -> int:

@@ -54,7 +54,10 @@ struct whir_ctx {
     std::recursive_mutex mu;
     whir_gpu::DevBuf scratch, partials, zpow, hin, hout, hnodes, hidx, hcomp;
     // prove loop: PoW words, term descriptors and tables, opened rows, vector, covector
-    whir_gpu::DevBuf powbuf, tdesc, ttab, oidx, orows, pvec, pcov;
+    whir_gpu::DevBuf powbuf, tdesc, ttab, oidx, orows;
+    // when set, LDE passes use this caller-owned scratch instead of `scratch`
+    void* ext_scratch = nullptr;
+    size_t ext_scratch_bytes = 0;
     std::map<std::pair<uint32_t, bool>, whir_gpu::Twiddles> tw;
     uint64_t* t1024[2] = {nullptr, nullptr};
     // stage timing (HIP events on `stream`)

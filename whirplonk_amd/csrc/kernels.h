@@ -26,9 +26,14 @@ struct LdeShape {
     uint32_t in_el = 1, in_comp = 0, out_el = 1, out_comp = 0;
     bool row_major = true;
     bool inverse = false;
+    // Ext3 AoS input as 3 base columns per Ext3 column: column c reads
+    // component c % 3 of Ext3 column c / 3 (C counts base columns, in_el = 3,
+    // out_el = 1), so a row-major output row is the AoS Ext3 row.
+    bool split3 = false;
 };
 
-// Scratch needed by lde_launch (elements of u64).
+// Scratch needed by lde_launch (elements of u64); 0 when m <= 1024 (one
+// pass reads the input directly).
 size_t lde_scratch_elems(const LdeShape& s);
 // Returns 0, or >0 when the shape is not supported (not a power of two, m > 2^20).
 // With leaves != null and a row-major 16-column codeword the BLAKE3 row

@@ -32,6 +32,8 @@
 #include "dft.h"
 #include "kernels.h"
 
+#include <mutex>
+
 namespace whir_gpu {
 namespace {
 
@@ -49,6 +51,9 @@ struct LdeArgs {
     uint32_t in_el, in_comp, out_el, out_comp;
     uint32_t TA, TB, CB, row_major;
     uint32_t tw_in_b;  // inter-pass twiddle applied in pass B (TB == 1)
+    // split3: column c is component c % 3 of Ext3 column c / 3 (AoS input);
+    // direct: M1 == 1, pass B reads the input itself (no pass A, no scratch)
+    uint32_t split3, direct;
     // log2 of the power-of-two tile sizes: every index split is a shift/mask
     uint32_t lgTA, lgTB, lgCB, lgNJ, lgNK;
 };
@@ -58,6 +63,12 @@ constexpr int FUSE_COLS = 16;  // row width (u64) of the fused leaf hashing
 // swizzled position of (row, col) in the 16-wide LDS row stage: rows are
 // read one per lane, so columns rotate with the row to spread the banks
 __device__ __forceinline__ int stage_idx(int row, int col) { return row * FUSE_COLS + (col ^ (row & 15)); }
+
+// input index of coefficient j of column c
+__device__ __forceinline__ uint64_t in_index(const LdeArgs& a, uint64_t c, uint64_t j) {
+    if (a.split3) return ((c / 3) * a.m + j) * 3 + (c % 3);
+    return (c * a.m + j) * a.in_el + a.in_comp;
+}
 
 // w_n^e for e < n from the two-level table
 __device__ __forceinline__ uint64_t twid(const LdeArgs& a, uint64_t e) {
@@ -76,7 +87,7 @@ __global__ __launch_bounds__(THREADS) void lde_pass_a(LdeArgs a, uint64_t s0) {
     const uint32_t J = (blockIdx.x & ((1u << a.lgNJ) - 1)) << lgTA;
     const uint64_t s = s0 + blockIdx.y;
     uint64_t* z = a.z + blockIdx.y * a.zstride + (uint64_t)c * a.M2 + J;   // Z[k][c][J + t] = z[k*C*M2 + t]
-    const uint64_t* col = a.in + ((uint64_t)c * a.m + J) * a.in_el + a.in_comp;
+    const uint64_t* col = a.in + in_index(a, c, J);
     const uint32_t in_el = a.in_el, M2 = (uint32_t)a.M2, zrow = (uint32_t)(a.C * a.M2);
     uint64_t* twist = lds + TA * PAD;  // w_L^{s j1} = w_n^{s j1 M2}, shared by the TA columns
 
@@ -139,7 +150,7 @@ __global__ __launch_bounds__(THREADS) void lde_pass_b(LdeArgs a, uint64_t s0) {
         uint32_t j2 = q % M2, d = q / M2;
         uint32_t tb = a.row_major ? d >> lgCB : d & (TB - 1);
         uint32_t cb = a.row_major ? d & (CB - 1) : d >> lgTB;
-        uint64_t v = z[tb * CM2 + cb * M2 + j2];
+        uint64_t v = a.direct ? a.in[in_index(a, C0 + cb, j2)] : z[tb * CM2 + cb * M2 + j2];
         if (a.tw_in_b) v = gl::mul(v, twt[j2]);
         lds[d * PAD + dft::pidx<M2>(j2)] = v;
     }
@@ -270,6 +281,7 @@ KernelFn pick_b(uint64_t M) {
 struct Plan {
     uint64_t M1, M2, E, L, G;  // G cosets per launch
     uint32_t TA, TB, CB;
+    bool direct;
 };
 
 // Largest power of two dividing x (capped).
@@ -286,7 +298,8 @@ bool make_plan(const LdeShape& s, Plan& p) {
     p.M2 = std::min<uint64_t>(s.m, 1024);
     p.M1 = s.m / p.M2;
     if (p.M1 > 1024) return false;
-    if (s.m * s.C * s.in_el >= (1ull << 31)) return false;  // per-launch offsets are 32-bit
+    if (s.m * s.C * (s.split3 ? 1 : s.in_el) >= (1ull << 31)) return false;  // per-launch offsets are 32-bit
+    if (s.split3 && (s.C % 3 || s.in_el != 3 || s.out_el != 1)) return false;
     p.E = s.n / s.m;
     p.L = p.E * p.M1;
     p.TA = (uint32_t)std::min<uint64_t>(p.M2, THREADS / split_r2(p.M1));
@@ -299,7 +312,9 @@ bool make_plan(const LdeShape& s, Plan& p) {
         p.CB = (uint32_t)pow2_div(s.C, DB / p.TB);
     }
     uint64_t per = s.m * s.C;
-    p.G = std::max<uint64_t>(1, std::min<uint64_t>({p.E, SCRATCH_TARGET / per, 32768}));
+    p.direct = p.M1 == 1;
+    p.G = p.direct ? std::min<uint64_t>(p.E, 32768)
+                   : std::max<uint64_t>(1, std::min<uint64_t>({p.E, SCRATCH_TARGET / per, 32768}));
     return true;
 }
 
@@ -308,7 +323,7 @@ bool make_plan(const LdeShape& s, Plan& p) {
 size_t lde_scratch_elems(const LdeShape& s) {
     Plan p;
     if (!make_plan(s, p)) return 0;
-    return (size_t)(p.G * s.m * s.C);
+    return p.direct ? 0 : (size_t)(p.G * s.m * s.C);
 }
 
 int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* scratch,
@@ -349,21 +364,27 @@ int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* s
     a.lgNJ = (uint32_t)ilog2u(p.M2 / p.TA);
     a.lgNK = (uint32_t)ilog2u(p.M1 / p.TB);
     a.row_major = s.row_major ? 1 : 0;
+    a.split3 = s.split3 ? 1 : 0;
+    a.direct = p.direct ? 1 : 0;
 
     KernelFn ka = s.inverse ? pick_a<true>(p.M1) : pick_a<false>(p.M1);
     KernelFn kb = s.inverse ? pick_b<true>(p.M2) : pick_b<false>(p.M2);
     size_t lds_a = ((size_t)p.TA * padded(p.M1) + p.M1) * 8;
     size_t words_b = fuse ? std::max<size_t>(16 * padded(p.M2), FUSE_COLS * p.M2) : (size_t)p.TB * p.CB * padded(p.M2);
     size_t lds_b = (words_b + (a.tw_in_b ? p.M2 : 0)) * 8;
+    static std::mutex attr_mu;
     static bool attr_done[2][2][11] = {};
     int ia = ilog2u(p.M1), ib = ilog2u(p.M2), inv = s.inverse ? 1 : 0;
-    if (!attr_done[inv][0][ia]) {
-        (void)hipFuncSetAttribute((const void*)ka, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr_done[inv][0][ia] = true;
-    }
-    if (!attr_done[inv][1][ib]) {
-        (void)hipFuncSetAttribute((const void*)kb, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr_done[inv][1][ib] = true;
+    {
+        std::lock_guard<std::mutex> g(attr_mu);
+        if (!attr_done[inv][0][ia]) {
+            (void)hipFuncSetAttribute((const void*)ka, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            attr_done[inv][0][ia] = true;
+        }
+        if (!attr_done[inv][1][ib]) {
+            (void)hipFuncSetAttribute((const void*)kb, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            attr_done[inv][1][ib] = true;
+        }
     }
     dim3 ga((unsigned)(s.C * (p.M2 / p.TA)), 1, 1);
     dim3 gb((unsigned)((p.M1 / p.TB) * (s.C / p.CB)), 1, 1);
@@ -371,7 +392,7 @@ int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* s
         uint64_t g = std::min<uint64_t>(p.G, p.E - s0);
         ga.y = (unsigned)g;
         gb.y = (unsigned)g;
-        hipLaunchKernelGGL(ka, ga, dim3(THREADS), lds_a, st, a, s0);
+        if (!p.direct) hipLaunchKernelGGL(ka, ga, dim3(THREADS), lds_a, st, a, s0);
         hipLaunchKernelGGL(kb, gb, dim3(THREADS), lds_b, st, a, s0);
     }
     return 0;

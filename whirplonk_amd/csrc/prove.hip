@@ -1,9 +1,10 @@
 // prove.hip — kernels of the WHIR prove loop beyond commit and plain sumcheck.
 //
 //  * pow_search: the smallest nonce n with LE64(BLAKE3(ch || LE64(n) || 0^24))
-//    <= threshold (protocols/proof_of_work.hpp:95-193).  Each launch scans one
-//    window of nonces in full and keeps the minimum hit, so windows scanned in
-//    increasing order return the global minimum, as the reference's scan does.
+//    <= threshold (protocols/proof_of_work.hpp:95-193).  Each launch covers one
+//    window of nonces and keeps the minimum hit (a lane stops once the best
+//    hit is below its next nonce), so windows taken in increasing order return
+//    the global minimum, as the reference's scan does.
 //  * term tables + term_accumulate: acc[i] (+)= sum_t s_t * w_t(i) for terms
 //      eq terms   w(i) = prod_k (bit_k(i) ? x_k : 1 - x_k), x_0 the top bit
 //                 (eval_eq, algebra/multilinear.hpp:120-133)
@@ -72,7 +73,7 @@ __device__ __forceinline__ void block_sum256(e3* v) {
 // ------------------------------------------------------------------ PoW
 __global__ __launch_bounds__(256) void pow_search_kernel(const uint32_t* __restrict__ ch, uint64_t threshold,
                                                          uint64_t base, uint32_t per_thread,
-                                                         unsigned long long* __restrict__ best) {
+                                                         unsigned long long* best) {
     uint32_t m[16];
 #pragma unroll
     for (int i = 0; i < 8; ++i) m[i] = ch[i];
@@ -83,16 +84,20 @@ __global__ __launch_bounds__(256) void pow_search_kernel(const uint32_t* __restr
     for (int i = 0; i < 8; ++i) cv[i] = b3::iv(i);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     uint64_t nonce = base + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long found = ~0ull;
     for (uint32_t k = 0; k < per_thread; ++k, nonce += stride) {
+        // Each lane visits its nonces in increasing order, so once the best
+        // hit so far is below the lane's next nonce, the lane cannot improve it.
+        if (k && nonce > __hip_atomic_load(best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
         m[8] = (uint32_t)nonce;
         m[9] = (uint32_t)(nonce >> 32);
         uint32_t out[8];
         b3::compress(cv, m, 0, 64, b3::CHUNK_START | b3::CHUNK_END | b3::ROOT, out);
         uint64_t h = ((uint64_t)out[1] << 32) | out[0];
-        if (h <= threshold && nonce < found) found = nonce;
+        if (h <= threshold) {
+            atomicMin(best, (unsigned long long)nonce);
+            break;
+        }
     }
-    if (found != ~0ull) atomicMin(best, found);
 }
 
 // ------------------------------------------------------------ term tables

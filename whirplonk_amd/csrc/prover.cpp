@@ -9,6 +9,8 @@
 // and only messages, challenges, opened rows and path hashes cross PCIe.
 #include "../../include/whir_gpu.h"
 
+#include <chrono>
+#include <memory>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -43,6 +45,60 @@ struct Opened {
 
 uint32_t lg(uint64_t n) { return log2u(n); }
 
+size_t align256(size_t x) { return (x + 255) & ~size_t{255}; }
+
+// Device memory of one proof, carved from one region: the committed
+// codewords and trees (the initial one and two alternating round slots), the
+// folded vector and the covector, and the LDE scratch.  Offsets in bytes.
+struct Layout {
+    size_t init_matrix = 0, init_nodes = 0, rw_matrix[2] = {0, 0}, rw_nodes[2] = {0, 0}, vec = 0, cov = 0,
+           scratch = 0, scratch_bytes = 0, total = 0;
+};
+
+size_t lde_scratch_bytes(size_t vector_size, size_t depth, size_t num_vectors, size_t cw, bool ext) {
+    LdeShape s;
+    s.m = vector_size / depth;
+    s.C = (uint64_t)num_vectors * depth * (ext ? 3 : 1);
+    s.n = cw;
+    s.row_major = true;
+    if (ext) {
+        s.in_el = 3;
+        s.split3 = true;
+    }
+    return lde_scratch_elems(s) * 8;
+}
+
+Layout plan_layout(const whir_host::WhirConfig& cfg, size_t size) {
+    Layout L;
+    const auto& ic = cfg.initial;
+    size_t mat[2] = {0, 0}, nod[2] = {0, 0};
+    size_t scratch = lde_scratch_bytes(ic.vector_size, ic.depth, ic.num_vectors, ic.codeword_length, false);
+    for (size_t k = 0; k < cfg.rounds.size(); ++k) {
+        const auto& r = cfg.rounds[k].irs;
+        mat[k & 1] = std::max(mat[k & 1], r.codeword_length * r.num_cols() * 24);
+        nod[k & 1] = std::max(nod[k & 1], whir_merkle_num_nodes(r.codeword_length) * 32);
+        scratch = std::max(scratch, lde_scratch_bytes(r.vector_size, r.depth, 1, r.codeword_length, true));
+    }
+    size_t off = 0;
+    auto put = [&off](size_t bytes) {
+        size_t o = off;
+        off += align256(std::max<size_t>(bytes, 8));
+        return o;
+    };
+    L.init_matrix = put(ic.codeword_length * ic.num_cols() * 8);
+    L.init_nodes = put(whir_merkle_num_nodes(ic.codeword_length) * 32);
+    for (int k = 0; k < 2; ++k) {
+        L.rw_matrix[k] = put(mat[k]);
+        L.rw_nodes[k] = put(nod[k]);
+    }
+    L.vec = put((ic.num_vectors > 1 ? size : size / 2 + 1) * 24);
+    L.cov = put(size * 24);
+    L.scratch_bytes = align256(scratch);
+    L.scratch = put(scratch);
+    L.total = off;
+    return L;
+}
+
 // EngineId of the BLAKE3 hash engine (hash/blake3_engine.hpp:50-55): an
 // identifier fixed by the protocol, hashed into the domain separator.
 const uint8_t kBlake3EngineId[32] = {0x03, 0xe0, 0x17, 0x49, 0xeb, 0xcc, 0x04, 0x77, 0x92, 0x42, 0x54,
@@ -61,25 +117,43 @@ struct whir_prover {
     size_t num_vectors = 0;
     DevWitness initial;
     DevWitness rw[2];
+    Layout L;
+    uint8_t* region = nullptr;  // L.total bytes of device memory
+    bool own_region = false;
     bool committed = false, proved = false;
     std::vector<e3> eval_point, forms_rlc;
-    std::vector<void*> owned;
     size_t pow_hashes = 0;
+    // host wall time per phase (include the waits on the device)
+    double phase_ms[WHIR_PROVER_PHASES] = {};
+    int phase = WHIR_PROVER_PHASES - 1;
+    std::chrono::steady_clock::time_point t_last = std::chrono::steady_clock::now();
+
+    int enter(int k) {
+        auto now = std::chrono::steady_clock::now();
+        phase_ms[phase] += std::chrono::duration<double, std::milli>(now - t_last).count();
+        t_last = now;
+        int prev = phase;
+        phase = k;
+        return prev;
+    }
 
     ~whir_prover() {
         if (ctx) (void)hipStreamSynchronize(ctx->stream);
-        for (void* p : owned) (void)hipFree(p);
-    }
-
-    int alloc(size_t bytes, void** out) {
-        HIPCHK(hipMalloc(out, std::max<size_t>(bytes, 256)));
-        owned.push_back(*out);
-        return 0;
+        if (own_region && region) (void)hipFree(region);
     }
     hipStream_t st() const { return ctx->stream; }
 };
 
 namespace {
+
+// Attributes the wall time of a scope to one phase (nested scopes pause the outer one).
+struct Phase {
+    whir_prover* P;
+    int prev;
+    Phase(whir_prover* p, int k) : P(p), prev(p->enter(k)) {}
+    ~Phase() { P->enter(prev); }
+};
+enum { PH_COMMIT = 0, PH_OOD, PH_WEIGHTS, PH_SUMCHECK, PH_POW, PH_ROUND_COMMIT, PH_OPEN, PH_OTHER };
 
 int sync(whir_prover* P) {
     HIPCHK(hipStreamSynchronize(P->st()));
@@ -94,8 +168,14 @@ int pow_device(whir_ctx* ctx, const uint8_t ch[32], uint64_t threshold, uint64_t
     if ((rc = ctx->ensure(ctx->powbuf, 64))) return rc;
     uint64_t* d = (uint64_t*)ctx->powbuf.p;  // [0..4): challenge words, [4]: best
     HIPCHK(hipMemcpyAsync(d, ch, 32, hipMemcpyHostToDevice, ctx->stream));
-    const uint32_t blocks = 1024, per_thread = 16;
-    const uint64_t window = (uint64_t)blocks * 256 * per_thread;
+    // Window of candidates per launch: about 8x the expected work 2^d
+    // (d = 64 - log2(threshold)), between 2^14 and 2^24, at most 2^18 lanes.
+    uint32_t lg_expect = 0;
+    while (lg_expect < 64 && (threshold >> (63 - lg_expect)) == 0) ++lg_expect;  // floor(d)
+    const uint32_t lg_window = std::min<uint32_t>(24, std::max<uint32_t>(14, lg_expect + 3));
+    const uint32_t lg_lanes = std::min<uint32_t>(lg_window, 18);
+    const uint32_t blocks = 1u << (lg_lanes - 8), per_thread = 1u << (lg_window - lg_lanes);
+    const uint64_t window = 1ull << lg_window;
     for (uint64_t base = 0;; base += window) {
         set_u64_launch(d + 4, ~0ull, ctx->stream);
         pow_search_launch((const uint32_t*)d, threshold, base, blocks, per_thread, (unsigned long long*)(d + 4),
@@ -104,8 +184,8 @@ int pow_device(whir_ctx* ctx, const uint8_t ch[32], uint64_t threshold, uint64_t
         uint64_t best = ~0ull;
         HIPCHK(hipMemcpyAsync(&best, d + 4, 8, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
-        if (scanned) *scanned += window;
         if (best != ~0ull) {
+            if (scanned) *scanned += best + 1;  // the serial scan's work
             *nonce = best;
             return 0;
         }
@@ -115,6 +195,7 @@ int pow_device(whir_ctx* ctx, const uint8_t ch[32], uint64_t threshold, uint64_t
 
 int pow_prove(whir_prover* P, uint64_t threshold) {
     if (threshold == UINT64_MAX) return 0;
+    Phase ph(P, PH_POW);
     uint8_t ch[32];
     P->ts.squeeze(ch, 32);
     uint64_t nonce = 0;
@@ -155,7 +236,9 @@ struct Terms {
 uint32_t split_bits(uint32_t nv) { return nv < 10 ? nv : 10; }
 
 // acc[i] = (accumulate ? acc[i] : 0) + sum of the terms, i < 2^nv
-int terms_apply(whir_ctx* ctx, const Terms& T, uint64_t* d_acc, uint32_t nv, bool accumulate) {
+int terms_apply(whir_ctx* ctx, const Terms& T, uint64_t* d_acc, uint32_t nv, bool accumulate, whir_prover* P = nullptr) {
+    std::unique_ptr<Phase> ph;
+    if (P) ph.reset(new Phase(P, PH_WEIGHTS));
     const uint32_t L = split_bits(nv);
     const uint64_t H = 1ull << (nv - L), LO = 1ull << L;
     int rc;
@@ -206,6 +289,7 @@ int sumcheck_prove(whir_prover* P, const uint64_t* a, bool a_base, uint64_t* a_o
     whir_ctx* ctx = P->ctx;
     rs.clear();
     if (rounds == 0) return 0;
+    Phase ph(P, PH_SUMCHECK);
     int rc;
     if ((rc = ctx->ensure(ctx->partials, 6 * SUMCHECK_MAX_BLOCKS * 8))) return rc;
     uint64_t* part = (uint64_t*)ctx->partials.p;
@@ -244,20 +328,28 @@ int sumcheck_prove(whir_prover* P, const uint64_t* a, bool a_base, uint64_t* a_o
 // --------------------------------------------------------------- commit
 // irs_commit::Config::commit: encode + leaves + tree, root message, OOD points
 // squeezed, OOD values (point-major, vector-minor) sent.
-int commit_witness(whir_prover* P, const whir_host::IrsConfig& c, const uint64_t* d_vecs, bool ext, DevWitness& w) {
+int commit_witness(whir_prover* P, const whir_host::IrsConfig& c, const uint64_t* d_vecs, bool ext, DevWitness& w,
+                   size_t matrix_off, size_t nodes_off) {
     whir_ctx* ctx = P->ctx;
     int rc;
     w.rows = c.codeword_length;
     w.num_cols = c.num_cols();
     w.ext = ext;
-    size_t mbytes = w.rows * w.row_u64() * 8, nbytes = whir_merkle_num_nodes(w.rows) * 32;
-    if (!w.matrix && (rc = P->alloc(mbytes, (void**)&w.matrix))) return rc;
-    if (!w.nodes && (rc = P->alloc(nbytes, (void**)&w.nodes))) return rc;
+    w.matrix = (uint64_t*)(P->region + matrix_off);
+    w.nodes = P->region + nodes_off;
     uint8_t root[32];
-    if ((rc = commit_device(ctx, d_vecs, c.num_vectors, c.vector_size, c.codeword_length, c.depth, w.matrix, w.nodes,
-                            root, ext ? 1 : 0)))
-        return rc;
+    {
+        Phase ph(P, ext ? PH_ROUND_COMMIT : PH_COMMIT);
+        ctx->ext_scratch = P->region + P->L.scratch;
+        ctx->ext_scratch_bytes = P->L.scratch_bytes;
+        rc = commit_device(ctx, d_vecs, c.num_vectors, c.vector_size, c.codeword_length, c.depth, w.matrix, w.nodes,
+                           root, ext ? 1 : 0);
+        ctx->ext_scratch = nullptr;
+        ctx->ext_scratch_bytes = 0;
+        if (rc) return rc;
+    }
     P->ts.message(root, 32);
+    Phase ph(P, PH_OOD);
     w.ood_points.clear();
     w.ood_values.clear();
     for (size_t k = 0; k < c.out_domain; ++k) w.ood_points.push_back(P->ts.challenge_e3());
@@ -279,6 +371,7 @@ int commit_witness(whir_prover* P, const whir_host::IrsConfig& c, const uint64_t
 int open_witness(whir_prover* P, const whir_host::IrsConfig& c, const DevWitness& w, Opened& o) {
     whir_ctx* ctx = P->ctx;
     int rc;
+    Phase ph(P, PH_OPEN);
     std::vector<size_t> idx = P->ts.indices(c.codeword_length, c.in_domain, false);
     const size_t k = idx.size(), ru = w.row_u64();
     const uint64_t gen = gl::root_of_unity(lg(c.codeword_length));
@@ -346,11 +439,8 @@ int prove_impl(whir_prover* P, size_t num_forms, const uint64_t* points, const u
     P->forms_rlc.assign(crlc.begin(), crlc.begin() + std::min(num_forms, crlc.size()));
 
     // vector and covector buffers
-    uint64_t *vec = nullptr, *cov = nullptr;
-    if ((rc = ctx->ensure(ctx->pvec, (nvec > 1 ? n : n / 2 + 1) * 24))) return rc;
-    if ((rc = ctx->ensure(ctx->pcov, n * 24))) return rc;
-    vec = (uint64_t*)ctx->pvec.p;
-    cov = (uint64_t*)ctx->pcov.p;
+    uint64_t* vec = (uint64_t*)(P->region + P->L.vec);
+    uint64_t* cov = (uint64_t*)(P->region + P->L.cov);
     const uint64_t* a = P->d_vectors;
     bool a_base = true;
     if (nvec > 1) {
@@ -376,7 +466,7 @@ int prove_impl(whir_prover* P, size_t num_forms, const uint64_t* points, const u
         for (size_t f = 0; f < num_forms; ++f) T.eq(points + 3 * nv * f, nv, crlc[f]);
         for (size_t o = 0; o < P->initial.ood_points.size(); ++o)
             T.power(P->initial.ood_points[o], crlc[num_forms + o], L);
-        if ((rc = terms_apply(ctx, T, cov, nv, false))) return rc;
+        if ((rc = terms_apply(ctx, T, cov, nv, false, P))) return rc;
     }
 
     // the_sum (whir_prover.hpp:171-197)
@@ -429,7 +519,7 @@ int prove_impl(whir_prover* P, size_t num_forms, const uint64_t* points, const u
     for (size_t k = 0; k < cfg.rounds.size(); ++k) {
         const auto& rc_cfg = cfg.rounds[k];
         DevWitness& w = P->rw[k & 1];
-        if ((rc = commit_witness(P, rc_cfg.irs, vec, true, w))) return rc;
+        if ((rc = commit_witness(P, rc_cfg.irs, vec, true, w, P->L.rw_matrix[k & 1], P->L.rw_nodes[k & 1]))) return rc;
         if ((rc = pow_prove(P, rc_cfg.pow))) return rc;
         Opened o;
         bool o_ext;
@@ -464,7 +554,7 @@ int prove_impl(whir_prover* P, size_t num_forms, const uint64_t* points, const u
             for (size_t j = 0; j < w.ood_points.size(); ++j) T.power(w.ood_points[j], srlc[j], L);
             for (size_t j = 0; j < o.points.size(); ++j)
                 T.power(e3{o.points[j], 0, 0}, srlc[w.ood_points.size() + j], L);
-            if (nstir && (rc = terms_apply(ctx, T, cov, lv, true))) return rc;
+            if (nstir && (rc = terms_apply(ctx, T, cov, lv, true, P))) return rc;
         }
         for (size_t j = 0; j < nstir; ++j) sum = gl::e3_add(sum, gl::e3_mul(srlc[j], stir_eval[j]));
         // round sumcheck
@@ -589,8 +679,9 @@ int whir_domain_separator_cli(const whir_params* p, const char* session, uint8_t
     return 0;
 }
 
-int whir_prover_create(whir_ctx* ctx, const whir_params* params, uint64_t size, const uint8_t* protocol_id,
-                       const uint8_t* session_id, const uint8_t* instance, size_t instance_len, whir_prover** out) {
+static int prover_create(whir_ctx* ctx, const whir_params* params, uint64_t size, const uint8_t* protocol_id,
+                         const uint8_t* session_id, const uint8_t* instance, size_t instance_len, void* d_workspace,
+                         size_t workspace_bytes, whir_prover** out) {
     if (!ctx || !out || !protocol_id || !session_id) return bad_arg("null argument");
     int rc = check_params(params, size);
     if (rc) return rc;
@@ -599,9 +690,43 @@ int whir_prover_create(whir_ctx* ctx, const whir_params* params, uint64_t size, 
     P->params = to_params(params);
     P->size = size;
     P->cfg = whir_host::WhirConfig::derive(size, P->params);
+    P->L = plan_layout(P->cfg, size);
+    if (d_workspace) {
+        if (workspace_bytes < P->L.total || ((uintptr_t)d_workspace & 255)) {
+            delete P;
+            return bad_arg("workspace too small or not 256-byte aligned");
+        }
+        P->region = (uint8_t*)d_workspace;
+    } else {
+        hipError_t e = hipMalloc((void**)&P->region, P->L.total);
+        if (e != hipSuccess) {
+            delete P;
+            return fail("hipMalloc(prover region)", e);
+        }
+        P->own_region = true;
+    }
     P->ts.start(protocol_id, session_id, instance, instance_len);
     *out = P;
     return 0;
+}
+
+int whir_prover_create(whir_ctx* ctx, const whir_params* params, uint64_t size, const uint8_t* protocol_id,
+                       const uint8_t* session_id, const uint8_t* instance, size_t instance_len, whir_prover** out) {
+    return prover_create(ctx, params, size, protocol_id, session_id, instance, instance_len, nullptr, 0, out);
+}
+
+int whir_prover_create_in(whir_ctx* ctx, const whir_params* params, uint64_t size, const uint8_t* protocol_id,
+                          const uint8_t* session_id, const uint8_t* instance, size_t instance_len, void* d_workspace,
+                          size_t workspace_bytes, whir_prover** out) {
+    if (!d_workspace) return bad_arg("null workspace");
+    return prover_create(ctx, params, size, protocol_id, session_id, instance, instance_len, d_workspace,
+                         workspace_bytes, out);
+}
+
+size_t whir_prover_workspace_bytes(const whir_params* params, uint64_t size) {
+    if (check_params(params, size)) return 0;
+    auto cfg = whir_host::WhirConfig::derive(size, to_params(params));
+    return plan_layout(cfg, size).total;
 }
 
 int whir_prover_destroy(whir_prover* P) {
@@ -618,7 +743,7 @@ int whir_prover_commit(whir_prover* P, const uint64_t* d_vectors, size_t num_vec
     std::lock_guard<std::recursive_mutex> g(P->ctx->mu);
     P->d_vectors = d_vectors;
     P->num_vectors = num_vectors;
-    int rc = commit_witness(P, P->cfg.initial, d_vectors, false, P->initial);
+    int rc = commit_witness(P, P->cfg.initial, d_vectors, false, P->initial, P->L.init_matrix, P->L.init_nodes);
     if (rc) return rc;
     P->committed = true;
     return 0;
@@ -653,6 +778,13 @@ int whir_prover_proof(whir_prover* P, uint8_t* narg, size_t narg_cap, size_t* na
 }
 
 size_t whir_prover_pow_hashes(const whir_prover* P) { return P ? P->pow_hashes : 0; }
+
+int whir_prover_phase_times(whir_prover* P, double* out_ms) {
+    if (!P || !out_ms) return bad_arg("null argument");
+    P->enter(P->phase);  // close the running interval
+    for (int k = 0; k < WHIR_PROVER_PHASES; ++k) out_ms[k] = P->phase_ms[k];
+    return 0;
+}
 
 // ------------------------------------------------------------ transcript
 

@@ -1,0 +1,351 @@
+// server.cpp — a serving loop for commit+prove requests (C ABI: whir_server_*).
+//
+// Requests of different sizes share one parameter set.  A request is admitted
+// when a slot is idle and its device footprint (the prover's workspace plus
+// its coefficients) can be carved from a fixed HBM arena of the configured
+// budget; otherwise it waits, and smaller requests behind it may pass it
+// until it has been passed too often.  Each slot is a worker thread with its
+// own context and HIP stream, so admitted requests run concurrently on the
+// device: their kernels, copies and host transcript work overlap, and the
+// codewords, trees and vectors of each request stay resident in its arena
+// block across all WHIR rounds.  Results are byte-identical to the plain
+// single-request path (the same prover code runs on the same inputs).
+#include "../../include/whir_gpu.h"
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "ctx.h"
+
+using namespace whir_gpu;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+// First-fit allocator over one device region, with coalescing on free.
+class Arena {
+public:
+    void init(uint8_t* base, size_t bytes) {
+        base_ = base;
+        free_.clear();
+        free_[0] = bytes;
+    }
+    bool alloc(size_t bytes, size_t* off) {
+        bytes = (bytes + 255) & ~size_t{255};
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                *off = it->first;
+                size_t rest = it->second - bytes, at = it->first + bytes;
+                free_.erase(it);
+                if (rest) free_[at] = rest;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes) {
+        bytes = (bytes + 255) & ~size_t{255};
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    uint8_t* at(size_t off) const { return base_ + off; }
+
+private:
+    uint8_t* base_ = nullptr;
+    std::map<size_t, size_t> free_;
+};
+
+struct Request {
+    uint64_t id = 0;
+    uint64_t size = 0;
+    size_t num_forms = 0;
+    std::vector<uint64_t> coeffs, points, evals;
+    bool have_evals = false;
+    size_t footprint = 0, ws_bytes = 0, coeff_bytes = 0, arena_off = 0;
+    size_t skipped = 0;
+    bool done = false;
+    int status = 0;
+    std::string error;
+    std::vector<uint8_t> narg, hints;
+    Clock::time_point t_submit, t_admit, t_done;
+};
+
+struct Slot {
+    whir_ctx* ctx = nullptr;
+    uint8_t* pinned = nullptr;
+    size_t pinned_bytes = 0;
+    std::thread th;
+};
+
+}  // namespace
+
+struct whir_server {
+    int device = 0;
+    whir_params params{};
+    uint8_t protocol_id[64], session_id[32];
+    size_t budget = 0, in_use = 0, peak = 0;
+    uint8_t* region = nullptr;
+    Arena arena;
+    std::vector<std::unique_ptr<Slot>> slots;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> queue;
+    std::map<uint64_t, std::unique_ptr<Request>> reqs;
+    uint64_t next_id = 1;
+    size_t completed = 0, running = 0;
+    bool stop = false;
+
+    size_t footprint(uint64_t size, size_t* ws, size_t* cb) const {
+        *ws = whir_prover_workspace_bytes(&params, size);
+        *cb = ((size_t)params.batch_size * size * 8 + 255) & ~size_t{255};
+        return *ws ? *ws + *cb : 0;
+    }
+
+    // the first admissible request (and its arena block), or null
+    Request* pick() {
+        const size_t max_skips = 4 * slots.size();
+        for (size_t i = 0; i < queue.size(); ++i) {
+            Request* r = queue[i];
+            size_t off;
+            if (arena.alloc(r->footprint, &off)) {
+                r->arena_off = off;
+                for (size_t j = 0; j < i; ++j) queue[j]->skipped++;
+                queue.erase(queue.begin() + (long)i);
+                return r;
+            }
+            if (r->skipped >= max_skips) return nullptr;  // keep its place: no more passing
+        }
+        return nullptr;
+    }
+
+    void run(Slot& s, Request& r);
+    void worker(Slot* s);
+};
+
+void whir_server::run(Slot& s, Request& r) {
+    auto fail_with = [&r](int rc) {
+        r.status = rc;
+        r.error = last_error();
+    };
+    uint8_t* blk = arena.at(r.arena_off);
+    uint64_t* d_coeffs = (uint64_t*)(blk + r.ws_bytes);
+    // stage the coefficients through pinned memory
+    const size_t cbytes = r.coeffs.size() * 8;
+    if (s.pinned_bytes < cbytes) {
+        if (s.pinned) (void)hipHostFree(s.pinned);
+        s.pinned = nullptr;
+        s.pinned_bytes = 0;
+        hipError_t e = hipHostMalloc((void**)&s.pinned, cbytes, hipHostMallocDefault);
+        if (e != hipSuccess) return fail_with(fail("hipHostMalloc", e));
+        s.pinned_bytes = cbytes;
+    }
+    std::memcpy(s.pinned, r.coeffs.data(), cbytes);
+    hipError_t e = hipMemcpyAsync(d_coeffs, s.pinned, cbytes, hipMemcpyHostToDevice, s.ctx->stream);
+    if (e != hipSuccess) return fail_with(fail("hipMemcpyAsync", e));
+    const uint32_t nv = log2u(r.size);
+    const size_t nvec = params.batch_size;
+    int rc;
+    if (!r.have_evals) {
+        r.evals.assign(3 * r.num_forms * nvec, 0);
+        for (size_t f = 0; f < r.num_forms; ++f)
+            for (size_t v = 0; v < nvec; ++v)
+                if ((rc = whir_dev_mle_evaluate(s.ctx, d_coeffs + v * r.size, 1, nv, r.points.data() + 3 * nv * f,
+                                                r.evals.data() + 3 * (f * nvec + v))))
+                    return fail_with(rc);
+    }
+    whir_prover* P = nullptr;
+    if ((rc = whir_prover_create_in(s.ctx, &params, r.size, protocol_id, session_id, nullptr, 0, blk, r.ws_bytes,
+                                    &P)))
+        return fail_with(rc);
+    std::unique_ptr<whir_prover, int (*)(whir_prover*)> guard(P, whir_prover_destroy);
+    if ((rc = whir_prover_commit(P, d_coeffs, nvec))) return fail_with(rc);
+    if ((rc = whir_prover_prove(P, r.num_forms, r.num_forms ? r.points.data() : nullptr,
+                                r.num_forms ? r.evals.data() : nullptr, nullptr, nullptr)))
+        return fail_with(rc);
+    size_t na = 0, nh = 0;
+    whir_prover_proof(P, nullptr, 0, &na, nullptr, 0, &nh);
+    r.narg.resize(na);
+    r.hints.resize(nh);
+    if ((rc = whir_prover_proof(P, r.narg.data(), na, &na, r.hints.data(), nh, &nh))) return fail_with(rc);
+    r.status = 0;
+}
+
+void whir_server::worker(Slot* s) {
+    (void)hipSetDevice(device);
+    for (;;) {
+        Request* r = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_work.wait(lk, [&] { return stop || (r = pick()) != nullptr; });
+            if (!r) return;  // stopping with nothing admissible
+            r->t_admit = Clock::now();
+            in_use += r->footprint;
+            peak = std::max(peak, in_use);
+            ++running;
+        }
+        run(*s, *r);
+        r->coeffs = std::vector<uint64_t>();  // inputs are no longer needed
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            arena.release(r->arena_off, r->footprint);
+            in_use -= r->footprint;
+            --running;
+            r->t_done = Clock::now();
+            r->done = true;
+            ++completed;
+        }
+        cv_done.notify_all();
+        cv_work.notify_all();  // freed memory may admit a waiting request
+    }
+}
+
+extern "C" {
+
+int whir_server_create(int device, const whir_params* params, uint32_t num_slots, size_t hbm_budget_bytes,
+                       whir_server** out) {
+    if (!params || !out || num_slots == 0 || hbm_budget_bytes == 0) return bad_arg("bad server arguments");
+    if (params->hash_id != WHIR_HASH_BLAKE3) return bad_arg("only the BLAKE3 engine is supported");
+    HIPCHK(hipSetDevice(device));
+    std::unique_ptr<whir_server> S(new whir_server());
+    S->device = device;
+    S->params = *params;
+    int rc = whir_domain_separator_cli(params, nullptr, S->protocol_id, S->session_id);
+    if (rc) return rc;
+    S->budget = (hbm_budget_bytes + 255) & ~size_t{255};
+    HIPCHK(hipMalloc((void**)&S->region, S->budget));
+    S->arena.init(S->region, S->budget);
+    for (uint32_t i = 0; i < num_slots; ++i) {
+        auto sl = std::make_unique<Slot>();
+        if ((rc = whir_ctx_create(device, &sl->ctx))) {
+            for (auto& x : S->slots) whir_ctx_destroy(x->ctx);
+            (void)hipFree(S->region);
+            return rc;
+        }
+        S->slots.push_back(std::move(sl));
+    }
+    for (auto& sl : S->slots) sl->th = std::thread(&whir_server::worker, S.get(), sl.get());
+    *out = S.release();
+    return 0;
+}
+
+int whir_server_destroy(whir_server* S) {
+    if (!S) return 0;
+    {
+        std::unique_lock<std::mutex> lk(S->mu);
+        // let everything queued finish first
+        S->cv_done.wait(lk, [&] { return S->queue.empty() && S->running == 0; });
+        S->stop = true;
+    }
+    S->cv_work.notify_all();
+    for (auto& sl : S->slots) {
+        if (sl->th.joinable()) sl->th.join();
+        whir_ctx_destroy(sl->ctx);
+        if (sl->pinned) (void)hipHostFree(sl->pinned);
+    }
+    (void)hipFree(S->region);
+    delete S;
+    return 0;
+}
+
+size_t whir_server_request_bytes(whir_server* S, uint64_t size) {
+    size_t ws, cb;
+    return S ? S->footprint(size, &ws, &cb) : 0;
+}
+
+int whir_server_submit(whir_server* S, uint64_t size, const uint64_t* coeffs, size_t num_forms, const uint64_t* points,
+                       const uint64_t* evaluations, uint64_t* out_id) {
+    if (!S || !coeffs || !out_id || (num_forms && !points)) return bad_arg("null argument");
+    auto r = std::make_unique<Request>();
+    r->size = size;
+    r->footprint = S->footprint(size, &r->ws_bytes, &r->coeff_bytes);
+    if (!r->footprint) return bad_arg("invalid request size");
+    if (r->footprint > S->budget) return bad_arg("request larger than the HBM budget");
+    const size_t nvec = S->params.batch_size;
+    const uint32_t nv = log2u(size);
+    r->coeffs.assign(coeffs, coeffs + nvec * size);
+    r->num_forms = num_forms;
+    if (num_forms) r->points.assign(points, points + 3 * nv * num_forms);
+    if (evaluations) {
+        r->evals.assign(evaluations, evaluations + 3 * num_forms * nvec);
+        r->have_evals = true;
+    }
+    r->t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        r->id = S->next_id++;
+        *out_id = r->id;
+        S->queue.push_back(r.get());
+        S->reqs[r->id] = std::move(r);
+    }
+    S->cv_work.notify_all();
+    return 0;
+}
+
+int whir_server_wait(whir_server* S, uint64_t id, uint8_t* narg, size_t narg_cap, size_t* narg_len, uint8_t* hints,
+                     size_t hints_cap, size_t* hints_len, double* out_ms) {
+    if (!S) return bad_arg("null server");
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end()) return bad_arg("unknown request id");
+    Request* r = it->second.get();
+    S->cv_done.wait(lk, [&] { return r->done; });
+    if (narg_len) *narg_len = r->narg.size();
+    if (hints_len) *hints_len = r->hints.size();
+    if (out_ms) {
+        out_ms[0] = ms_between(r->t_submit, r->t_admit);
+        out_ms[1] = ms_between(r->t_admit, r->t_done);
+        out_ms[2] = ms_between(r->t_submit, r->t_done);
+    }
+    if (r->status) {
+        last_error() = r->error;
+        int st = r->status;
+        S->reqs.erase(it);
+        return st;
+    }
+    if ((narg && r->narg.size() > narg_cap) || (hints && r->hints.size() > hints_cap))
+        return bad_arg("proof buffer too small");  // the result is kept for another call
+    if (narg) std::memcpy(narg, r->narg.data(), r->narg.size());
+    if (hints) std::memcpy(hints, r->hints.data(), r->hints.size());
+    if (narg || hints) S->reqs.erase(it);
+    return 0;
+}
+
+int whir_server_stats(whir_server* S, uint64_t* out) {
+    if (!S || !out) return bad_arg("null argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    out[0] = S->completed;
+    out[1] = S->running;
+    out[2] = S->queue.size();
+    out[3] = S->in_use;
+    out[4] = S->peak;
+    out[5] = S->budget;
+    out[6] = S->slots.size();
+    return 0;
+}
+
+}  // extern "C"

@@ -88,9 +88,18 @@ int run_lde(whir_ctx* ctx, LdeShape s, const uint64_t* in, uint64_t* out, uint8_
     Twiddles tw;
     int rc = ctx->twiddles(log2u(s.n), s.inverse, tw);
     if (rc) return rc;
-    rc = ctx->ensure(ctx->scratch, lde_scratch_elems(s) * 8);
-    if (rc) return rc;
-    rc = lde_launch(s, in, out, (uint64_t*)ctx->scratch.p, tw, ctx->stream, leaves, fused);
+    const size_t need = lde_scratch_elems(s) * 8;
+    uint64_t* scratch;
+    if (need == 0) {
+        scratch = nullptr;
+    } else if (ctx->ext_scratch) {
+        if (need > ctx->ext_scratch_bytes) return bad_arg("workspace scratch too small for this LDE");
+        scratch = (uint64_t*)ctx->ext_scratch;
+    } else {
+        if ((rc = ctx->ensure(ctx->scratch, need))) return rc;
+        scratch = (uint64_t*)ctx->scratch.p;
+    }
+    rc = lde_launch(s, in, out, scratch, tw, ctx->stream, leaves, fused);
     if (rc > 0) return bad_arg("LDE shape not supported");
     HIPCHK(hipGetLastError());
     return 0;
@@ -105,15 +114,11 @@ int rs_encode_any(whir_ctx* ctx, const uint64_t* d_in, size_t num_polys, size_t 
     int rc = rs_shape(num_polys, poly_size, cw, depth, s);
     if (rc) return rc;
     if (!ext) return run_lde(ctx, s, d_in, d_out, leaves, fused);
+    // the three components of every Ext3 column in one transform
+    s.C *= 3;
     s.in_el = 3;
-    s.out_el = 3;
-    for (uint32_t c = 0; c < 3; ++c) {
-        s.in_comp = c;
-        s.out_comp = c;
-        rc = run_lde(ctx, s, d_in, d_out);
-        if (rc) return rc;
-    }
-    return 0;
+    s.split3 = true;
+    return run_lde(ctx, s, d_in, d_out);
 }
 
 template <typename F>
@@ -210,8 +215,7 @@ int whir_ctx_destroy(whir_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (DevBuf* b : {&ctx->scratch, &ctx->partials, &ctx->zpow, &ctx->hin, &ctx->hout, &ctx->hnodes, &ctx->hidx,
-                      &ctx->hcomp, &ctx->powbuf, &ctx->tdesc, &ctx->ttab, &ctx->oidx, &ctx->orows, &ctx->pvec,
-                      &ctx->pcov})
+                      &ctx->hcomp, &ctx->powbuf, &ctx->tdesc, &ctx->ttab, &ctx->oidx, &ctx->orows})
         if (b->p) (void)hipFree(b->p);
     for (auto& kv : ctx->tw) {
         (void)hipFree(kv.second.lo);
