@@ -327,6 +327,44 @@ int whir_dev_geometric_accumulate(whir_ctx* ctx, uint64_t* d_acc, size_t n, cons
 int whir_dev_mle_evaluate(whir_ctx* ctx, const uint64_t* d_f, int f_is_base, size_t num_variables,
                           const uint64_t* point, uint64_t* out);
 
+/* --------------------------------------------------------- batched commit */
+
+/* LDE autotuner: times every valid radix split m = M1 * M2 (M2 = the second
+ * pass's DFT size, which is also the number of rows, and so of row leaves,
+ * each second-pass workgroup produces) for one commit shape (message length,
+ * codeword rows, interleaved columns, base or Ext3) and records the fastest
+ * for the process; later encodes of that shape use it.  Every split gives
+ * the same codeword. */
+int whir_autotune_lde(whir_ctx* ctx, size_t message_length, size_t codeword_length, size_t columns, int ext3,
+                      double* out_best_ms, uint32_t* out_lg_m2);
+/* The tuning table as 9-word records (shape key, log2 M2), for saving and
+ * loading across processes.  export returns the word count. */
+size_t whir_tuning_export(uint64_t* out, size_t cap);
+int whir_tuning_import(const uint64_t* records, size_t words);
+
+/* Many independent base-field polynomials of (possibly different) power-of-two
+ * sizes, each committed as irs_commit::Config::commit does (RS at rate
+ * 2^-log_inv_rate with interleaving depth 2^folding_factor, BLAKE3 row
+ * leaves, Merkle tree; irs_commit.hpp:344-460).  create plans one device
+ * region for all of them, autotunes each shape (autotune != 0), and captures
+ * every encode, leaf and tree kernel, spread over num_branches streams, plus a
+ * final root gather, into one HIP graph.  Fill the coefficients
+ * (whir_batch_coeffs), then each whir_batch_run is one graph launch on the
+ * context's stream; whir_batch_roots waits and copies the count x 32 roots. */
+typedef struct whir_batch whir_batch;
+int whir_batch_create(whir_ctx* ctx, size_t count, const uint64_t* sizes, uint32_t log_inv_rate,
+                      uint32_t folding_factor, uint32_t num_branches, int autotune, whir_batch** out);
+int whir_batch_destroy(whir_batch* batch);
+uint64_t* whir_batch_coeffs(whir_batch* batch, size_t k);
+uint64_t* whir_batch_matrix(whir_batch* batch, size_t k);
+uint8_t* whir_batch_nodes(whir_batch* batch, size_t k);
+size_t whir_batch_rows(whir_batch* batch, size_t k);
+size_t whir_batch_device_bytes(whir_batch* batch);
+int whir_batch_run(whir_batch* batch);
+/* The same work enqueued directly, without the graph. */
+int whir_batch_run_eager(whir_batch* batch);
+int whir_batch_roots(whir_batch* batch, uint8_t* out_roots);
+
 /* --------------------------------------------------------------- serving */
 
 /* A serving loop for commit+prove requests of any size under one parameter

@@ -1,0 +1,111 @@
+"""Batched commits in one graph launch, and the LDE autotuner: every radix
+split gives the same codeword, and every graph replay the same roots as the
+eager run and as per-polynomial commits of the reference build / oracle."""
+import random
+
+import numpy as np
+import pytest
+
+import oracle
+import whirplonk_amd as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = W.Context()
+    yield c
+    c.close()
+
+
+def _commit_ref(coeffs, rate=2, ff=4):
+    """Root of one polynomial's commit: the reference build when present,
+    else the oracle's restatement."""
+    n = coeffs.size
+    depth = 1 << ff
+    cw = (n // depth) << rate
+    R = oracle.ref()
+    if R is not None:
+        root = np.zeros(32, dtype=np.uint8)
+        R.ref_commit_base(oracle._ptr64(coeffs), n, cw, depth, None, oracle._ptr8(root))
+        return root.tobytes()
+    m = oracle.rs_encode(coeffs, 1, n, cw, depth)
+    leaves = oracle.commit_leaves_base(m, cw, depth)
+    return oracle.merkle_build(leaves, cw)[-32:].tobytes()
+
+
+def _polys(sizes, seed):
+    out, st = [], seed
+    for s in sizes:
+        c, st = W.lcg_uniform(st, s)
+        out.append(c)
+    return out
+
+
+def test_batch_roots_and_replays(ctx):
+    rnd = random.Random(3)
+    sizes = [1 << rnd.randint(8, 16) for _ in range(20)] + [1 << 12, 1 << 12]
+    polys = _polys(sizes, 0xBA7C)
+    print("batch", sizes, flush=True)
+    with W.Batch(ctx, sizes, log_inv_rate=2, folding_factor=4, branches=4) as b:
+        for k, c in enumerate(polys):
+            b.upload(k, c)
+        b.run(graph=False)
+        eager = b.roots()
+        assert eager == [_commit_ref(c) for c in polys]
+        # the codeword of one polynomial against the oracle
+        k = sizes.index(1 << 12)
+        assert np.array_equal(b.matrix(k), oracle.rs_encode(polys[k], 1, sizes[k], (sizes[k] // 16) << 2, 16))
+        # graph replays, with the program's own eager work in between
+        scratch = ctx.upload(polys[0])
+        d_m = ctx.alloc(8 * (sizes[0] // 4) * 16)
+        d_n = ctx.alloc(32 * W.merkle_num_nodes(sizes[0] // 4))
+        for rep in range(3):
+            b.run(graph=True)
+            assert b.roots() == eager, rep
+            ctx.commit(scratch, 1, sizes[0], sizes[0] // 4, 16, d_m, d_n)
+            ctx.sync()
+        # new coefficients are picked up by the next replay
+        newc, _ = W.lcg_uniform(77, sizes[1])
+        b.upload(1, newc)
+        b.run(graph=True)
+        r = b.roots()
+        assert r[1] == _commit_ref(newc) and r[0] == eager[0] and r[2:] == eager[2:]
+
+
+def _force(rec_key, lg):
+    W.tuning_import(np.array(list(rec_key) + [lg], dtype=np.uint64))
+
+
+@pytest.mark.parametrize("m,n,ext3", [(1 << 14, 1 << 16, False), (1 << 12, 1 << 17, True), (1 << 16, 1 << 18, False)])
+def test_every_split_same_codeword(ctx, m, n, ext3):
+    depth = 16
+    coeffs, _ = W.lcg_uniform(m + n, m * depth * (3 if ext3 else 1))
+    d_in = ctx.upload(coeffs)
+    d_out = ctx.alloc(8 * n * depth * (3 if ext3 else 1))
+    ctx.rs_encode(d_in, 1, m * depth, n, depth, d_out, ext3=ext3)
+    want = ctx.download(d_out, n * depth * (3 if ext3 else 1))
+    if not ext3:
+        assert np.array_equal(want, oracle.rs_encode(coeffs, 1, m * depth, n, depth))
+    else:
+        assert np.array_equal(want, oracle.rs_encode_ext3(coeffs, 1, m * depth, n, depth))
+    key = (m, n, depth * 3, 3, 1, 1, 1, 0) if ext3 else (m, n, depth, 1, 1, 0, 1, 0)
+    default = min(10, m.bit_length() - 1)
+    tried = 0
+    for lg in range(default, 0, -1):
+        if (m >> lg) > 1024:
+            break
+        _force(key, lg)
+        ctx.rs_encode(d_in, 1, m * depth, n, depth, d_out, ext3=ext3)
+        got = ctx.download(d_out, n * depth * (3 if ext3 else 1))
+        assert np.array_equal(got, want), lg
+        tried += 1
+    assert tried >= 2
+    _force(key, default)
+    ms, lg = W.autotune_lde(ctx, m, n, depth, ext3)
+    assert ms > 0 and (m >> lg) <= 1024
+    ctx.rs_encode(d_in, 1, m * depth, n, depth, d_out, ext3=ext3)
+    assert np.array_equal(ctx.download(d_out, n * depth * (3 if ext3 else 1)), want)
+    ctx.free(d_in)
+    ctx.free(d_out)

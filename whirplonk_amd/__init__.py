@@ -114,6 +114,21 @@ _SIGS = [
     ("whir_dev_eq_accumulate", _int, [_vp, _vp, _sz, _u64p, _u64p, _int]),
     ("whir_dev_geometric_accumulate", _int, [_vp, _vp, _sz, _u64p, _u64p, _sz]),
     ("whir_dev_mle_evaluate", _int, [_vp, _vp, _int, _sz, _u64p, _u64p]),
+    ("whir_autotune_lde", _int, [_vp, _sz, _sz, _sz, _int, ctypes.POINTER(ctypes.c_double),
+                                 ctypes.POINTER(ctypes.c_uint32)]),
+    ("whir_tuning_export", _sz, [_u64p, _sz]),
+    ("whir_tuning_import", _int, [_u64p, _sz]),
+    ("whir_batch_create", _int, [_vp, _sz, _u64p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _int,
+                                 ctypes.POINTER(_vp)]),
+    ("whir_batch_destroy", _int, [_vp]),
+    ("whir_batch_coeffs", _vp, [_vp, _sz]),
+    ("whir_batch_matrix", _vp, [_vp, _sz]),
+    ("whir_batch_nodes", _vp, [_vp, _sz]),
+    ("whir_batch_rows", _sz, [_vp, _sz]),
+    ("whir_batch_device_bytes", _sz, [_vp]),
+    ("whir_batch_run", _int, [_vp]),
+    ("whir_batch_run_eager", _int, [_vp]),
+    ("whir_batch_roots", _int, [_vp, _u8p]),
     ("whir_server_create", _int, [_int, _vp, ctypes.c_uint32, _sz, ctypes.POINTER(_vp)]),
     ("whir_server_destroy", _int, [_vp]),
     ("whir_server_request_bytes", _sz, [_vp, ctypes.c_uint64]),
@@ -577,6 +592,69 @@ class Prover:
         ms = (ctypes.c_double * len(self.PHASES))()
         _check(load().whir_prover_phase_times(self.h, ms))
         return {k: ms[i] for i, k in enumerate(self.PHASES)}
+
+
+def autotune_lde(ctx: "Context", m: int, n: int, columns: int = 16, ext3: bool = False) -> tuple[float, int]:
+    ms = ctypes.c_double()
+    lg = ctypes.c_uint32()
+    _check(load().whir_autotune_lde(ctx.h, m, n, columns, int(ext3), ctypes.byref(ms), ctypes.byref(lg)))
+    return ms.value, lg.value
+
+
+def tuning_export() -> np.ndarray:
+    n = load().whir_tuning_export(None, 0)
+    out = np.zeros(max(n, 1), dtype=np.uint64)
+    load().whir_tuning_export(_p64(out), n)
+    return out[:n]
+
+
+def tuning_import(records) -> None:
+    r = _u64(records)
+    _check(load().whir_tuning_import(_p64(r), r.size))
+
+
+class Batch:
+    """Batched commits of many polynomials in one graph launch (whir_batch_*)."""
+
+    def __init__(self, ctx: "Context", sizes, log_inv_rate: int = 2, folding_factor: int = 4,
+                 branches: int = 8, autotune: bool = True):
+        self.ctx = ctx
+        self.sizes = [int(s) for s in sizes]
+        sz = _u64(self.sizes)
+        h = _vp()
+        _check(load().whir_batch_create(ctx.h, len(self.sizes), _p64(sz), log_inv_rate, folding_factor, branches,
+                                        int(autotune), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            load().whir_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, k: int, coeffs):
+        c = _u64(coeffs)
+        _check(load().whir_memcpy_h2d(self.ctx.h, load().whir_batch_coeffs(self.h, k), c.ctypes.data, c.nbytes))
+
+    def matrix(self, k: int) -> np.ndarray:
+        rows = load().whir_batch_rows(self.h, k)
+        return self.ctx.download(load().whir_batch_matrix(self.h, k), rows * 16)
+
+    def run(self, graph: bool = True):
+        _check((load().whir_batch_run if graph else load().whir_batch_run_eager)(self.h))
+
+    def roots(self) -> list[bytes]:
+        out = np.zeros(32 * len(self.sizes), dtype=np.uint8)
+        _check(load().whir_batch_roots(self.h, _p8(out)))
+        return [out[32 * k:32 * k + 32].tobytes() for k in range(len(self.sizes))]
+
+    def device_bytes(self) -> int:
+        return load().whir_batch_device_bytes(self.h)
 
 
 class Server:

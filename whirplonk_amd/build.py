@@ -20,7 +20,7 @@ LIB = os.path.join(OUT, "libwhir_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["lde.hip", "merkle.hip", "sumcheck.hip", "prove.hip", "whir_gpu.cpp", "prover.cpp", "server.cpp"]
+SOURCES = ["lde.hip", "merkle.hip", "sumcheck.hip", "prove.hip", "whir_gpu.cpp", "prover.cpp", "server.cpp", "batch.cpp"]
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -39,7 +39,7 @@ def _compile(src: str) -> str:
 
 def build(verbose: bool = False) -> str:
     os.makedirs(OUT, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(7, len(SOURCES))) as ex:
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]

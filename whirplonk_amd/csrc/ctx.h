@@ -147,6 +147,16 @@ struct whir_ctx {
 
 namespace whir_gpu {
 
+// Autotuned LDE splits (log2 M2) per shape; 0 = not tuned.
+uint32_t tuned_split(const LdeShape& s);
+void set_tuned_split(const LdeShape& s, uint32_t lg_m2);
+// Flat records of 9 words (8 key words, then log2 M2); returns the word count.
+size_t tuned_entries(uint64_t* out, size_t cap);
+
+// One LDE on ctx's stream (tuned split unless s.lg_m2 is set).
+int lde_device(whir_ctx* ctx, const LdeShape& s, const uint64_t* in, uint64_t* out, uint8_t* leaves = nullptr,
+               bool* fused = nullptr);
+
 // RS encode + leaves + Merkle tree of num_polys vectors already on the device;
 // the root is copied to out_root (host) when non-null.
 int commit_device(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_polys, size_t poly_size, size_t cw,

@@ -30,11 +30,16 @@ struct LdeShape {
     // component c % 3 of Ext3 column c / 3 (C counts base columns, in_el = 3,
     // out_el = 1), so a row-major output row is the AoS Ext3 row.
     bool split3 = false;
+    // log2 of the second-pass DFT size M2 (m = M1 * M2); 0 = the default
+    // split M2 = min(m, 1024).  Any valid split gives the same output.
+    uint32_t lg_m2 = 0;
 };
 
 // Scratch needed by lde_launch (elements of u64); 0 when m <= 1024 (one
 // pass reads the input directly).
 size_t lde_scratch_elems(const LdeShape& s);
+// The valid second-pass sizes for a shape (log2), default first.
+int lde_splits(const LdeShape& s, uint32_t* lg_m2, int cap);
 // Returns 0, or >0 when the shape is not supported (not a power of two, m > 2^20).
 // With leaves != null and a row-major 16-column codeword the BLAKE3 row
 // leaves are hashed inside the last pass; *fused tells whether that happened.

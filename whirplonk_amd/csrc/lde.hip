@@ -296,6 +296,11 @@ constexpr uint64_t SCRATCH_TARGET = 1ull << 22;  // u64 per launch group for sma
 bool make_plan(const LdeShape& s, Plan& p) {
     if (s.C == 0 || !is_pow2(s.m) || !is_pow2(s.n) || s.n < s.m || s.n > (1ull << 32)) return false;
     p.M2 = std::min<uint64_t>(s.m, 1024);
+    if (s.lg_m2) {
+        uint64_t m2 = 1ull << s.lg_m2;
+        if (m2 > s.m || m2 > 1024) return false;
+        p.M2 = m2;
+    }
     p.M1 = s.m / p.M2;
     if (p.M1 > 1024) return false;
     if (s.m * s.C * (s.split3 ? 1 : s.in_el) >= (1ull << 31)) return false;  // per-launch offsets are 32-bit
@@ -319,6 +324,22 @@ bool make_plan(const LdeShape& s, Plan& p) {
 }
 
 }  // namespace
+
+int lde_splits(const LdeShape& s, uint32_t* lg_m2, int cap) {
+    int k = 0;
+    LdeShape t = s;
+    t.lg_m2 = 0;
+    Plan p;
+    if (!make_plan(t, p)) return 0;
+    const uint32_t def = (uint32_t)ilog2u(p.M2);
+    if (k < cap) lg_m2[k++] = def;
+    for (uint32_t lg = def; lg-- > 0 && k < cap;) {
+        t.lg_m2 = lg;
+        if (lg == 0 || !make_plan(t, p)) break;
+        lg_m2[k++] = lg;
+    }
+    return k;
+}
 
 size_t lde_scratch_elems(const LdeShape& s) {
     Plan p;

@@ -46,6 +46,44 @@ size_t g_threshold = 65536;
 
 }  // namespace
 
+namespace whir_gpu {
+
+// Autotuned LDE splits, process-wide: shape -> log2(M2).
+namespace {
+std::mutex g_tune_mu;
+std::map<std::vector<uint64_t>, uint32_t> g_tune;
+}  // namespace
+
+std::vector<uint64_t> tune_key(const LdeShape& s) {
+    return {s.m, s.n, s.C, s.in_el, s.out_el, (uint64_t)s.split3, (uint64_t)s.row_major, (uint64_t)s.inverse};
+}
+
+uint32_t tuned_split(const LdeShape& s) {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune.find(tune_key(s));
+    return it == g_tune.end() ? 0 : it->second;
+}
+
+void set_tuned_split(const LdeShape& s, uint32_t lg_m2) {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune[tune_key(s)] = lg_m2;
+}
+
+size_t tuned_entries(uint64_t* out, size_t cap) {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    size_t k = 0;
+    for (const auto& kv : g_tune) {
+        if (out && k + 9 <= cap) {
+            for (size_t i = 0; i < 8; ++i) out[k + i] = kv.first[i];
+            out[k + 8] = kv.second;
+        }
+        k += 9;
+    }
+    return k;
+}
+
+}  // namespace whir_gpu
+
 namespace {
 
 std::mutex g_default_mu;
@@ -85,6 +123,7 @@ int rs_shape(size_t num_polys, size_t poly_size, size_t cw, size_t depth, LdeSha
 
 int run_lde(whir_ctx* ctx, LdeShape s, const uint64_t* in, uint64_t* out, uint8_t* leaves = nullptr,
             bool* fused = nullptr) {
+    if (!s.lg_m2) s.lg_m2 = tuned_split(s);
     Twiddles tw;
     int rc = ctx->twiddles(log2u(s.n), s.inverse, tw);
     if (rc) return rc;
@@ -135,6 +174,11 @@ int with_default(F f) {
 }
 
 }  // namespace
+
+int whir_gpu::lde_device(whir_ctx* ctx, const LdeShape& s, const uint64_t* in, uint64_t* out, uint8_t* leaves,
+                         bool* fused) {
+    return run_lde(ctx, s, in, out, leaves, fused);
+}
 
 int whir_gpu::commit_device(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_polys, size_t poly_size, size_t cw,
                       size_t depth, uint64_t* d_matrix, uint8_t* d_nodes, uint8_t* out_root, int ext) {
