@@ -19,15 +19,29 @@ HBM: algorithmic bytes 8*(2^24 + 2^26) per commit against 8 TB/s) from HIP
 events on the launch stream, and the reference's own CPU path (oracle/_ref,
 OpenMP) timed on the host over the same workload.
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu]
+Beside the headline line's workload it measures, in the same JSON line:
+  * full_prove: the whole WHIR prove (commit, every round, PoW, openings) at
+    num_variables 22, rate 1/2, folding factor 4, 3 MLE constraints, against
+    the reference's own prover on the host (bit-exact proof bytes);
+  * batched: 64 polynomials of 2^14..2^22 coefficients (rate 1/4) committed by
+    one graph launch, against one-at-a-time commits and the CPU roots;
+  * served: commit+prove requests (sizes 2^16..2^24 with weights
+    .3/.3/.2/.15/.05, rate 1/4, 3 MLE constraints) through the serving loop:
+    capacity, Poisson arrivals at a fixed offered rate with p50/p99 latency,
+    and the plain one-at-a-time GPU path on the same requests.
+
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--legs commit,prove,batched,served]
 """
 from __future__ import annotations
 
 import argparse
+import hashlib
 import json
 import os
+import random
 import statistics
 import sys
+import threading
 import time
 
 import numpy as np
@@ -137,12 +151,257 @@ def cpu_leg(coeffs, b, r, reps: int = 3):
             "root": root.tobytes(), "c0": c0.copy(), "c2": c2.copy(), "reps": reps}
 
 
+# ------------------------------------------------------------ full prove
+
+PROVE_NV, PROVE_SEED = 22, 0x5748495200000004
+
+
+def prove_inputs(nv, seed, nforms=3):
+    import whirplonk_amd as W
+
+    coeffs, st = W.lcg_uniform(seed, 1 << nv)
+    pts, _ = W.lcg_uniform(st, 3 * nv * nforms)
+    return coeffs, pts.reshape(nforms, nv, 3)
+
+
+def prove_leg(steps: int, warmup: int, cpu: bool):
+    """Full WHIR prove at 2^22 (rate 1/2, folding factor 4, 3 MLE constraints),
+    the CLI's parameters and domain separator; inputs resident in HBM."""
+    import whirplonk_amd as W
+
+    coeffs, pts = prove_inputs(PROVE_NV, PROVE_SEED)
+    params = W.Params.default(starting_log_inv_rate=1)
+    pid, sid = W.domain_separator_cli(params)
+    ctx = W.Context(0)
+    d = ctx.upload(coeffs)
+    ev = np.array([ctx.mle_evaluate(d, True, pts[f]) for f in range(len(pts))], dtype=np.uint64)
+    ws_bytes = W.workspace_bytes(1 << PROVE_NV, params)
+    ws = ctx.alloc(ws_bytes)
+
+    def one():
+        with W.Prover(ctx, params, 1 << PROVE_NV, pid, sid, workspace=(ws, ws_bytes)) as pr:
+            pr.commit(d, 1)
+            pr.prove(pts, ev)
+            return pr.proof(), pr.phase_times(), pr.pow_hashes()
+
+    for _ in range(warmup):
+        one()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        (narg, hints), phases, pow_h = one()
+    ctx.sync()
+    ms = 1000.0 * (time.perf_counter() - t0) / steps
+    ctx.close()
+    out = {"workload": "full WHIR prove nv=22 rate 1/2 folding_factor 4, 3 MLE constraints, CLI parameters",
+           "ms_per_proof": round(ms, 3), "proofs_per_s": round(1000.0 / ms, 2),
+           "phase_ms": {k: round(v, 3) for k, v in phases.items()}, "pow_hashes": int(pow_h),
+           "narg_bytes": len(narg), "hints_bytes": len(hints),
+           "proof_sha256": hashlib.sha256(narg + hints).hexdigest()[:16],
+           "cpu_reference_ms": None, "bit_exact_vs_cpu": None}
+    if cpu:
+        import oracle
+
+        if oracle.ref() is not None:
+            t0 = time.perf_counter()
+            _, n2, h2 = oracle.ref_prove(PROVE_NV, coeffs, pts, rate=1)
+            out["cpu_reference_ms"] = round(1000.0 * (time.perf_counter() - t0), 1)
+            out["cpu_threads"] = int(oracle.ref().ref_num_threads())
+            out["bit_exact_vs_cpu"] = (n2 == narg and h2 == hints)
+    return out
+
+
+# ------------------------------------------------------------ batched commit
+
+BATCH_SEED = 0x5748495200000003
+
+
+def batched_leg(steps: int, cpu: bool):
+    """64 polynomials, num_variables uniform in [14, 22] from the seed, rate
+    1/4, folding factor 4: one graph launch per step vs one-at-a-time commits."""
+    import whirplonk_amd as W
+
+    rnd = random.Random(BATCH_SEED)
+    sizes = [1 << rnd.randint(14, 22) for _ in range(64)]
+    polys, st = [], BATCH_SEED
+    for sz in sizes:
+        c, st = W.lcg_uniform(st, sz)
+        polys.append(c)
+    committed = sum((sz // 16) * 4 * 16 for sz in sizes)
+    ctx = W.Context(0)
+    t0 = time.perf_counter()
+    b = W.Batch(ctx, sizes, log_inv_rate=2, folding_factor=4, branches=4, autotune=True)
+    create_s = time.perf_counter() - t0
+    for k, c in enumerate(polys):
+        b.upload(k, c)
+    b.run(graph=True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.run(graph=True)
+    ctx.sync()
+    ms = 1000.0 * (time.perf_counter() - t0) / steps
+    roots = b.roots()
+    # the plain path: each polynomial committed alone, one after another
+    d = [ctx.upload(c) for c in polys]
+    bufs = {}
+    for sz in set(sizes):
+        cw = (sz // 16) * 4
+        bufs[sz] = (ctx.alloc(8 * cw * 16), ctx.alloc(32 * W.merkle_num_nodes(cw)))
+    plain_roots = [ctx.commit(d[k], 1, sz, (sz // 16) * 4, 16, *bufs[sz]) for k, sz in enumerate(sizes)]
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for k, sz in enumerate(sizes):
+            ctx.commit(d[k], 1, sz, (sz // 16) * 4, 16, *bufs[sz])
+    ctx.sync()
+    plain_ms = 1000.0 * (time.perf_counter() - t0) / steps
+    b.close()
+    ctx.close()
+    out = {"workload": "batched commit: 64 polynomials nv uniform in [14,22] (seeded), rate 1/4, folding_factor 4, "
+                       "one HIP graph launch (4 branch streams, autotuned radix split)",
+           "polynomials": len(sizes), "total_coefficients": sum(sizes), "committed_elems": committed,
+           "ms_per_batch": round(ms, 3), "elem_per_s": round(committed / ms * 1000.0, 1),
+           "plain_ms_per_batch": round(plain_ms, 3), "speedup_vs_plain": round(plain_ms / ms, 3),
+           "plan_and_tune_s": round(create_s, 3), "roots_equal_plain": roots == plain_roots,
+           "bit_exact_vs_cpu": None}
+    if cpu:
+        import oracle
+
+        R = oracle.ref()
+        if R is not None:
+            ok = True
+            for k, sz in enumerate(sizes):
+                root = np.zeros(32, dtype=np.uint8)
+                R.ref_commit_base(oracle._ptr64(polys[k]), sz, (sz // 16) * 4, 16, None, oracle._ptr8(root))
+                ok &= root.tobytes() == roots[k]
+            out["bit_exact_vs_cpu"] = bool(ok)
+    return out
+
+
+# ------------------------------------------------------------ served
+
+SERVE_SIZES, SERVE_WEIGHTS, SERVE_SEED = (16, 18, 20, 22, 24), (0.3, 0.3, 0.2, 0.15, 0.05), 0x5748495200000005
+
+
+def served_leg(n_req: int, offered: float, slots: int, budget_gb: float, cpu: bool):
+    """Commit+prove requests through the serving loop (rate 1/4, folding
+    factor 4, 3 MLE constraints each, evaluations computed on the device).
+    Inputs come from 4 seeded input sets per size in pinned host memory."""
+    import whirplonk_amd as W
+
+    rnd = random.Random(SERVE_SEED)
+    sizes = rnd.choices(SERVE_SIZES, weights=SERVE_WEIGHTS, k=n_req)
+    pools = {}
+    for nv in SERVE_SIZES:
+        pools[nv] = []
+        for k in range(4):
+            c, pts = prove_inputs(nv, SERVE_SEED + 16 * nv + k)
+            pin = W.Pinned(c.size)
+            pin.array[:] = c
+            pools[nv].append((pin, pts))
+    reqs = [(nv, pools[nv][i % 4]) for i, nv in enumerate(sizes)]
+    params = W.Params.default(starting_log_inv_rate=2)
+    pid, sid = W.domain_separator_cli(params)
+
+    # plain one-at-a-time GPU path: upload, evaluate the constraints, prove
+    ctx = W.Context(0)
+    dbuf = {nv: ctx.alloc(8 << nv) for nv in SERVE_SIZES}
+    ws = {nv: W.workspace_bytes(1 << nv, params) for nv in SERVE_SIZES}
+    wbuf = {nv: ctx.alloc(ws[nv]) for nv in SERVE_SIZES}
+
+    def plain(nv, pin, pts):
+        W._check(W.load().whir_memcpy_h2d(ctx.h, dbuf[nv], pin.ptr, 8 << nv))
+        ev = np.array([ctx.mle_evaluate(dbuf[nv], True, pts[f]) for f in range(len(pts))], dtype=np.uint64)
+        with W.Prover(ctx, params, 1 << nv, pid, sid, workspace=(wbuf[nv], ws[nv])) as pr:
+            pr.commit(dbuf[nv], 1)
+            pr.prove(pts, ev)
+            return pr.proof()
+
+    for nv in SERVE_SIZES:
+        plain(nv, *pools[nv][0])
+    t0 = time.perf_counter()
+    ref = [plain(nv, pin, pts) for nv, (pin, pts) in reqs]
+    plain_s = time.perf_counter() - t0
+    ctx.close()
+
+    budget = int(budget_gb * (1 << 30))
+    out = {"workload": f"served commit+prove: {n_req} requests, nv in {list(SERVE_SIZES)} weights "
+                       f"{list(SERVE_WEIGHTS)}, rate 1/4, folding_factor 4, 3 MLE constraints",
+           "requests": n_req, "slots": slots, "hbm_budget_bytes": budget,
+           "plain_proofs_per_s": round(n_req / plain_s, 2)}
+    with W.Server(params, num_slots=slots, hbm_budget=budget) as srv:
+        for nv in SERVE_SIZES:  # warm every slot's context
+            ids = [srv.submit(1 << nv, pools[nv][0][0].array, pools[nv][0][1], borrow=True) for _ in range(slots)]
+            [srv.wait(i) for i in ids]
+        # capacity: everything submitted at once
+        t0 = time.perf_counter()
+        ids = [srv.submit(1 << nv, pin.array, pts, borrow=True) for nv, (pin, pts) in reqs]
+        res = [srv.wait(i) for i in ids]
+        cap_s = time.perf_counter() - t0
+        same_cap = all(res[i][:2] == ref[i] for i in range(n_req))
+        # Poisson arrivals at the fixed offered rate
+        arr = np.cumsum(np.random.default_rng(SERVE_SEED).exponential(1.0 / offered, n_req))
+        ids = [None] * n_req
+
+        def submitter():
+            t_start = time.perf_counter()
+            for i, (nv, (pin, pts)) in enumerate(reqs):
+                dt = t_start + arr[i] - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                ids[i] = srv.submit(1 << nv, pin.array, pts, borrow=True)
+
+        th = threading.Thread(target=submitter)
+        t0 = time.perf_counter()
+        th.start()
+        lat, same_p = [], True
+        for i in range(n_req):
+            while ids[i] is None:
+                time.sleep(0.0002)
+            narg, hints, ms = srv.wait(ids[i])
+            lat.append(ms[2])
+            same_p &= (narg, hints) == ref[i]
+        th.join()
+        run_s = time.perf_counter() - t0
+        st = srv.stats()
+    lat.sort()
+    out.update({
+        "capacity_proofs_per_s": round(n_req / cap_s, 2),
+        "speedup_vs_plain": round(plain_s / cap_s, 3),
+        "offered_rate": offered,
+        "proofs_per_s": round(n_req / run_s, 2),
+        "p50_ms": round(lat[len(lat) // 2], 3), "p99_ms": round(lat[min(len(lat) - 1, int(0.99 * len(lat)))], 3),
+        "peak_hbm_bytes": st["peak"],
+        "bit_exact_vs_plain": bool(same_cap and same_p),
+        "bit_exact_vs_cpu": None,
+    })
+    if cpu:
+        import oracle
+
+        if oracle.ref() is not None:
+            ok = True
+            for nv in SERVE_SIZES:  # input set 0 of every size against the reference prover
+                pin, pts = pools[nv][0]
+                i = next(j for j, (v, (pp, _)) in enumerate(reqs) if v == nv and pp is pin)
+                _, n2, h2 = oracle.ref_prove(nv, np.array(pin.array), pts, rate=2)
+                ok &= (n2, h2) == ref[i]
+            out["bit_exact_vs_cpu"] = bool(ok)
+            out["cpu_sample"] = "input set 0 of each size (5 proofs) vs the reference prover; every request vs plain"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--legs", default="commit,prove,batched,served")
+    ap.add_argument("--requests", type=int, default=2000, help="served requests")
+    ap.add_argument("--offered", type=float, default=250.0, help="served offered rate (requests/s, Poisson)")
+    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--budget-gb", type=float, default=64.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 512 MiB device copy kernel (reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -202,8 +461,12 @@ def main():
         "cpu_baseline": cpu,
         "bit_exact_vs_cpu": root_ok,
         "root": root.hex(),
-        "served": None,
     }
+    legs = set(args.legs.split(","))
+    out["full_prove"] = prove_leg(args.steps, args.warmup, not args.no_cpu) if "prove" in legs else None
+    out["batched"] = batched_leg(args.steps, not args.no_cpu) if "batched" in legs else None
+    out["served"] = (served_leg(args.requests, args.offered, args.slots, args.budget_gb, not args.no_cpu)
+                     if "served" in legs else None)
     print(json.dumps(out))
 
 

@@ -381,12 +381,18 @@ int whir_server_create(int device, const whir_params* params, uint32_t num_slots
 int whir_server_destroy(whir_server* server);
 /* Device bytes a request of `size` coefficients holds while it runs. */
 size_t whir_server_request_bytes(whir_server* server, uint64_t size);
-/* Queue a request: batch_size vectors of `size` base-field coefficients (host,
- * copied), num_forms MLE points (host, num_forms x log2(size) Ext3) and their
- * claimed values (num_forms x batch_size Ext3), or evaluations = NULL to have
- * them computed on the device.  Returns its id. */
-int whir_server_submit(whir_server* server, uint64_t size, const uint64_t* coeffs, size_t num_forms,
+/* Queue a request: batch_size vectors of `size` base-field coefficients
+ * (host; copied at submit, or with borrow != 0 read in place, so they must
+ * stay valid until the request is waited for), num_forms MLE points (host,
+ * num_forms x log2(size) Ext3, copied) and their claimed values (num_forms x
+ * batch_size Ext3), or evaluations = NULL to have them computed on the
+ * device.  Returns its id. */
+int whir_server_submit(whir_server* server, uint64_t size, const uint64_t* coeffs, int borrow, size_t num_forms,
                        const uint64_t* points, const uint64_t* evaluations, uint64_t* out_id);
+/* Pinned host memory (hipHostMalloc) for request inputs that are copied to
+ * the device by DMA. */
+int whir_host_malloc(size_t bytes, void** out);
+int whir_host_free(void* ptr);
 /* Block until request `id` is done.  Lengths are always written; with
  * buffers large enough the proof is copied and the request released (with
  * null buffers it is kept for a later call).  out_ms (optional, 3 doubles):

@@ -52,7 +52,7 @@ def test_mix_equals_alone(slots):
         ids = []
         for i, (nv, c, p) in enumerate(inputs):
             ev = alone[i][1] if i % 2 else None  # half computed on the device
-            ids.append(srv.submit(1 << nv, c, p, ev))
+            ids.append(srv.submit(1 << nv, c, p, ev, borrow=(i % 3 == 0)))
         for i, rid in enumerate(ids):
             narg, hints, ms = srv.wait(rid)
             assert (narg, hints) == alone[i][0], i

@@ -132,7 +132,9 @@ _SIGS = [
     ("whir_server_create", _int, [_int, _vp, ctypes.c_uint32, _sz, ctypes.POINTER(_vp)]),
     ("whir_server_destroy", _int, [_vp]),
     ("whir_server_request_bytes", _sz, [_vp, ctypes.c_uint64]),
-    ("whir_server_submit", _int, [_vp, ctypes.c_uint64, _u64p, _sz, _u64p, _u64p, _u64p]),
+    ("whir_server_submit", _int, [_vp, ctypes.c_uint64, _u64p, _int, _sz, _u64p, _u64p, _u64p]),
+    ("whir_host_malloc", _int, [_sz, ctypes.POINTER(_vp)]),
+    ("whir_host_free", _int, [_vp]),
     ("whir_server_wait", _int, [_vp, ctypes.c_uint64, _u8p, _sz, ctypes.POINTER(_sz), _u8p, _sz, ctypes.POINTER(_sz),
                                 ctypes.POINTER(ctypes.c_double)]),
     ("whir_server_stats", _int, [_vp, _u64p]),
@@ -453,6 +455,10 @@ class Params(ctypes.Structure):
         return p
 
 
+def workspace_bytes(size: int, params: Params) -> int:
+    return load().whir_prover_workspace_bytes(ctypes.byref(params), size)
+
+
 def config_flat(size: int, params: Params) -> np.ndarray:
     """whir::Config::from_params flattened (layout in include/whir_gpu.h)."""
     n = load().whir_config_flat(size, ctypes.byref(params), None, 0)
@@ -538,15 +544,21 @@ class Prover:
     """One proof: whir_prover_create / commit / prove / proof."""
 
     def __init__(self, ctx: Context, params: Params, size: int, protocol_id: bytes, session_id: bytes,
-                 instance: bytes = b""):
+                 instance: bytes = b"", workspace: tuple[int, int] | None = None):
+        """workspace=(device pointer, bytes) carves the proof's memory from a
+        caller-owned region (whir_prover_create_in) instead of allocating."""
         self.ctx = ctx
         self.size = size
         pid = np.frombuffer(protocol_id, dtype=np.uint8).copy()
         sid = np.frombuffer(session_id, dtype=np.uint8).copy()
         ins = np.frombuffer(instance, dtype=np.uint8).copy() if instance else np.zeros(1, dtype=np.uint8)
         h = _vp()
-        _check(load().whir_prover_create(ctx.h, ctypes.byref(params), size, _p8(pid), _p8(sid), _p8(ins),
-                                         len(instance), ctypes.byref(h)))
+        if workspace is None:
+            _check(load().whir_prover_create(ctx.h, ctypes.byref(params), size, _p8(pid), _p8(sid), _p8(ins),
+                                             len(instance), ctypes.byref(h)))
+        else:
+            _check(load().whir_prover_create_in(ctx.h, ctypes.byref(params), size, _p8(pid), _p8(sid), _p8(ins),
+                                                len(instance), workspace[0], workspace[1], ctypes.byref(h)))
         self.h = h
 
     def close(self):
@@ -657,6 +669,23 @@ class Batch:
         return load().whir_batch_device_bytes(self.h)
 
 
+class Pinned:
+    """Pinned host memory (whir_host_malloc) viewed as a numpy array; freed
+    when this object is collected, so keep it alive while the array is used."""
+
+    def __init__(self, count: int, dtype=np.uint64):
+        nbytes = max(int(count) * np.dtype(dtype).itemsize, 1)
+        p = _vp()
+        _check(load().whir_host_malloc(nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+        self.array = np.frombuffer((ctypes.c_uint8 * nbytes).from_address(self.ptr), dtype=dtype, count=count)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            load().whir_host_free(self.ptr)
+            self.ptr = None
+
+
 class Server:
     """The serving loop (whir_server_*): submit requests, wait for proofs."""
 
@@ -679,13 +708,14 @@ class Server:
     def request_bytes(self, size: int) -> int:
         return load().whir_server_request_bytes(self.h, size)
 
-    def submit(self, size: int, coeffs, points=None, evaluations=None) -> int:
+    def submit(self, size: int, coeffs, points=None, evaluations=None, borrow: bool = False) -> int:
+        """borrow=True reads coeffs in place (keep the array alive until wait)."""
         c = _u64(coeffs)
         pts = _u64(points).reshape(-1) if points is not None and np.size(points) else None
         nf = pts.size // (3 * (size.bit_length() - 1)) if pts is not None else 0
         ev = _u64(evaluations).reshape(-1) if evaluations is not None else None
         out = np.zeros(1, dtype=np.uint64)
-        _check(load().whir_server_submit(self.h, size, _p64(c), nf, _p64(pts) if nf else None,
+        _check(load().whir_server_submit(self.h, size, _p64(c), int(borrow), nf, _p64(pts) if nf else None,
                                          _p64(ev) if ev is not None else None, _p64(out)))
         return int(out[0])
 

@@ -14,6 +14,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -85,6 +86,7 @@ struct Request {
     uint64_t size = 0;
     size_t num_forms = 0;
     std::vector<uint64_t> coeffs, points, evals;
+    const uint64_t* borrowed = nullptr;  // caller-owned coefficients (no copy)
     bool have_evals = false;
     size_t footprint = 0, ws_bytes = 0, coeff_bytes = 0, arena_off = 0;
     size_t skipped = 0;
@@ -154,18 +156,24 @@ void whir_server::run(Slot& s, Request& r) {
     };
     uint8_t* blk = arena.at(r.arena_off);
     uint64_t* d_coeffs = (uint64_t*)(blk + r.ws_bytes);
-    // stage the coefficients through pinned memory
-    const size_t cbytes = r.coeffs.size() * 8;
-    if (s.pinned_bytes < cbytes) {
-        if (s.pinned) (void)hipHostFree(s.pinned);
-        s.pinned = nullptr;
-        s.pinned_bytes = 0;
-        hipError_t e = hipHostMalloc((void**)&s.pinned, cbytes, hipHostMallocDefault);
-        if (e != hipSuccess) return fail_with(fail("hipHostMalloc", e));
-        s.pinned_bytes = cbytes;
+    // coefficients: a borrowed buffer is copied by DMA straight from the
+    // caller's memory (pinned memory is fastest); an owned copy goes through
+    // this slot's pinned staging buffer
+    const size_t cbytes = (size_t)params.batch_size * r.size * 8;
+    const void* src = r.borrowed;
+    if (!src) {
+        if (s.pinned_bytes < cbytes) {
+            if (s.pinned) (void)hipHostFree(s.pinned);
+            s.pinned = nullptr;
+            s.pinned_bytes = 0;
+            hipError_t e = hipHostMalloc((void**)&s.pinned, cbytes, hipHostMallocDefault);
+            if (e != hipSuccess) return fail_with(fail("hipHostMalloc", e));
+            s.pinned_bytes = cbytes;
+        }
+        std::memcpy(s.pinned, r.coeffs.data(), cbytes);
+        src = s.pinned;
     }
-    std::memcpy(s.pinned, r.coeffs.data(), cbytes);
-    hipError_t e = hipMemcpyAsync(d_coeffs, s.pinned, cbytes, hipMemcpyHostToDevice, s.ctx->stream);
+    hipError_t e = hipMemcpyAsync(d_coeffs, src, cbytes, hipMemcpyHostToDevice, s.ctx->stream);
     if (e != hipSuccess) return fail_with(fail("hipMemcpyAsync", e));
     const uint32_t nv = log2u(r.size);
     const size_t nvec = params.batch_size;
@@ -226,6 +234,17 @@ void whir_server::worker(Slot* s) {
 
 extern "C" {
 
+int whir_host_malloc(size_t bytes, void** out) {
+    if (!out) return bad_arg("null output");
+    HIPCHK(hipHostMalloc(out, std::max<size_t>(bytes, 1), hipHostMallocDefault));
+    return 0;
+}
+
+int whir_host_free(void* p) {
+    if (p) HIPCHK(hipHostFree(p));
+    return 0;
+}
+
 int whir_server_create(int device, const whir_params* params, uint32_t num_slots, size_t hbm_budget_bytes,
                        whir_server** out) {
     if (!params || !out || num_slots == 0 || hbm_budget_bytes == 0) return bad_arg("bad server arguments");
@@ -277,8 +296,8 @@ size_t whir_server_request_bytes(whir_server* S, uint64_t size) {
     return S ? S->footprint(size, &ws, &cb) : 0;
 }
 
-int whir_server_submit(whir_server* S, uint64_t size, const uint64_t* coeffs, size_t num_forms, const uint64_t* points,
-                       const uint64_t* evaluations, uint64_t* out_id) {
+int whir_server_submit(whir_server* S, uint64_t size, const uint64_t* coeffs, int borrow, size_t num_forms,
+                       const uint64_t* points, const uint64_t* evaluations, uint64_t* out_id) {
     if (!S || !coeffs || !out_id || (num_forms && !points)) return bad_arg("null argument");
     auto r = std::make_unique<Request>();
     r->size = size;
@@ -287,7 +306,10 @@ int whir_server_submit(whir_server* S, uint64_t size, const uint64_t* coeffs, si
     if (r->footprint > S->budget) return bad_arg("request larger than the HBM budget");
     const size_t nvec = S->params.batch_size;
     const uint32_t nv = log2u(size);
-    r->coeffs.assign(coeffs, coeffs + nvec * size);
+    if (borrow)
+        r->borrowed = coeffs;
+    else
+        r->coeffs.assign(coeffs, coeffs + nvec * size);
     r->num_forms = num_forms;
     if (num_forms) r->points.assign(points, points + 3 * nv * num_forms);
     if (evaluations) {
