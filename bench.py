@@ -391,25 +391,8 @@ def served_leg(n_req: int, offered: float, slots: int, budget_gb: float, cpu: bo
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--legs", default="commit,prove,batched,served")
-    ap.add_argument("--requests", type=int, default=2000, help="served requests")
-    ap.add_argument("--offered", type=float, default=250.0, help="served offered rate (requests/s, Poisson)")
-    ap.add_argument("--slots", type=int, default=8)
-    ap.add_argument("--budget-gb", type=float, default=64.0)
-    ap.add_argument("--calibrate-copy", action="store_true",
-                    help="also run one 512 MiB device copy kernel (reference for HBM counter scaling)")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build runs the commit-and-fold path on a single MI355X; "
-              "multi-GPU sharding is design-only (see the sharding section of the design notes).", file=sys.stderr)
-        sys.exit(2)
-
+def commit_leg(args):
+    """The headline workload: plain GPU commit at 2^24 + first sumcheck round."""
     poly, depth, cw = shapes()
     n_cw = cw * depth
     coeffs, b, r = make_inputs()
@@ -462,7 +445,34 @@ def main():
         "bit_exact_vs_cpu": root_ok,
         "root": root.hex(),
     }
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--legs", default="commit,prove,batched,served")
+    ap.add_argument("--requests", type=int, default=2000, help="served requests")
+    ap.add_argument("--offered", type=float, default=250.0, help="served offered rate (requests/s, Poisson)")
+    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--budget-gb", type=float, default=64.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one 512 MiB device copy kernel (reference for HBM counter scaling)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build runs the commit-and-fold path on a single MI355X; "
+              "multi-GPU sharding is design-only (see the sharding section of the design notes).", file=sys.stderr)
+        sys.exit(2)
+
     legs = set(args.legs.split(","))
+    out = commit_leg(args) if "commit" in legs else {
+        "metric": METRIC, "value": None, "unit": "elem/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64", "data": "synthetic", "config": {"workload": "commit leg skipped"},
+        "roofline": None, "cpu_baseline": None}
     out["full_prove"] = prove_leg(args.steps, args.warmup, not args.no_cpu) if "prove" in legs else None
     out["batched"] = batched_leg(args.steps, not args.no_cpu) if "batched" in legs else None
     out["served"] = (served_leg(args.requests, args.offered, args.slots, args.budget_gb, not args.no_cpu)
