@@ -255,8 +255,7 @@ int whir_batch_create(whir_ctx* ctx, size_t count, const uint64_t* sizes, uint32
         size_t need = 0;
         for (size_t k : B->branch_polys[b]) {
             LdeShape s = commit_shape(sizes[k], depth, log_inv_rate);
-            s.lg_m2 = tuned_split(s);
-            need = std::max(need, lde_scratch_elems(s) * 8);
+            need = std::max(need, lde_scratch_elems_any_split(s) * 8);
         }
         B->scratch_bytes.push_back(a256(need));
         B->scratch_off.push_back(put(need));

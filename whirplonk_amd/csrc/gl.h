@@ -178,16 +178,89 @@ __host__ __device__ __forceinline__ e3 e3_add(e3 a, e3 b) { return {add(a.c0, b.
 __host__ __device__ __forceinline__ e3 e3_sub(e3 a, e3 b) { return {sub(a.c0, b.c0), sub(a.c1, b.c1), sub(a.c2, b.c2)}; }
 
 // X^3 = 2: c0 = a0b0 + 2(a1b2 + a2b1), c1 = a0b1 + a1b0 + 2 a2b2, c2 = a0b2 + a1b1 + a2b0
-__host__ __device__ __forceinline__ e3 e3_mul(e3 a, e3 b) {
-    uint64_t t = add(mul(a.c1, b.c2), mul(a.c2, b.c1));
-    uint64_t c0 = add(mul(a.c0, b.c0), dbl(t));
-    uint64_t c1 = add(add(mul(a.c0, b.c1), mul(a.c1, b.c0)), dbl(mul(a.c2, b.c2)));
-    uint64_t c2 = add(add(mul(a.c0, b.c2), mul(a.c1, b.c1)), mul(a.c2, b.c0));
-    return {c0, c1, c2};
+__host__ __device__ __forceinline__ e3 e3_mul(e3 a, e3 b);
+
+// ---- exact accumulation of 128-bit products, reduced once ----
+// value = top * 2^128 + hi * 2^64 + lo; 2^128 = -2^32 (mod p), so the value
+// is reduce128(lo, hi) - top * 2^32 (top stays far below 2^31 here).
+struct acc128 {
+    uint64_t lo, hi, top;
+};
+
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __forceinline__ void acc_add(acc128& a, uint64_t plo, uint64_t phi) {
+    uint32_t c1, c2, c3, c4;
+    uint32_t l0 = __builtin_addc(lo32(a.lo), lo32(plo), 0u, &c1);
+    uint32_t l1 = __builtin_addc(hi32(a.lo), hi32(plo), c1, &c2);
+    uint32_t h0 = __builtin_addc(lo32(a.hi), lo32(phi), c2, &c3);
+    uint32_t h1 = __builtin_addc(hi32(a.hi), hi32(phi), c3, &c4);
+    a.lo = mk64(l0, l1);
+    a.hi = mk64(h0, h1);
+    a.top += c4;
+}
+#else
+__host__ __forceinline__ void acc_add(acc128& a, uint64_t plo, uint64_t phi) {
+    unsigned __int128 v = ((unsigned __int128)a.hi << 64) | a.lo;
+    unsigned __int128 p = ((unsigned __int128)phi << 64) | plo;
+    unsigned __int128 s = v + p;
+    a.top += s < p;
+    a.lo = (uint64_t)s;
+    a.hi = (uint64_t)(s >> 64);
+}
+#endif
+
+// acc += x * y (exact)
+__host__ __device__ __forceinline__ void acc_mac(acc128& a, uint64_t x, uint64_t y) {
+    uint64_t lo, hi;
+    mul128(x, y, lo, hi);
+    acc_add(a, lo, hi);
+}
+
+__host__ __device__ __forceinline__ uint64_t acc_reduce(const acc128& a) {
+    return sub(reduce128(a.lo, a.hi), a.top << 32);
+}
+
+// Ext3 accumulator: three exact component sums.
+struct e3acc {
+    acc128 c[3];
+};
+
+__host__ __device__ __forceinline__ e3acc e3acc_zero() { return {{{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}}; }
+
+// acc += a * b over Ext3 (X^3 = 2), with no reduction
+__host__ __device__ __forceinline__ void e3_mac(e3acc& acc, e3 a, e3 b) {
+    const uint64_t a1d = dbl(a.c1), a2d = dbl(a.c2);
+    acc_mac(acc.c[0], a.c0, b.c0);
+    acc_mac(acc.c[0], a1d, b.c2);
+    acc_mac(acc.c[0], a2d, b.c1);
+    acc_mac(acc.c[1], a.c0, b.c1);
+    acc_mac(acc.c[1], a.c1, b.c0);
+    acc_mac(acc.c[1], a2d, b.c2);
+    acc_mac(acc.c[2], a.c0, b.c2);
+    acc_mac(acc.c[2], a.c1, b.c1);
+    acc_mac(acc.c[2], a.c2, b.c0);
+}
+
+// acc += a * b with b in the base field (component-wise)
+__host__ __device__ __forceinline__ void e3_mac_base(e3acc& acc, e3 a, uint64_t b) {
+    acc_mac(acc.c[0], a.c0, b);
+    acc_mac(acc.c[1], a.c1, b);
+    acc_mac(acc.c[2], a.c2, b);
+}
+
+__host__ __device__ __forceinline__ e3 e3acc_reduce(const e3acc& acc) {
+    return {acc_reduce(acc.c[0]), acc_reduce(acc.c[1]), acc_reduce(acc.c[2])};
 }
 
 // base x Ext3, component-wise (Basefield::mixed_mul, embedding.hpp:75-77)
 __host__ __device__ __forceinline__ e3 e3_mul_base(e3 a, uint64_t b) { return {mul(a.c0, b), mul(a.c1, b), mul(a.c2, b)}; }
+
+// each component is a sum of three products reduced once (e3_mac)
+__host__ __device__ __forceinline__ e3 e3_mul(e3 a, e3 b) {
+    e3acc acc = e3acc_zero();
+    e3_mac(acc, a, b);
+    return e3acc_reduce(acc);
+}
 
 __host__ __device__ __forceinline__ e3 e3_pow(e3 a, uint64_t e) {
     e3 r = {1, 0, 0};

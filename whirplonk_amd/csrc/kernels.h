@@ -38,6 +38,9 @@ struct LdeShape {
 // Scratch needed by lde_launch (elements of u64); 0 when m <= 1024 (one
 // pass reads the input directly).
 size_t lde_scratch_elems(const LdeShape& s);
+// The most scratch any valid split of the shape needs (for buffers planned
+// before the split is known).
+size_t lde_scratch_elems_any_split(const LdeShape& s);
 // The valid second-pass sizes for a shape (log2), default first.
 int lde_splits(const LdeShape& s, uint32_t* lg_m2, int cap);
 // Returns 0, or >0 when the shape is not supported (not a power of two, m > 2^20).

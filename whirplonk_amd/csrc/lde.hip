@@ -38,6 +38,12 @@ namespace whir_gpu {
 namespace {
 
 constexpr int THREADS = 512;
+#ifndef LDE_THREADS_A
+#define LDE_THREADS_A 512
+#endif
+// pass A uses fewer threads (and LDS) per workgroup than pass B, so that two
+// workgroups share a CU and one's loads overlap the other's arithmetic
+constexpr int THREADS_A = LDE_THREADS_A;
 
 struct LdeArgs {
     const uint64_t* in;
@@ -54,6 +60,9 @@ struct LdeArgs {
     // split3: column c is component c % 3 of Ext3 column c / 3 (AoS input);
     // direct: M1 == 1, pass B reads the input itself (no pass A, no scratch)
     uint32_t split3, direct;
+    // pass A: cosets per block (the input tile is reused for each) and the
+    // number of cosets in this launch
+    uint32_t cpb, ncos;
     // log2 of the power-of-two tile sizes: every index split is a shift/mask
     uint32_t lgTA, lgTB, lgCB, lgNJ, lgNK;
 };
@@ -78,50 +87,57 @@ __device__ __forceinline__ uint64_t twid(const LdeArgs& a, uint64_t e) {
 }
 
 template <int M1, bool INV>
-__global__ __launch_bounds__(THREADS) void lde_pass_a(LdeArgs a, uint64_t s0) {
+__global__ __launch_bounds__(THREADS_A) void lde_pass_a(LdeArgs a, uint64_t s0) {
     using S = dft::Split<M1>;
     constexpr int R1 = S::R1, R2 = S::R2, PAD = S::PADDED + 1;
     extern __shared__ uint64_t lds[];
     const uint32_t TA = a.TA, lgTA = a.lgTA;
     const uint32_t c = blockIdx.x >> a.lgNJ;
     const uint32_t J = (blockIdx.x & ((1u << a.lgNJ) - 1)) << lgTA;
-    const uint64_t s = s0 + blockIdx.y;
-    uint64_t* z = a.z + blockIdx.y * a.zstride + (uint64_t)c * a.M2 + J;   // Z[k][c][J + t] = z[k*C*M2 + t]
     const uint64_t* col = a.in + in_index(a, c, J);
     const uint32_t in_el = a.in_el, M2 = (uint32_t)a.M2, zrow = (uint32_t)(a.C * a.M2);
     uint64_t* twist = lds + TA * PAD;  // w_L^{s j1} = w_n^{s j1 M2}, shared by the TA columns
-
-    if (s != 0) {
-        for (uint32_t j1 = threadIdx.x; j1 < M1; j1 += blockDim.x) twist[j1] = twid(a, (s * j1 * a.M2) & (a.n - 1));
-        __syncthreads();
-    }
-    for (uint32_t q = threadIdx.x; q < TA * M1; q += blockDim.x) {
-        uint32_t t = q & (TA - 1), j1 = q >> lgTA;
-        uint64_t x = col[(t + M2 * j1) * in_el];
-        if (s != 0) x = gl::mul(x, twist[j1]);
-        lds[t * PAD + dft::pidx<M1>(j1)] = x;
-    }
-    __syncthreads();
-    for (uint32_t w = threadIdx.x; w < TA * R2; w += blockDim.x) {
-        uint32_t n2 = w % R2, t = w / R2;
-        dft::lds_dft_step1<M1, INV>(lds + t * PAD, n2, a.t1024);
-    }
-    __syncthreads();
-    for (uint32_t w = threadIdx.x; w < TA * R1; w += blockDim.x) {
-        uint32_t t = w & (TA - 1), k1 = w >> lgTA;
-        uint64_t v[R2];
-        dft::lds_dft_step2<M1, INV>(lds + t * PAD, k1, v);
-        const uint64_t j2 = J + t;
-#pragma unroll
-        for (int k2 = 0; k2 < R2; ++k2) {
-            uint32_t k = k1 + R1 * k2;
-            uint64_t zv = v[k2];
-            if (!a.tw_in_b) {
-                uint64_t e = ((s + a.E * k) * j2) & (a.n - 1);
-                if (e) zv = gl::mul(zv, twid(a, e));
-            }
-            z[k * zrow + t] = zv;
+    // The block transforms its input tile (TA columns x M1 values) for each of
+    // its a.cpb cosets; the re-reads of the tile hit the cache.
+    const uint32_t nq = TA * M1;
+    for (uint32_t g = 0; g < a.cpb; ++g) {
+        const uint32_t zi = blockIdx.y * a.cpb + g;  // coset slot of this launch
+        if (zi >= a.ncos) break;                    // uniform over the block
+        const uint64_t s = s0 + zi;
+        uint64_t* z = a.z + zi * a.zstride + (uint64_t)c * a.M2 + J;  // Z[k][c][J + t] = z[k*C*M2 + t]
+        if (s != 0) {
+            for (uint32_t j1 = threadIdx.x; j1 < M1; j1 += THREADS_A) twist[j1] = twid(a, (s * j1 * a.M2) & (a.n - 1));
+            __syncthreads();
         }
+        for (uint32_t q = threadIdx.x; q < nq; q += THREADS_A) {
+            uint32_t t = q & (TA - 1), j1 = q >> lgTA;
+            uint64_t x = col[(t + M2 * j1) * in_el];
+            if (s != 0) x = gl::mul(x, twist[j1]);
+            lds[t * PAD + dft::pidx<M1>(j1)] = x;
+        }
+        __syncthreads();
+        for (uint32_t w = threadIdx.x; w < TA * R2; w += THREADS_A) {
+            uint32_t n2 = w % R2, t = w / R2;
+            dft::lds_dft_step1<M1, INV>(lds + t * PAD, n2, a.t1024);
+        }
+        __syncthreads();
+        for (uint32_t w = threadIdx.x; w < TA * R1; w += THREADS_A) {
+            uint32_t t = w & (TA - 1), k1 = w >> lgTA;
+            uint64_t v[R2];
+            dft::lds_dft_step2<M1, INV>(lds + t * PAD, k1, v);
+            const uint64_t j2 = J + t;
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) {
+                uint32_t k = k1 + R1 * k2;
+                uint64_t zv = v[k2];
+                if (!a.tw_in_b) {
+                    uint64_t e = ((s + a.E * k) * j2) & (a.n - 1);
+                    if (e) zv = gl::mul(zv, twid(a, e));
+                }
+                z[k * zrow + t] = zv;
+            }
+        }
+        __syncthreads();  // the next coset rewrites the LDS tile and twist table
     }
 }
 
@@ -291,7 +307,13 @@ uint64_t pow2_div(uint64_t x, uint64_t cap) {
     return d;
 }
 
-constexpr uint64_t SCRATCH_TARGET = 1ull << 22;  // u64 per launch group for small problems
+#ifndef LDE_SCRATCH_LOG
+#define LDE_SCRATCH_LOG 26
+#endif
+#ifndef LDE_BLOCKS_TARGET
+#define LDE_BLOCKS_TARGET 1024
+#endif
+constexpr uint64_t SCRATCH_TARGET = 1ull << LDE_SCRATCH_LOG;  // u64 per launch group for small problems
 
 bool make_plan(const LdeShape& s, Plan& p) {
     if (s.C == 0 || !is_pow2(s.m) || !is_pow2(s.n) || s.n < s.m || s.n > (1ull << 32)) return false;
@@ -307,7 +329,7 @@ bool make_plan(const LdeShape& s, Plan& p) {
     if (s.split3 && (s.C % 3 || s.in_el != 3 || s.out_el != 1)) return false;
     p.E = s.n / s.m;
     p.L = p.E * p.M1;
-    p.TA = (uint32_t)std::min<uint64_t>(p.M2, THREADS / split_r2(p.M1));
+    p.TA = (uint32_t)std::min<uint64_t>(p.M2, std::max<uint64_t>(1, THREADS_A / split_r2(p.M1)));
     uint64_t DB = std::min<uint64_t>(THREADS / split_r2(p.M2), s.C * p.M1);
     if (s.row_major) {
         p.CB = (uint32_t)pow2_div(s.C, DB);
@@ -333,12 +355,25 @@ int lde_splits(const LdeShape& s, uint32_t* lg_m2, int cap) {
     if (!make_plan(t, p)) return 0;
     const uint32_t def = (uint32_t)ilog2u(p.M2);
     if (k < cap) lg_m2[k++] = def;
+    if (p.direct) return k;  // one pass, no scratch: the only split offered
     for (uint32_t lg = def; lg-- > 0 && k < cap;) {
         t.lg_m2 = lg;
         if (lg == 0 || !make_plan(t, p)) break;
         lg_m2[k++] = lg;
     }
     return k;
+}
+
+size_t lde_scratch_elems_any_split(const LdeShape& s) {
+    uint32_t cand[11];
+    int nc = lde_splits(s, cand, 11);
+    size_t need = 0;
+    LdeShape t = s;
+    for (int c = 0; c < nc; ++c) {
+        t.lg_m2 = cand[c];
+        need = std::max(need, lde_scratch_elems(t));
+    }
+    return need;
 }
 
 size_t lde_scratch_elems(const LdeShape& s) {
@@ -409,11 +444,17 @@ int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* s
     }
     dim3 ga((unsigned)(s.C * (p.M2 / p.TA)), 1, 1);
     dim3 gb((unsigned)((p.M1 / p.TB) * (s.C / p.CB)), 1, 1);
+    // cosets per pass-A block: enough blocks to fill the chip, the rest reuse
+    // the loaded tile
+    const uint64_t bpc = ga.x;
     for (uint64_t s0 = 0; s0 < p.E; s0 += p.G) {
         uint64_t g = std::min<uint64_t>(p.G, p.E - s0);
-        ga.y = (unsigned)g;
+        uint64_t cpb = std::max<uint64_t>(1, std::min<uint64_t>(g, g * bpc / LDE_BLOCKS_TARGET));
+        a.cpb = (uint32_t)cpb;
+        a.ncos = (uint32_t)g;
+        ga.y = (unsigned)((g + cpb - 1) / cpb);
         gb.y = (unsigned)g;
-        if (!p.direct) hipLaunchKernelGGL(ka, ga, dim3(THREADS), lds_a, st, a, s0);
+        if (!p.direct) hipLaunchKernelGGL(ka, ga, dim3(THREADS_A), lds_a, st, a, s0);
         hipLaunchKernelGGL(kb, gb, dim3(THREADS), lds_b, st, a, s0);
     }
     return 0;

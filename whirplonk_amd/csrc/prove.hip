@@ -162,11 +162,12 @@ __global__ __launch_bounds__(256) void term_accumulate_kernel(uint64_t* __restri
     if (i >= n) return;
     const uint64_t H = n >> L, LO = 1ull << L;
     const uint64_t h = i >> L, l = i & (LO - 1);
-    e3 sum = accumulate ? ld3(acc, i) : e3{0, 0, 0};
-    for (uint32_t t = 0; t < te; ++t)
-        sum = gl::e3_add(sum, gl::e3_mul(ld3(hi_e, t * H + h), ld3(lo_e, t * LO + l)));
-    for (uint32_t t = 0; t < tb; ++t)
-        sum = gl::e3_add(sum, gl::e3_mul_base(ld3(hi_b, t * H + h), lo_b[t * LO + l]));
+    // exact sum of all term products, reduced once
+    gl::e3acc s = gl::e3acc_zero();
+    for (uint32_t t = 0; t < te; ++t) gl::e3_mac(s, ld3(hi_e, t * H + h), ld3(lo_e, t * LO + l));
+    for (uint32_t t = 0; t < tb; ++t) gl::e3_mac_base(s, ld3(hi_b, t * H + h), lo_b[t * LO + l]);
+    e3 sum = gl::e3acc_reduce(s);
+    if (accumulate) sum = gl::e3_add(sum, ld3(acc, i));
     st3(acc, i, sum);
 }
 
@@ -175,13 +176,16 @@ __global__ __launch_bounds__(256) void mle_eval_kernel(const uint64_t* __restric
                                                        uint32_t L, const uint64_t* __restrict__ hi,
                                                        const uint64_t* __restrict__ lo,
                                                        uint64_t* __restrict__ partials) {
-    e3 acc[1] = {{0, 0, 0}};
+    gl::e3acc s = gl::e3acc_zero();
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x, mask = (1ull << L) - 1;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         e3 w = gl::e3_mul(ld3(hi, i >> L), ld3(lo, i & mask));
-        e3 v = f_base ? gl::e3_mul_base(w, f[i]) : gl::e3_mul(w, ld3(f, i));
-        acc[0] = gl::e3_add(acc[0], v);
+        if (f_base)
+            gl::e3_mac_base(s, w, f[i]);
+        else
+            gl::e3_mac(s, w, ld3(f, i));
     }
+    e3 acc[1] = {gl::e3acc_reduce(s)};
     block_sum256<1>(acc);
     if (threadIdx.x == 0) st3(partials, blockIdx.x, acc[0]);
 }
@@ -203,7 +207,7 @@ __global__ __launch_bounds__(256) void sum_partials_kernel(uint64_t* partials, u
 __global__ __launch_bounds__(256) void fold_compute_kernel(const uint64_t* a, int a_base, const uint64_t* b,
                                                            uint64_t q, e3 r, uint64_t* a_out, uint64_t* b_out,
                                                            uint64_t* __restrict__ partials) {
-    e3 acc[2] = {{0, 0, 0}, {0, 0, 0}};
+    gl::e3acc s0 = gl::e3acc_zero(), s2 = gl::e3acc_zero();
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < q; i += stride) {
         e3 fa0, fa1;
@@ -225,9 +229,10 @@ __global__ __launch_bounds__(256) void fold_compute_kernel(const uint64_t* a, in
         st3(a_out, i + q, fa1);
         st3(b_out, i, fb0);
         st3(b_out, i + q, fb1);
-        acc[0] = gl::e3_add(acc[0], gl::e3_mul(fa0, fb0));
-        acc[1] = gl::e3_add(acc[1], gl::e3_mul(gl::e3_sub(fa1, fa0), gl::e3_sub(fb1, fb0)));
+        gl::e3_mac(s0, fa0, fb0);
+        gl::e3_mac(s2, gl::e3_sub(fa1, fa0), gl::e3_sub(fb1, fb0));
     }
+    e3 acc[2] = {gl::e3acc_reduce(s0), gl::e3acc_reduce(s2)};
     block_sum256<2>(acc);
     if (threadIdx.x == 0) {
         st3(partials, 2 * blockIdx.x, acc[0]);

@@ -65,7 +65,7 @@ size_t lde_scratch_bytes(size_t vector_size, size_t depth, size_t num_vectors, s
         s.in_el = 3;
         s.split3 = true;
     }
-    return lde_scratch_elems(s) * 8;
+    return lde_scratch_elems_any_split(s) * 8;
 }
 
 Layout plan_layout(const whir_host::WhirConfig& cfg, size_t size) {

@@ -64,21 +64,23 @@ __device__ __forceinline__ void block_sum(e3* v) {
 __global__ __launch_bounds__(256) void sumcheck_compute_kernel(const uint64_t* __restrict__ a, int a_base,
                                                                const uint64_t* __restrict__ b, uint64_t half,
                                                                uint64_t* __restrict__ partials) {
-    e3 acc[2] = {{0, 0, 0}, {0, 0, 0}};
+    // exact sums of products per lane, reduced once
+    gl::e3acc s0 = gl::e3acc_zero(), s2 = gl::e3acc_zero();
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < half; i += stride) {
         e3 b0 = load_e3(b, i), b1 = load_e3(b, i + half);
         e3 db = gl::e3_sub(b1, b0);
         if (a_base) {
             uint64_t a0 = a[i], a1 = a[i + half];
-            acc[0] = gl::e3_add(acc[0], gl::e3_mul_base(b0, a0));
-            acc[1] = gl::e3_add(acc[1], gl::e3_mul_base(db, gl::sub(a1, a0)));
+            gl::e3_mac_base(s0, b0, a0);
+            gl::e3_mac_base(s2, db, gl::sub(a1, a0));
         } else {
             e3 a0 = load_e3(a, i), a1 = load_e3(a, i + half);
-            acc[0] = gl::e3_add(acc[0], gl::e3_mul(a0, b0));
-            acc[1] = gl::e3_add(acc[1], gl::e3_mul(gl::e3_sub(a1, a0), db));
+            gl::e3_mac(s0, a0, b0);
+            gl::e3_mac(s2, gl::e3_sub(a1, a0), db);
         }
     }
+    e3 acc[2] = {gl::e3acc_reduce(s0), gl::e3acc_reduce(s2)};
     block_sum<2>(acc);
     if (threadIdx.x == 0) {
         store_e3(partials, 2 * blockIdx.x, acc[0]);
@@ -96,7 +98,7 @@ __global__ __launch_bounds__(256) void sumcheck_compute_kernel(const uint64_t* _
 __global__ __launch_bounds__(256) void sumcheck_round_kernel(const uint64_t* a, int a_base, const uint64_t* b,
                                                              uint64_t half, e3 r, uint64_t* a_out, uint64_t* b_out,
                                                              uint64_t* __restrict__ partials) {
-    e3 acc[2] = {{0, 0, 0}, {0, 0, 0}};
+    gl::e3acc s0 = gl::e3acc_zero(), s2 = gl::e3acc_zero();
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < half; i += stride) {
         e3 b0 = load_e3(b, i), b1 = load_e3(b, i + half);
@@ -105,20 +107,21 @@ __global__ __launch_bounds__(256) void sumcheck_round_kernel(const uint64_t* a, 
         if (a_base) {
             uint64_t a0 = a[i], a1 = a[i + half];
             uint64_t da = gl::sub(a1, a0);
-            acc[0] = gl::e3_add(acc[0], gl::e3_mul_base(b0, a0));
-            acc[1] = gl::e3_add(acc[1], gl::e3_mul_base(db, da));
+            gl::e3_mac_base(s0, b0, a0);
+            gl::e3_mac_base(s2, db, da);
             fa = gl::e3_mul_base(r, da);
             fa.c0 = gl::add(fa.c0, a0);
         } else {
             e3 a0 = load_e3(a, i), a1 = load_e3(a, i + half);
             e3 da = gl::e3_sub(a1, a0);
-            acc[0] = gl::e3_add(acc[0], gl::e3_mul(a0, b0));
-            acc[1] = gl::e3_add(acc[1], gl::e3_mul(da, db));
+            gl::e3_mac(s0, a0, b0);
+            gl::e3_mac(s2, da, db);
             fa = gl::e3_add(a0, gl::e3_mul(da, r));
         }
         store_e3(a_out, i, fa);
         store_e3(b_out, i, gl::e3_add(b0, gl::e3_mul(db, r)));
     }
+    e3 acc[2] = {gl::e3acc_reduce(s0), gl::e3acc_reduce(s2)};
     block_sum<2>(acc);
     if (threadIdx.x == 0) {
         store_e3(partials, 2 * blockIdx.x, acc[0]);
