@@ -55,6 +55,7 @@ struct whir_ctx {
     whir_gpu::DevBuf scratch, partials, zpow, hin, hout, hnodes, hidx, hcomp;
     // prove loop: PoW words, term descriptors and tables, opened rows, vector, covector
     whir_gpu::DevBuf powbuf, tdesc, ttab, oidx, orows;
+    uint32_t pow_slot = 0;  // PoW result slot of the next search window
     // when set, LDE passes use this caller-owned scratch instead of `scratch`
     void* ext_scratch = nullptr;
     size_t ext_scratch_bytes = 0;
@@ -96,6 +97,9 @@ struct whir_ctx {
         b.bytes = want;
         return 0;
     }
+
+    // partial-sum buffer of the two-stage reductions
+    int ensure_partials() { return ensure(partials, 6 * whir_gpu::SUMCHECK_MAX_BLOCKS * 8); }
 
     int twiddles(uint32_t log_n, bool inverse, whir_gpu::Twiddles& out) {
         auto key = std::make_pair(log_n, inverse);

@@ -65,6 +65,7 @@ void copy_launch(void* dst, const void* src, uint64_t bytes, hipStream_t st);
 // partials must hold 2*3*SUMCHECK_MAX_BLOCKS u64; result c0[3], c2[3] left in
 // partials[0..6) on the device.
 constexpr int SUMCHECK_MAX_BLOCKS = 2048;
+
 void sumcheck_compute_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint64_t n,
                              uint64_t* partials, hipStream_t st);
 // compute + both folds in one sweep; an Ext3 input may be folded in place
@@ -82,9 +83,13 @@ void horner_launch(const uint64_t* f, bool f_ext3, uint64_t n, const uint64_t* z
 // ------------------------------------------------------------- prove loop
 // Scan nonces base + k, k < blocks*256*per_thread, for the smallest one whose
 // BLAKE3(ch || LE64(nonce) || 0^24) read as LE64 is <= threshold; *best
-// (initialised to ~0 by the caller) takes the minimum hit.
-void pow_search_launch(const uint32_t* d_ch, uint64_t threshold, uint64_t base, uint32_t blocks,
-                       uint32_t per_thread, unsigned long long* d_best, hipStream_t st);
+// (~0 on entry) takes the minimum hit, and *reset is set to ~0 for the next
+// window (results alternate between two slots).
+struct PowChallenge {
+    uint32_t w[8];
+};
+void pow_search_launch(const PowChallenge& ch, uint64_t threshold, uint64_t base, uint32_t blocks,
+                       uint32_t per_thread, unsigned long long* d_best, unsigned long long* d_reset, hipStream_t st);
 void set_u64_launch(uint64_t* p, uint64_t v, hipStream_t st);
 
 // Weight terms sum_t s_t w_t(i) over i < 2^nv, factored as hi_t[i >> L] * lo_t[i & (2^L-1)].

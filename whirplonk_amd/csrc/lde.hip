@@ -41,6 +41,9 @@ constexpr int THREADS = 512;
 #ifndef LDE_THREADS_A
 #define LDE_THREADS_A 512
 #endif
+#ifndef LDE_THREADS_B_SMALL
+#define LDE_THREADS_B_SMALL 512
+#endif
 // pass A uses fewer threads (and LDS) per workgroup than pass B, so that two
 // workgroups share a CU and one's loads overlap the other's arithmetic
 constexpr int THREADS_A = LDE_THREADS_A;
@@ -297,6 +300,7 @@ KernelFn pick_b(uint64_t M) {
 struct Plan {
     uint64_t M1, M2, E, L, G;  // G cosets per launch
     uint32_t TA, TB, CB;
+    uint32_t threads_b;  // pass-B workgroup size
     bool direct;
 };
 
@@ -330,7 +334,11 @@ bool make_plan(const LdeShape& s, Plan& p) {
     p.E = s.n / s.m;
     p.L = p.E * p.M1;
     p.TA = (uint32_t)std::min<uint64_t>(p.M2, std::max<uint64_t>(1, THREADS_A / split_r2(p.M1)));
-    uint64_t DB = std::min<uint64_t>(THREADS / split_r2(p.M2), s.C * p.M1);
+    // 16-column base codewords keep 512 threads (the fused row leaves need all
+    // 16 columns of a row in one workgroup); other shapes may use fewer
+    const bool fusable = s.C == FUSE_COLS && s.row_major && s.out_el == 1 && !s.split3;
+    p.threads_b = fusable ? THREADS : LDE_THREADS_B_SMALL;
+    uint64_t DB = std::min<uint64_t>(std::max<uint64_t>(1, p.threads_b / split_r2(p.M2)), s.C * p.M1);
     if (s.row_major) {
         p.CB = (uint32_t)pow2_div(s.C, DB);
         p.TB = (uint32_t)std::min<uint64_t>(DB / p.CB, p.M1);
@@ -455,7 +463,7 @@ int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* s
         ga.y = (unsigned)((g + cpb - 1) / cpb);
         gb.y = (unsigned)g;
         if (!p.direct) hipLaunchKernelGGL(ka, ga, dim3(THREADS_A), lds_a, st, a, s0);
-        hipLaunchKernelGGL(kb, gb, dim3(THREADS), lds_b, st, a, s0);
+        hipLaunchKernelGGL(kb, gb, dim3(p.threads_b), lds_b, st, a, s0);
     }
     return 0;
 }

@@ -23,6 +23,7 @@
 #include "blake3_dev.h"
 #include "gl.h"
 #include "kernels.h"
+#include "reduce_dev.h"
 
 #include <algorithm>
 
@@ -38,45 +39,19 @@ __device__ __forceinline__ void st3(uint64_t* p, uint64_t i, e3 v) {
     p[3 * i + 2] = v.c2;
 }
 
-__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
-    uint32_t lo = __shfl_xor((uint32_t)v, m, 64);
-    uint32_t hi = __shfl_xor((uint32_t)(v >> 32), m, 64);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-// Sum of NV Ext3 values over a 256-thread workgroup; thread 0 gets the totals.
-template <int NV>
-__device__ __forceinline__ void block_sum256(e3* v) {
-    __shared__ e3 sh[NV][4];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-#pragma unroll
-        for (int m = 32; m >= 1; m >>= 1) {
-            v[k].c0 = gl::add(v[k].c0, shfl_xor64(v[k].c0, m));
-            v[k].c1 = gl::add(v[k].c1, shfl_xor64(v[k].c1, m));
-            v[k].c2 = gl::add(v[k].c2, shfl_xor64(v[k].c2, m));
-        }
-    }
-    if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) sh[k][wave] = v[k];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k)
-            v[k] = gl::e3_add(gl::e3_add(sh[k][0], sh[k][1]), gl::e3_add(sh[k][2], sh[k][3]));
-    }
-}
+using red::block_sum;
+using red::store_block_totals;
 
 // ------------------------------------------------------------------ PoW
-__global__ __launch_bounds__(256) void pow_search_kernel(const uint32_t* __restrict__ ch, uint64_t threshold,
-                                                         uint64_t base, uint32_t per_thread,
-                                                         unsigned long long* best) {
+__global__ __launch_bounds__(256) void pow_search_kernel(PowChallenge ch, uint64_t threshold, uint64_t base,
+                                                         uint32_t per_thread, unsigned long long* best,
+                                                         unsigned long long* reset) {
+    // the other result slot is cleared here for the next window (the host has
+    // already read it)
+    if (blockIdx.x == 0 && threadIdx.x == 0) *reset = ~0ull;
     uint32_t m[16];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) m[i] = ch[i];
+    for (int i = 0; i < 8; ++i) m[i] = ch.w[i];
 #pragma unroll
     for (int i = 10; i < 16; ++i) m[i] = 0;
     uint32_t cv[8];
@@ -174,8 +149,7 @@ __global__ __launch_bounds__(256) void term_accumulate_kernel(uint64_t* __restri
 // partial sums of f_i * hi[i >> L] * lo[i & mask] (one eq term)
 __global__ __launch_bounds__(256) void mle_eval_kernel(const uint64_t* __restrict__ f, int f_base, uint64_t n,
                                                        uint32_t L, const uint64_t* __restrict__ hi,
-                                                       const uint64_t* __restrict__ lo,
-                                                       uint64_t* __restrict__ partials) {
+                                                       const uint64_t* __restrict__ lo, uint64_t* partials) {
     gl::e3acc s = gl::e3acc_zero();
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x, mask = (1ull << L) - 1;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -186,17 +160,8 @@ __global__ __launch_bounds__(256) void mle_eval_kernel(const uint64_t* __restric
             gl::e3_mac(s, w, ld3(f, i));
     }
     e3 acc[1] = {gl::e3acc_reduce(s)};
-    block_sum256<1>(acc);
-    if (threadIdx.x == 0) st3(partials, blockIdx.x, acc[0]);
-}
-
-__global__ __launch_bounds__(256) void sum_partials_kernel(uint64_t* partials, uint32_t count, uint32_t nv) {
-    e3 acc[2] = {{0, 0, 0}, {0, 0, 0}};
-    for (uint32_t k = threadIdx.x; k < count; k += blockDim.x)
-        for (uint32_t v = 0; v < nv; ++v) acc[v] = gl::e3_add(acc[v], ld3(partials, (uint64_t)nv * k + v));
-    block_sum256<2>(acc);
-    if (threadIdx.x == 0)
-        for (uint32_t v = 0; v < nv; ++v) st3(partials, v, acc[v]);
+    block_sum<1>(acc);
+    store_block_totals<1>(acc, partials);
 }
 
 // ------------------------------------------------------------ fold+compute
@@ -206,7 +171,7 @@ __global__ __launch_bounds__(256) void sum_partials_kernel(uint64_t* partials, u
 // outputs may overwrite Ext3 inputs in place.
 __global__ __launch_bounds__(256) void fold_compute_kernel(const uint64_t* a, int a_base, const uint64_t* b,
                                                            uint64_t q, e3 r, uint64_t* a_out, uint64_t* b_out,
-                                                           uint64_t* __restrict__ partials) {
+                                                           uint64_t* partials) {
     gl::e3acc s0 = gl::e3acc_zero(), s2 = gl::e3acc_zero();
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < q; i += stride) {
@@ -233,11 +198,8 @@ __global__ __launch_bounds__(256) void fold_compute_kernel(const uint64_t* a, in
         gl::e3_mac(s2, gl::e3_sub(fa1, fa0), gl::e3_sub(fb1, fb0));
     }
     e3 acc[2] = {gl::e3acc_reduce(s0), gl::e3acc_reduce(s2)};
-    block_sum256<2>(acc);
-    if (threadIdx.x == 0) {
-        st3(partials, 2 * blockIdx.x, acc[0]);
-        st3(partials, 2 * blockIdx.x + 1, acc[1]);
-    }
+    block_sum<2>(acc);
+    store_block_totals<2>(acc, partials);
 }
 
 // ------------------------------------------------------------ row gather
@@ -270,9 +232,10 @@ unsigned blocks_of(uint64_t work) { return (unsigned)((work + 255) / 256); }
 
 }  // namespace
 
-void pow_search_launch(const uint32_t* d_ch, uint64_t threshold, uint64_t base, uint32_t blocks, uint32_t per_thread,
-                       unsigned long long* d_best, hipStream_t st) {
-    hipLaunchKernelGGL(pow_search_kernel, dim3(blocks), dim3(256), 0, st, d_ch, threshold, base, per_thread, d_best);
+void pow_search_launch(const PowChallenge& ch, uint64_t threshold, uint64_t base, uint32_t blocks,
+                       uint32_t per_thread, unsigned long long* d_best, unsigned long long* d_reset, hipStream_t st) {
+    hipLaunchKernelGGL(pow_search_kernel, dim3(blocks), dim3(256), 0, st, ch, threshold, base, per_thread, d_best,
+                       d_reset);
 }
 
 void set_u64_launch(uint64_t* p, uint64_t v, hipStream_t st) {
@@ -298,7 +261,7 @@ void mle_eval_launch(const uint64_t* f, bool f_base, uint64_t n, uint32_t L, con
                      uint64_t* partials, hipStream_t st) {
     unsigned g = std::min<unsigned>(blocks_of(n), SUMCHECK_MAX_BLOCKS);
     hipLaunchKernelGGL(mle_eval_kernel, dim3(g), dim3(256), 0, st, f, f_base ? 1 : 0, n, L, hi, lo, partials);
-    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, partials, g, 1u);
+    hipLaunchKernelGGL(red::sum_blocks_kernel<1>, dim3(1), dim3(256), 0, st, partials, g);
 }
 
 void fold_compute_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint64_t n, const uint64_t r[3],
@@ -307,7 +270,7 @@ void fold_compute_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint
     e3 rr = {r[0], r[1], r[2]};
     hipLaunchKernelGGL(fold_compute_kernel, dim3(g), dim3(256), 0, st, a, a_base ? 1 : 0, b, n / 4, rr, a_out, b_out,
                        partials);
-    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, partials, g, 2u);
+    hipLaunchKernelGGL(red::sum_blocks_kernel<2>, dim3(1), dim3(256), 0, st, partials, g);
 }
 
 void gather_rows_launch(const uint64_t* matrix, uint64_t row_u64, const uint64_t* d_idx, uint64_t count, uint64_t* out,

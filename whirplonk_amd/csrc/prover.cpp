@@ -165,9 +165,15 @@ int sync(whir_prover* P) {
 // challenge, find the smallest nonce, send it as 8 LE bytes.
 int pow_device(whir_ctx* ctx, const uint8_t ch[32], uint64_t threshold, uint64_t* nonce, size_t* scanned) {
     int rc;
-    if ((rc = ctx->ensure(ctx->powbuf, 64))) return rc;
-    uint64_t* d = (uint64_t*)ctx->powbuf.p;  // [0..4): challenge words, [4]: best
-    HIPCHK(hipMemcpyAsync(d, ch, 32, hipMemcpyHostToDevice, ctx->stream));
+    if (!ctx->powbuf.p) {
+        if ((rc = ctx->ensure(ctx->powbuf, 64))) return rc;
+        set_u64_launch((uint64_t*)ctx->powbuf.p, ~0ull, ctx->stream);
+        set_u64_launch((uint64_t*)ctx->powbuf.p + 1, ~0ull, ctx->stream);
+        ctx->pow_slot = 0;
+    }
+    unsigned long long* slots = (unsigned long long*)ctx->powbuf.p;  // two result slots
+    PowChallenge pc;
+    std::memcpy(pc.w, ch, 32);
     // Window of candidates per launch: about 8x the expected work 2^d
     // (d = 64 - log2(threshold)), between 2^14 and 2^24, at most 2^18 lanes.
     uint32_t lg_expect = 0;
@@ -177,12 +183,12 @@ int pow_device(whir_ctx* ctx, const uint8_t ch[32], uint64_t threshold, uint64_t
     const uint32_t blocks = 1u << (lg_lanes - 8), per_thread = 1u << (lg_window - lg_lanes);
     const uint64_t window = 1ull << lg_window;
     for (uint64_t base = 0;; base += window) {
-        set_u64_launch(d + 4, ~0ull, ctx->stream);
-        pow_search_launch((const uint32_t*)d, threshold, base, blocks, per_thread, (unsigned long long*)(d + 4),
-                          ctx->stream);
+        const uint32_t k = ctx->pow_slot;
+        ctx->pow_slot ^= 1;
+        pow_search_launch(pc, threshold, base, blocks, per_thread, slots + k, slots + (k ^ 1), ctx->stream);
         HIPCHK(hipGetLastError());
         uint64_t best = ~0ull;
-        HIPCHK(hipMemcpyAsync(&best, d + 4, 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(&best, slots + k, 8, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
         if (best != ~0ull) {
             if (scanned) *scanned += best + 1;  // the serial scan's work
@@ -291,7 +297,7 @@ int sumcheck_prove(whir_prover* P, const uint64_t* a, bool a_base, uint64_t* a_o
     if (rounds == 0) return 0;
     Phase ph(P, PH_SUMCHECK);
     int rc;
-    if ((rc = ctx->ensure(ctx->partials, 6 * SUMCHECK_MAX_BLOCKS * 8))) return rc;
+    if ((rc = ctx->ensure_partials())) return rc;
     uint64_t* part = (uint64_t*)ctx->partials.p;
     e3 c0, c2;
     sumcheck_compute_launch(a, a_base, b, n, part, ctx->stream);
@@ -911,7 +917,7 @@ int whir_dev_mle_evaluate(whir_ctx* ctx, const uint64_t* d_f, int f_is_base, siz
     const uint64_t H = 1ull << (nv - L), LO = 1ull << L;
     if ((rc = ctx->ensure(ctx->tdesc, (T.desc_e.size() + T.coords.size() + 4) * 8))) return rc;
     if ((rc = ctx->ensure(ctx->ttab, (H + LO) * 24 + 256))) return rc;
-    if ((rc = ctx->ensure(ctx->partials, 6 * SUMCHECK_MAX_BLOCKS * 8))) return rc;
+    if ((rc = ctx->ensure_partials())) return rc;
     std::vector<uint64_t> all(T.desc_e);
     all.insert(all.end(), T.coords.begin(), T.coords.end());
     uint64_t* dd = (uint64_t*)ctx->tdesc.p;

@@ -10,6 +10,7 @@
 // one-workgroup pass over the per-workgroup partials.
 #include "gl.h"
 #include "kernels.h"
+#include "reduce_dev.h"
 
 namespace whir_gpu {
 namespace {
@@ -23,47 +24,12 @@ __device__ __forceinline__ void store_e3(uint64_t* p, uint64_t i, e3 v) {
     p[3 * i + 2] = v.c2;
 }
 
-__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
-    uint32_t lo = __shfl_xor((uint32_t)v, m, 64);
-    uint32_t hi = __shfl_xor((uint32_t)(v >> 32), m, 64);
-    return ((uint64_t)hi << 32) | lo;
-}
-
-__device__ __forceinline__ e3 wave_sum(e3 v) {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        v.c0 = gl::add(v.c0, shfl_xor64(v.c0, m));
-        v.c1 = gl::add(v.c1, shfl_xor64(v.c1, m));
-        v.c2 = gl::add(v.c2, shfl_xor64(v.c2, m));
-    }
-    return v;
-}
-
-// Sum of NV Ext3 values per thread over the workgroup; thread 0 gets the totals.
-template <int NV>
-__device__ __forceinline__ void block_sum(e3* v) {
-    __shared__ e3 sh[NV][16];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-#pragma unroll
-    for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
-    if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) sh[k][wave] = v[k];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) {
-            e3 s = {0, 0, 0};
-            for (int w = 0; w < nw; ++w) s = gl::e3_add(s, sh[k][w]);
-            v[k] = s;
-        }
-    }
-}
+using red::block_sum;
+using red::store_block_totals;
 
 __global__ __launch_bounds__(256) void sumcheck_compute_kernel(const uint64_t* __restrict__ a, int a_base,
                                                                const uint64_t* __restrict__ b, uint64_t half,
-                                                               uint64_t* __restrict__ partials) {
+                                                               uint64_t* partials) {
     // exact sums of products per lane, reduced once
     gl::e3acc s0 = gl::e3acc_zero(), s2 = gl::e3acc_zero();
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -82,10 +48,7 @@ __global__ __launch_bounds__(256) void sumcheck_compute_kernel(const uint64_t* _
     }
     e3 acc[2] = {gl::e3acc_reduce(s0), gl::e3acc_reduce(s2)};
     block_sum<2>(acc);
-    if (threadIdx.x == 0) {
-        store_e3(partials, 2 * blockIdx.x, acc[0]);
-        store_e3(partials, 2 * blockIdx.x + 1, acc[1]);
-    }
+    store_block_totals<2>(acc, partials);
 }
 
 // One sumcheck round in a single sweep: (c0, c2) partials of (a, b) and both
@@ -97,7 +60,7 @@ __global__ __launch_bounds__(256) void sumcheck_compute_kernel(const uint64_t* _
 // vectors may overwrite the lower halves of Ext3 inputs (no __restrict__).
 __global__ __launch_bounds__(256) void sumcheck_round_kernel(const uint64_t* a, int a_base, const uint64_t* b,
                                                              uint64_t half, e3 r, uint64_t* a_out, uint64_t* b_out,
-                                                             uint64_t* __restrict__ partials) {
+                                                             uint64_t* partials) {
     gl::e3acc s0 = gl::e3acc_zero(), s2 = gl::e3acc_zero();
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < half; i += stride) {
@@ -123,24 +86,7 @@ __global__ __launch_bounds__(256) void sumcheck_round_kernel(const uint64_t* a, 
     }
     e3 acc[2] = {gl::e3acc_reduce(s0), gl::e3acc_reduce(s2)};
     block_sum<2>(acc);
-    if (threadIdx.x == 0) {
-        store_e3(partials, 2 * blockIdx.x, acc[0]);
-        store_e3(partials, 2 * blockIdx.x + 1, acc[1]);
-    }
-}
-
-// partials[2*blk], partials[2*blk+1] for blk < nblocks -> partials[0], partials[1]
-__global__ __launch_bounds__(256) void reduce_pairs_kernel(uint64_t* partials, uint32_t nblocks) {
-    e3 acc[2] = {{0, 0, 0}, {0, 0, 0}};
-    for (uint32_t k = threadIdx.x; k < nblocks; k += blockDim.x) {
-        acc[0] = gl::e3_add(acc[0], load_e3(partials, 2 * k));
-        acc[1] = gl::e3_add(acc[1], load_e3(partials, 2 * k + 1));
-    }
-    block_sum<2>(acc);
-    if (threadIdx.x == 0) {
-        store_e3(partials, 0, acc[0]);
-        store_e3(partials, 1, acc[1]);
-    }
+    store_block_totals<2>(acc, partials);
 }
 
 __global__ __launch_bounds__(256) void sumcheck_fold_kernel(const uint64_t* __restrict__ in, int in_base,
@@ -206,7 +152,7 @@ void sumcheck_compute_launch(const uint64_t* a, bool a_base, const uint64_t* b, 
                              hipStream_t st) {
     unsigned g = grid_for(n / 2, SUMCHECK_MAX_BLOCKS);
     hipLaunchKernelGGL(sumcheck_compute_kernel, dim3(g), dim3(256), 0, st, a, a_base ? 1 : 0, b, n / 2, partials);
-    hipLaunchKernelGGL(reduce_pairs_kernel, dim3(1), dim3(256), 0, st, partials, g);
+    hipLaunchKernelGGL(red::sum_blocks_kernel<2>, dim3(1), dim3(256), 0, st, partials, g);
 }
 
 void sumcheck_round_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint64_t n, const uint64_t r[3],
@@ -215,7 +161,7 @@ void sumcheck_round_launch(const uint64_t* a, bool a_base, const uint64_t* b, ui
     e3 rr = {r[0], r[1], r[2]};
     hipLaunchKernelGGL(sumcheck_round_kernel, dim3(g), dim3(256), 0, st, a, a_base ? 1 : 0, b, n / 2, rr, a_out, b_out,
                        partials);
-    hipLaunchKernelGGL(reduce_pairs_kernel, dim3(1), dim3(256), 0, st, partials, g);
+    hipLaunchKernelGGL(red::sum_blocks_kernel<2>, dim3(1), dim3(256), 0, st, partials, g);
 }
 
 void sumcheck_fold_launch(const uint64_t* in, bool in_base, uint64_t n, const uint64_t r[3], uint64_t* out,

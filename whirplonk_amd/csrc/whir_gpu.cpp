@@ -394,7 +394,7 @@ int whir_dev_sumcheck_compute(whir_ctx* ctx, const uint64_t* d_a, int a_is_base,
                               uint64_t* out_c0, uint64_t* out_c2) {
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     if (n % 2) return bad_arg("sumcheck length must be even");
-    int rc = ctx->ensure(ctx->partials, 6 * SUMCHECK_MAX_BLOCKS * 8);
+    int rc = ctx->ensure_partials();
     if (rc) return rc;
     uint64_t* part = (uint64_t*)ctx->partials.p;
     sumcheck_compute_launch(d_a, a_is_base != 0, d_b, n, part, ctx->stream);
@@ -425,7 +425,7 @@ int whir_dev_horner(whir_ctx* ctx, const uint64_t* d_f, int f_is_ext3, size_t n,
     }
     int rc = ctx->ensure(ctx->zpow, 192 * 8 + 3 * 8);
     if (rc) return rc;
-    rc = ctx->ensure(ctx->partials, 6 * SUMCHECK_MAX_BLOCKS * 8);
+    rc = ctx->ensure_partials();
     if (rc) return rc;
     uint64_t h[192];
     gl::e3 p = {z[0], z[1], z[2]};
@@ -450,7 +450,7 @@ int whir_dev_sumcheck_round(whir_ctx* ctx, const uint64_t* d_a, int a_is_base, c
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     if (n % 2) return bad_arg("sumcheck length must be even");
     if (a_is_base && (const void*)d_a == (const void*)d_a_out) return bad_arg("base-field fold cannot run in place");
-    int rc = ctx->ensure(ctx->partials, 6 * SUMCHECK_MAX_BLOCKS * 8);
+    int rc = ctx->ensure_partials();
     if (rc) return rc;
     uint64_t* part = (uint64_t*)ctx->partials.p;
     if ((rc = ctx->mark(0))) return rc;
