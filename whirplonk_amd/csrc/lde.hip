@@ -144,6 +144,83 @@ __global__ __launch_bounds__(THREADS_A) void lde_pass_a(LdeArgs a, uint64_t s0) 
     }
 }
 
+#ifndef LDE_PA_PERSIST
+#define LDE_PA_PERSIST 1
+#endif
+
+// Pass A for M1 >= 256 when the tile fills the workgroup (TA * R2 threads).
+// Persistent: workgroup b walks the tiles b, b + gridDim.x, ... (tile = column
+// c, first j2 J, as blockIdx.x of lde_pass_a) and runs every coset of the
+// launch on each.  Lane (t, n2) = (tid % TA, tid / TA) holds its step-1
+// inputs x[t][R2*n1 + n2] in registers for all cosets (the tile is read from
+// HBM once, with no LDS staging pass), and issues the next tile's loads once
+// the last coset's step 1 is done, so they are in flight during its step 2.
+template <int M1, bool INV>
+__global__ __launch_bounds__(THREADS_A) void lde_pass_a_regs(LdeArgs a, uint64_t s0, uint32_t ntiles) {
+    using S = dft::Split<M1>;
+    constexpr int R1 = S::R1, R2 = S::R2, PAD = S::PADDED + 1, TA = THREADS_A / R2;
+    static_assert(R1 <= R2 && TA * R2 == THREADS_A, "step-2 items must fit the workgroup");
+    extern __shared__ uint64_t lds[];
+    uint64_t* twist = lds + TA * PAD;  // w_L^{s j1}, shared by the TA columns
+    const uint32_t M2 = (uint32_t)a.M2, in_el = a.in_el, zrow = (uint32_t)(a.C * a.M2);
+    const uint32_t jmask = (1u << a.lgNJ) - 1;
+    uint64_t pre[R1];
+    auto fetch = [&](uint32_t tile, uint32_t tid) {
+        const uint64_t* col = a.in + in_index(a, tile >> a.lgNJ, (tile & jmask) * TA);  // uniform
+        const uint32_t t = tid % TA, n2 = tid / TA;
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) pre[n1] = col[(t + M2 * (R2 * n1 + n2)) * in_el];
+    };
+    uint32_t tile = blockIdx.x;
+    if (tile < ntiles) fetch(tile, threadIdx.x);
+    for (; tile < ntiles; tile += gridDim.x) {
+        const uint32_t c = tile >> a.lgNJ, J = (tile & jmask) * TA;
+        const bool more = tile + gridDim.x < ntiles;
+        for (uint32_t g = 0; g < a.ncos; ++g) {
+            // lane indices from an opaque copy of the thread id, every coset
+            // (see lde_pass_b_rows)
+            uint32_t tid = threadIdx.x;
+            asm volatile("" : "+v"(tid));
+            const uint32_t t = tid % TA, n2 = tid / TA;
+            const uint64_t s = s0 + g;
+            uint64_t* z = a.z + g * a.zstride + (uint64_t)c * M2 + J;  // Z[k][c][J + t] = z[k*C*M2 + t]
+            if (s != 0) {
+                for (uint32_t j1 = tid; j1 < M1; j1 += THREADS_A) twist[j1] = twid(a, (s * j1 * M2) & (a.n - 1));
+                __syncthreads();
+            }
+            uint64_t v[R1];
+#pragma unroll
+            for (int n1 = 0; n1 < R1; ++n1) v[n1] = s != 0 ? gl::mul(pre[n1], twist[R2 * n1 + n2]) : pre[n1];
+            dft::dft_reg<R1, INV>(v);
+            if constexpr (R2 > 1) {
+#pragma unroll
+                for (int k1 = 1; k1 < R1; ++k1) v[k1] = gl::mul(v[k1], a.t1024[((n2 * k1) * (1024 / M1)) & 1023]);
+            }
+#pragma unroll
+            for (int k1 = 0; k1 < R1; ++k1) lds[t * PAD + dft::pidx<M1>(R2 * k1 + n2)] = v[k1];
+            if (g + 1 == a.ncos && more) fetch(tile + gridDim.x, tid);
+            __syncthreads();
+            for (uint32_t w = tid; w < TA * R1; w += THREADS_A) {
+                const uint32_t t2 = w % TA, k1 = w / TA;
+                uint64_t u[R2];
+                dft::lds_dft_step2<M1, INV>(lds + t2 * PAD, k1, u);
+                const uint64_t j2 = J + t2;
+#pragma unroll
+                for (int k2 = 0; k2 < R2; ++k2) {
+                    uint32_t k = k1 + R1 * k2;
+                    uint64_t zv = u[k2];
+                    if (!a.tw_in_b) {
+                        uint64_t e = ((s + a.E * k) * j2) & (a.n - 1);
+                        if (e) zv = gl::mul(zv, twid(a, e));
+                    }
+                    z[k * zrow + t2] = zv;
+                }
+            }
+            __syncthreads();  // the next coset rewrites the LDS tile and twist table
+        }
+    }
+}
+
 template <int M2, bool INV>
 __global__ __launch_bounds__(THREADS) void lde_pass_b(LdeArgs a, uint64_t s0) {
     using S = dft::Split<M2>;
@@ -247,6 +324,107 @@ __global__ __launch_bounds__(THREADS) void lde_pass_b(LdeArgs a, uint64_t s0) {
     }
 }
 
+#ifndef LDE_PB_PERSIST
+#define LDE_PB_PERSIST 1
+#endif
+
+// Pass B of a 16-column base codeword with M2 = 1024 and the row leaves fused
+// (every commit of a polynomial with m >= 2048 coefficients per column).
+// Persistent: workgroup b walks the tiles b, b + gridDim.x, ... (tile = coset
+// slot g * M1 + i1 index k) and, while it transforms and hashes one tile,
+// holds the loads of its next tile in flight in registers.  Thread
+// (c, n2) = (tid / 32, tid % 32) loads the step-1 inputs of column c straight
+// into registers, so the input needs no LDS staging pass; LDS carries the
+// step-1 -> step-2 transposition and the row-major staging of the leaves.
+template <bool INV>
+__global__ __launch_bounds__(THREADS) void lde_pass_b_rows(LdeArgs a, uint64_t s0, uint32_t ntiles) {
+    constexpr int M2 = 1024, R = 32, PAD = M2 + M2 / R + 1;
+    static_assert(THREADS == 16 * R, "one thread per (column, n2)");
+    extern __shared__ uint64_t lds[];
+    uint64_t* twt = lds + 16 * PAD;  // past both the DFT buffer and the row stage
+    const uint32_t kmask = (1u << a.lgNK) - 1;
+    uint64_t pre[R];
+    uint32_t off = (threadIdx.x / R) * M2 + threadIdx.x % R;
+    auto fetch = [&](uint32_t t) {
+        // uniform tile base + 32-bit lane offset: scalar-base addressing
+        const uint64_t* zb = a.z + (uint64_t)(t >> a.lgNK) * a.zstride + (uint64_t)(t & kmask) * (16 * M2);
+#pragma unroll
+        for (int n1 = 0; n1 < R; ++n1) pre[n1] = zb[off + R * n1];
+    };
+    uint32_t tile = blockIdx.x;
+    if (tile < ntiles) fetch(tile);
+    for (; tile < ntiles; tile += gridDim.x) {
+        // The lane indices are re-derived every tile from an opaque copy of
+        // the thread id: hoisted out of the loop, the per-lane twiddle and
+        // stage addresses they feed would stay live and spill.
+        uint32_t tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const uint32_t c = tid / R, n2 = tid % R;
+        const uint64_t i1 = s0 + (tile >> a.lgNK) + a.E * (tile & kmask);
+        for (uint32_t j2 = tid; j2 < M2; j2 += THREADS) twt[j2] = twid(a, (i1 * j2) & (a.n - 1));
+        __syncthreads();
+        uint64_t v[R];
+#pragma unroll
+        for (int n1 = 0; n1 < R; ++n1) v[n1] = gl::mul(pre[n1], twt[n2 + R * n1]);
+        dft::dft_reg<R, INV>(v);
+#pragma unroll
+        for (int k1 = 1; k1 < R; ++k1) v[k1] = gl::mul(v[k1], a.t1024[(n2 * k1) & 1023]);
+#pragma unroll
+        for (int k1 = 0; k1 < R; ++k1) lds[c * PAD + dft::pidx<M2>(R * k1 + n2)] = v[k1];
+        // the next tile's loads stay in flight through step 2 and the hashing
+        if (tile + gridDim.x < ntiles) fetch(tile + gridDim.x);
+        __syncthreads();
+        const uint32_t cb = tid % 16, k1 = tid / 16;
+        dft::lds_dft_step2<M2, INV>(lds + cb * PAD, k1, v);
+        __syncthreads();
+#pragma unroll
+        for (int k2 = 0; k2 < R; ++k2) lds[stage_idx(k1 + R * k2, cb)] = v[k2];
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < M2 / THREADS; ++h) {
+            const uint32_t r = tid + h * THREADS;
+            uint32_t cv[8], m[16];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) cv[i] = b3::iv(i);
+#pragma unroll
+            for (int blk = 0; blk < 2; ++blk) {
+#pragma unroll
+                for (int cc = 0; cc < 8; ++cc) {
+                    uint64_t x = lds[stage_idx(r, 8 * blk + cc)];
+                    m[2 * cc] = (uint32_t)x;
+                    m[2 * cc + 1] = (uint32_t)(x >> 32);
+                }
+                b3::compress(cv, m, 0, 64, blk == 0 ? b3::CHUNK_START : (b3::CHUNK_END | b3::ROOT), cv);
+            }
+            uint4* dst = reinterpret_cast<uint4*>(a.leaves + 32 * (i1 + a.L * r));
+            dst[0] = make_uint4(cv[0], cv[1], cv[2], cv[3]);
+            dst[1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
+        }
+        // rows out, two columns (16 bytes) per store: 8 lanes per 128-byte row
+#pragma unroll
+        for (int h = 0; h < 16 * M2 / (2 * THREADS); ++h) {
+            const uint32_t q = tid + h * THREADS;
+            const uint32_t r = q / 8, cp = 2 * (q % 8);
+            const uint64_t x0 = lds[stage_idx(r, cp)], x1 = lds[stage_idx(r, cp + 1)];
+            *reinterpret_cast<ulonglong2*>(a.out + (i1 + a.L * r) * 16 + cp) = make_ulonglong2(x0, x1);
+        }
+        __syncthreads();  // the next tile rewrites the twiddles and the stage
+    }
+}
+
+// compute units of the current device (cached per device)
+int cu_count() {
+    static int cache[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!cache[dev]) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        cache[dev] = n;
+    }
+    return cache[dev];
+}
+
 int ilog2u(uint64_t x) {
     int k = 0;
     while ((1ull << k) < x) ++k;
@@ -260,6 +438,7 @@ uint32_t split_r2(uint64_t M) {
 uint64_t padded(uint64_t M) { return M + M / split_r2(M) + 1; }
 
 using KernelFn = void (*)(LdeArgs, uint64_t);
+using RowsFn = void (*)(LdeArgs, uint64_t, uint32_t);
 
 template <bool INV>
 KernelFn pick_a(uint64_t M) {
@@ -450,7 +629,36 @@ int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* s
             attr_done[inv][1][ib] = true;
         }
     }
+    // fused 16-column rows with M2 = 1024: the persistent prefetching pass B
+    const bool rows = LDE_PB_PERSIST && fuse && p.M2 == 1024 && !p.direct;
+    RowsFn kr = s.inverse ? lde_pass_b_rows<true> : lde_pass_b_rows<false>;
+    if (rows) {
+        std::lock_guard<std::mutex> g(attr_mu);
+        static bool rows_attr[2] = {};
+        if (!rows_attr[inv]) {
+            (void)hipFuncSetAttribute((const void*)kr, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            rows_attr[inv] = true;
+        }
+    }
+    // M1 >= 256 with a full tile: the persistent register-tile pass A
+    RowsFn kar = nullptr;
+    if (LDE_PA_PERSIST && !p.direct && p.TA * split_r2(p.M1) == THREADS_A) {
+        switch (p.M1) {
+            case 256: kar = s.inverse ? lde_pass_a_regs<256, true> : lde_pass_a_regs<256, false>; break;
+            case 512: kar = s.inverse ? lde_pass_a_regs<512, true> : lde_pass_a_regs<512, false>; break;
+            case 1024: kar = s.inverse ? lde_pass_a_regs<1024, true> : lde_pass_a_regs<1024, false>; break;
+        }
+    }
+    if (kar) {
+        std::lock_guard<std::mutex> g(attr_mu);
+        static bool regs_attr[2][11] = {};
+        if (!regs_attr[inv][ia]) {
+            (void)hipFuncSetAttribute((const void*)kar, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            regs_attr[inv][ia] = true;
+        }
+    }
     dim3 ga((unsigned)(s.C * (p.M2 / p.TA)), 1, 1);
+    if (kar && ga.x < (unsigned)cu_count()) kar = nullptr;  // too few tiles: split cosets over blocks
     dim3 gb((unsigned)((p.M1 / p.TB) * (s.C / p.CB)), 1, 1);
     // cosets per pass-A block: enough blocks to fill the chip, the rest reuse
     // the loaded tile
@@ -462,8 +670,22 @@ int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* s
         a.ncos = (uint32_t)g;
         ga.y = (unsigned)((g + cpb - 1) / cpb);
         gb.y = (unsigned)g;
-        if (!p.direct) hipLaunchKernelGGL(ka, ga, dim3(THREADS_A), lds_a, st, a, s0);
-        hipLaunchKernelGGL(kb, gb, dim3(p.threads_b), lds_b, st, a, s0);
+        if (kar) {
+            a.cpb = (uint32_t)g;
+            const uint32_t per_cu = (uint32_t)std::max<size_t>(1, (160 * 1024) / lds_a);
+            const uint32_t grid = std::min<uint32_t>(ga.x, per_cu * (uint32_t)cu_count());
+            hipLaunchKernelGGL(kar, dim3(grid), dim3(THREADS_A), lds_a, st, a, s0, ga.x);
+        } else if (!p.direct) {
+            hipLaunchKernelGGL(ka, ga, dim3(THREADS_A), lds_a, st, a, s0);
+        }
+        if (rows) {
+            // one resident workgroup per CU (the LDS use allows no more)
+            const uint32_t ntiles = (uint32_t)(g * p.M1);
+            const uint32_t grid = std::min<uint32_t>(ntiles, (uint32_t)cu_count());
+            hipLaunchKernelGGL(kr, dim3(grid), dim3(THREADS), lds_b, st, a, s0, ntiles);
+        } else {
+            hipLaunchKernelGGL(kb, gb, dim3(p.threads_b), lds_b, st, a, s0);
+        }
     }
     return 0;
 }
