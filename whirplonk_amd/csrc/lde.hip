@@ -32,6 +32,7 @@
 #include "dft.h"
 #include "kernels.h"
 
+#include <map>
 #include <mutex>
 
 namespace whir_gpu {
@@ -148,7 +149,7 @@ __global__ __launch_bounds__(THREADS_A) void lde_pass_a(LdeArgs a, uint64_t s0) 
 #define LDE_PA_PERSIST 1
 #endif
 
-// Pass A for M1 >= 256 when the tile fills the workgroup (TA * R2 threads).
+// Pass A for M1 >= 16 when the tile fills the workgroup (TA * R2 threads).
 // Persistent: workgroup b walks the tiles b, b + gridDim.x, ... (tile = column
 // c, first j2 J, as blockIdx.x of lde_pass_a) and runs every coset of the
 // launch on each.  Lane (t, n2) = (tid % TA, tid / TA) holds its step-1
@@ -412,6 +413,105 @@ __global__ __launch_bounds__(THREADS) void lde_pass_b_rows(LdeArgs a, uint64_t s
     }
 }
 
+// Pass B with M2 = 1024 and 16 Z columns per tile (TB * CB = 16), no fused
+// leaves: Ext3 codewords (split3, 48 base columns), column-major output and
+// the plain encode.  Persistent with the next tile's loads in flight, as
+// lde_pass_b_rows; tile = coset slot g * nb + block index bx of lde_pass_b.
+template <bool INV>
+__global__ __launch_bounds__(THREADS) void lde_pass_b_cols(LdeArgs a, uint64_t s0, uint32_t nb, uint32_t ntiles) {
+    constexpr int M2 = 1024, R = 32, PAD = M2 + M2 / R + 1;
+    static_assert(THREADS == 16 * R, "one thread per (Z column, n2)");
+    extern __shared__ uint64_t lds[];
+    uint64_t* twt = lds + 16 * PAD;
+    const uint32_t TB = a.TB, CB = a.CB, lgTB = a.lgTB, lgCB = a.lgCB;
+    const uint32_t CM2 = (uint32_t)(a.C * M2);
+    uint64_t pre[R];
+    auto split = [&](uint32_t d, uint32_t& tb, uint32_t& cb) {
+        tb = a.row_major ? d >> lgCB : d & (TB - 1);
+        cb = a.row_major ? d & (CB - 1) : d >> lgTB;
+    };
+    auto fetch = [&](uint32_t t, uint32_t tid) {
+        const uint32_t g = t / nb, bx = t - g * nb;
+        const uint32_t K0 = (bx & ((1u << a.lgNK) - 1)) << lgTB, C0 = (bx >> a.lgNK) << lgCB;
+        const uint64_t* zb = a.z + (uint64_t)g * a.zstride + (uint64_t)K0 * CM2 + (uint64_t)C0 * M2;
+        uint32_t tb, cb;
+        split(tid / R, tb, cb);
+        const uint32_t off = tb * CM2 + cb * M2 + tid % R;
+#pragma unroll
+        for (int n1 = 0; n1 < R; ++n1) pre[n1] = zb[off + R * n1];
+    };
+    uint32_t tile = blockIdx.x;
+    if (tile < ntiles) fetch(tile, threadIdx.x);
+    for (; tile < ntiles; tile += gridDim.x) {
+        uint32_t tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));  // lane indices re-derived per tile (see lde_pass_b_rows)
+        const uint32_t d = tid / R, n2 = tid % R;
+        const uint32_t g = tile / nb, bx = tile - g * nb;
+        const uint32_t K0 = (bx & ((1u << a.lgNK) - 1)) << lgTB, C0 = (bx >> a.lgNK) << lgCB;
+        const uint64_t s = s0 + g;
+        if (a.tw_in_b) {
+            const uint64_t i1 = s + a.E * K0;
+            for (uint32_t j2 = tid; j2 < M2; j2 += THREADS) twt[j2] = twid(a, (i1 * j2) & (a.n - 1));
+            __syncthreads();
+        }
+        uint64_t v[R];
+#pragma unroll
+        for (int n1 = 0; n1 < R; ++n1) v[n1] = a.tw_in_b ? gl::mul(pre[n1], twt[n2 + R * n1]) : pre[n1];
+        dft::dft_reg<R, INV>(v);
+#pragma unroll
+        for (int k1 = 1; k1 < R; ++k1) v[k1] = gl::mul(v[k1], a.t1024[(n2 * k1) & 1023]);
+#pragma unroll
+        for (int k1 = 0; k1 < R; ++k1) lds[d * PAD + dft::pidx<M2>(R * k1 + n2)] = v[k1];
+        if (tile + gridDim.x < ntiles) fetch(tile + gridDim.x, tid);
+        __syncthreads();
+        {
+            const uint32_t d2 = tid % 16, k1 = tid / 16;
+            dft::lds_dft_step2<M2, INV>(lds + d2 * PAD, k1, v);
+            uint32_t tb, cb;
+            split(d2, tb, cb);
+            const uint64_t i1 = s + a.E * (K0 + tb);
+            const uint64_t c = C0 + cb;
+#pragma unroll
+            for (int k2 = 0; k2 < R; ++k2) {
+                const uint64_t i = i1 + a.L * (k1 + (uint64_t)R * k2);
+                const uint64_t idx = a.row_major ? (i * a.C + c) : (c * a.n + i);
+                a.out[idx * a.out_el + a.out_comp] = v[k2];
+            }
+        }
+        __syncthreads();  // the next tile rewrites the LDS buffer and twiddles
+    }
+}
+
+// m == 1: every codeword entry of column c is its single coefficient.
+__global__ __launch_bounds__(256) void lde_broadcast_kernel(LdeArgs a) {
+    const uint64_t total = a.n * a.C;
+    for (uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t i, c;
+        if (a.row_major) {
+            i = q / a.C;
+            c = q - i * a.C;
+        } else {
+            c = q / a.n;
+            i = q - c * a.n;
+        }
+        a.out[q * a.out_el + a.out_comp] = a.in[in_index(a, c, 0)];
+    }
+}
+
+// workgroups of `fn` resident per CU (registers and LDS), cached per kernel and
+// LDS size; at least 1
+int resident_blocks(const void* fn, int threads, size_t lds) {
+    static std::mutex mu;
+    static std::map<std::pair<const void*, size_t>, int> cache;
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find({fn, lds});
+    if (it != cache.end()) return it->second;
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, threads, lds) != hipSuccess || n < 1) n = 1;
+    cache[{fn, lds}] = n;
+    return n;
+}
+
 // compute units of the current device (cached per device)
 int cu_count() {
     static int cache[64] = {};
@@ -439,6 +539,7 @@ uint64_t padded(uint64_t M) { return M + M / split_r2(M) + 1; }
 
 using KernelFn = void (*)(LdeArgs, uint64_t);
 using RowsFn = void (*)(LdeArgs, uint64_t, uint32_t);
+using ColsFn = void (*)(LdeArgs, uint64_t, uint32_t, uint32_t);
 
 template <bool INV>
 KernelFn pick_a(uint64_t M) {
@@ -640,10 +741,33 @@ int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* s
             rows_attr[inv] = true;
         }
     }
-    // M1 >= 256 with a full tile: the persistent register-tile pass A
+    // other M2 = 1024 shapes with 16 Z columns per tile: the persistent
+    // prefetching pass B without leaves
+    const bool cols = LDE_PB_PERSIST && !fuse && p.M2 == 1024 && !p.direct && p.TB * p.CB == 16 &&
+                      p.threads_b == THREADS;
+    ColsFn kc = s.inverse ? lde_pass_b_cols<true> : lde_pass_b_cols<false>;
+    if (cols) {
+        std::lock_guard<std::mutex> g(attr_mu);
+        static bool cols_attr[2] = {};
+        if (!cols_attr[inv]) {
+            (void)hipFuncSetAttribute((const void*)kc, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            cols_attr[inv] = true;
+        }
+    }
+    if (s.m == 1 && !fuse) {  // a broadcast: one memory-bound launch for all cosets
+        const uint64_t total = s.n * s.C;
+        const unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, 8192);
+        hipLaunchKernelGGL(lde_broadcast_kernel, dim3(blocks), dim3(256), 0, st, a);
+        return 0;
+    }
+    // M1 >= 16 with a full tile: the persistent register-tile pass A
     RowsFn kar = nullptr;
     if (LDE_PA_PERSIST && !p.direct && p.TA * split_r2(p.M1) == THREADS_A) {
         switch (p.M1) {
+            case 16: kar = s.inverse ? lde_pass_a_regs<16, true> : lde_pass_a_regs<16, false>; break;
+            case 32: kar = s.inverse ? lde_pass_a_regs<32, true> : lde_pass_a_regs<32, false>; break;
+            case 64: kar = s.inverse ? lde_pass_a_regs<64, true> : lde_pass_a_regs<64, false>; break;
+            case 128: kar = s.inverse ? lde_pass_a_regs<128, true> : lde_pass_a_regs<128, false>; break;
             case 256: kar = s.inverse ? lde_pass_a_regs<256, true> : lde_pass_a_regs<256, false>; break;
             case 512: kar = s.inverse ? lde_pass_a_regs<512, true> : lde_pass_a_regs<512, false>; break;
             case 1024: kar = s.inverse ? lde_pass_a_regs<1024, true> : lde_pass_a_regs<1024, false>; break;
@@ -672,7 +796,7 @@ int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* s
         gb.y = (unsigned)g;
         if (kar) {
             a.cpb = (uint32_t)g;
-            const uint32_t per_cu = (uint32_t)std::max<size_t>(1, (160 * 1024) / lds_a);
+            const uint32_t per_cu = (uint32_t)resident_blocks((const void*)kar, THREADS_A, lds_a);
             const uint32_t grid = std::min<uint32_t>(ga.x, per_cu * (uint32_t)cu_count());
             hipLaunchKernelGGL(kar, dim3(grid), dim3(THREADS_A), lds_a, st, a, s0, ga.x);
         } else if (!p.direct) {
@@ -683,6 +807,10 @@ int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* s
             const uint32_t ntiles = (uint32_t)(g * p.M1);
             const uint32_t grid = std::min<uint32_t>(ntiles, (uint32_t)cu_count());
             hipLaunchKernelGGL(kr, dim3(grid), dim3(THREADS), lds_b, st, a, s0, ntiles);
+        } else if (cols) {
+            const uint32_t ntiles = (uint32_t)(g * gb.x);
+            const uint32_t grid = std::min<uint32_t>(ntiles, (uint32_t)cu_count());
+            hipLaunchKernelGGL(kc, dim3(grid), dim3(THREADS), lds_b, st, a, s0, (uint32_t)gb.x, ntiles);
         } else {
             hipLaunchKernelGGL(kb, gb, dim3(p.threads_b), lds_b, st, a, s0);
         }

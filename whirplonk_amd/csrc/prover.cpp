@@ -174,12 +174,15 @@ int pow_device(whir_ctx* ctx, const uint8_t ch[32], uint64_t threshold, uint64_t
     unsigned long long* slots = (unsigned long long*)ctx->powbuf.p;  // two result slots
     PowChallenge pc;
     std::memcpy(pc.w, ch, 32);
-    // Window of candidates per launch: about 8x the expected work 2^d
-    // (d = 64 - log2(threshold)), between 2^14 and 2^24, at most 2^18 lanes.
+    // Window of candidates per launch: 2^(floor(d) + 2) for difficulty d =
+    // 64 - log2(threshold) (2 to 4 times the expected work), 2^8 to 2^26
+    // nonces, one nonce per lane up to 2^20 lanes.  One wide step finishes
+    // sooner than a narrower grid walking several nonces per lane, and a
+    // window holds a hit with probability at least 1 - e^-2.
     uint32_t lg_expect = 0;
     while (lg_expect < 64 && (threshold >> (63 - lg_expect)) == 0) ++lg_expect;  // floor(d)
-    const uint32_t lg_window = std::min<uint32_t>(24, std::max<uint32_t>(14, lg_expect + 3));
-    const uint32_t lg_lanes = std::min<uint32_t>(lg_window, 18);
+    const uint32_t lg_window = std::min<uint32_t>(26, std::max<uint32_t>(8, lg_expect + 2));
+    const uint32_t lg_lanes = std::min<uint32_t>(lg_window, 20);
     const uint32_t blocks = 1u << (lg_lanes - 8), per_thread = 1u << (lg_window - lg_lanes);
     const uint64_t window = 1ull << lg_window;
     for (uint64_t base = 0;; base += window) {
