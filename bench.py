@@ -457,7 +457,7 @@ def main():
     ap.add_argument("--legs", default="commit,prove,batched,served")
     ap.add_argument("--requests", type=int, default=2000, help="served requests")
     ap.add_argument("--offered", type=float, default=250.0, help="served offered rate (requests/s, Poisson)")
-    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--slots", type=int, default=16)
     ap.add_argument("--budget-gb", type=float, default=64.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 512 MiB device copy kernel (reference for HBM counter scaling)")
