@@ -26,6 +26,10 @@
 // With C = 16 columns the pass-B workgroup holds complete 128-byte rows, so
 // the BLAKE3 row leaves (matrix_commit.hpp:169-195) are hashed right there,
 // from LDS, before the rows are written out.
+// Large shapes run persistent forms of both passes (lde_pass_a_regs,
+// lde_pass_b_rows, lde_pass_b_cols): one workgroup per CU walks its tiles and
+// keeps the next tile's loads in flight in registers while it transforms the
+// current one, so the CU's VALU is not idle while a tile loads.
 #include <algorithm>
 
 #include "blake3_dev.h"
@@ -45,8 +49,6 @@ constexpr int THREADS = 512;
 #ifndef LDE_THREADS_B_SMALL
 #define LDE_THREADS_B_SMALL 512
 #endif
-// pass A uses fewer threads (and LDS) per workgroup than pass B, so that two
-// workgroups share a CU and one's loads overlap the other's arithmetic
 constexpr int THREADS_A = LDE_THREADS_A;
 
 struct LdeArgs {
