@@ -11,7 +11,7 @@
 namespace whir_gpu {
 namespace {
 
-constexpr uint64_t TOP_WIDTH = 2048;  // levels at most this wide run in one workgroup
+constexpr uint64_t TOP_WIDTH = 2048;  // levels at most this wide run in one workgroup of TOP_WIDTH/2 lanes
 
 __device__ __forceinline__ void store_hash(uint8_t* dst, const uint32_t* h) {
     uint4* d = reinterpret_cast<uint4*>(dst);
@@ -130,16 +130,37 @@ __global__ __launch_bounds__(256) void merkle_2level_kernel(uint8_t* __restrict_
     store_hash(nodes + 32 * (parent_off + child_width / 2 + i), g);
 }
 
-// The top of the tree: levels of width <= TOP_WIDTH/2 parents, one workgroup.
+// The top of the tree (child level of width <= TOP_WIDTH), one workgroup: the
+// level being reduced lives in LDS, so each level costs one compression per
+// lane plus two barriers instead of a round trip through the cache; every
+// parent is also written to the flat node array.
 __global__ __launch_bounds__(1024) void merkle_top_kernel(uint8_t* __restrict__ nodes, uint64_t child_off,
                                                           uint64_t child_width) {
+    __shared__ uint32_t lvl[TOP_WIDTH * 8];
+    for (uint64_t i = threadIdx.x; i < child_width; i += blockDim.x) {
+        const uint4* src = reinterpret_cast<const uint4*>(nodes + 32 * (child_off + i));
+        uint4 a = src[0], b = src[1];
+        uint32_t* d = lvl + 8 * i;
+        d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w;
+        d[4] = b.x; d[5] = b.y; d[6] = b.z; d[7] = b.w;
+    }
+    __syncthreads();
     while (child_width > 1) {
-        uint64_t width = child_width / 2, parent_off = child_off + child_width;
-        for (uint64_t i = threadIdx.x; i < width; i += blockDim.x) {
-            uint32_t m[16], h[8];
-            load64(nodes + 32 * (child_off + 2 * i), m);
+        // width <= TOP_WIDTH / 2 = blockDim.x: one parent per lane
+        const uint64_t width = child_width / 2, parent_off = child_off + child_width;
+        const uint32_t i = threadIdx.x;
+        uint32_t h[8];
+        if (i < width) {
+            uint32_t m[16];
+#pragma unroll
+            for (int w = 0; w < 16; ++w) m[w] = lvl[16 * i + w];
             b3::hash64(m, h);
             store_hash(nodes + 32 * (parent_off + i), h);
+        }
+        __syncthreads();  // every child read before any parent overwrites the level
+        if (i < width) {
+#pragma unroll
+            for (int w = 0; w < 8; ++w) lvl[8 * i + w] = h[w];
         }
         __syncthreads();
         child_off = parent_off;
@@ -219,7 +240,7 @@ void merkle_build_launch(uint8_t* nodes, uint64_t n_leaves, hipStream_t st) {
         off += width;
         width = parents;
     }
-    if (width > 1) hipLaunchKernelGGL(merkle_top_kernel, dim3(1), dim3(1024), 0, st, nodes, off, width);
+    if (width > 1) hipLaunchKernelGGL(merkle_top_kernel, dim3(1), dim3(TOP_WIDTH / 2), 0, st, nodes, off, width);
 }
 
 void gather_hashes_launch(const uint8_t* nodes, const uint64_t* idx, uint64_t count, uint8_t* out, hipStream_t st) {
