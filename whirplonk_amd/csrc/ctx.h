@@ -101,6 +101,46 @@ struct whir_ctx {
     // partial-sum buffer of the two-stage reductions
     int ensure_partials() { return ensure(partials, 6 * whir_gpu::SUMCHECK_MAX_BLOCKS * 8); }
 
+    // Block partials written by the kernels straight into pinned, device-mapped
+    // host memory (fine-grained: the stores cross PCIe as the blocks finish),
+    // so a reduction whose result the host needs costs no second-stage kernel
+    // and no copy: the host sums the g block totals after the stream sync.
+    // Waits for the stream, then sums the nv Ext3 totals of g blocks left in
+    // hpart by a launcher called with device_sum = false (lazy 128-bit sums:
+    // g <= 2048 canonical limbs stay below 2^75).
+    int host_sums(unsigned g, int nv, gl::e3* out) {
+        HIPCHK(hipStreamSynchronize(stream));
+        for (int v = 0; v < nv; ++v) {
+            unsigned __int128 acc[3] = {0, 0, 0};
+            for (unsigned b = 0; b < g; ++b) {
+                const uint64_t* q = hpart + 3 * ((uint64_t)nv * b + v);
+                acc[0] += q[0];
+                acc[1] += q[1];
+                acc[2] += q[2];
+            }
+            uint64_t c[3];
+            for (int k = 0; k < 3; ++k) c[k] = gl::reduce128((uint64_t)acc[k], (uint64_t)(acc[k] >> 64));
+            out[v] = {c[0], c[1], c[2]};
+        }
+        return 0;
+    }
+    uint64_t* hpart = nullptr;   // host address
+    uint64_t* hpart_d = nullptr;  // device address of the same memory
+    int ensure_hpart() {
+        if (hpart) return 0;
+        void* h = nullptr;
+        HIPCHK(hipHostMalloc(&h, 6 * whir_gpu::SUMCHECK_MAX_BLOCKS * 8, hipHostMallocMapped | hipHostMallocCoherent));
+        void* d = nullptr;
+        hipError_t e = hipHostGetDevicePointer(&d, h, 0);
+        if (e != hipSuccess) {
+            (void)hipHostFree(h);
+            return whir_gpu::fail("hipHostGetDevicePointer", e);
+        }
+        hpart = (uint64_t*)h;
+        hpart_d = (uint64_t*)d;
+        return 0;
+    }
+
     int twiddles(uint32_t log_n, bool inverse, whir_gpu::Twiddles& out) {
         auto key = std::make_pair(log_n, inverse);
         auto it = tw.find(key);

@@ -62,15 +62,18 @@ void fill_zero_launch(uint8_t* p, uint64_t bytes, hipStream_t st);
 void copy_launch(void* dst, const void* src, uint64_t bytes, hipStream_t st);
 
 // sumcheck over Ext3 (a may be base-field, lifted on the fly)
-// partials must hold 2*3*SUMCHECK_MAX_BLOCKS u64; result c0[3], c2[3] left in
-// partials[0..6) on the device.
+// partials must hold 2*3*SUMCHECK_MAX_BLOCKS u64.  With device_sum, the
+// result c0[3], c2[3] is left in partials[0..6) on the device; without it,
+// block b's (c0, c2) is left in partials[6b..6b+6) for the caller to sum.
+// Returns the number of blocks.
 constexpr int SUMCHECK_MAX_BLOCKS = 2048;
 
-void sumcheck_compute_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint64_t n,
-                             uint64_t* partials, hipStream_t st);
+unsigned sumcheck_compute_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint64_t n,
+                                 uint64_t* partials, hipStream_t st, bool device_sum = true);
 // compute + both folds in one sweep; an Ext3 input may be folded in place
-void sumcheck_round_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint64_t n, const uint64_t r[3],
-                           uint64_t* a_out, uint64_t* b_out, uint64_t* partials, hipStream_t st);
+unsigned sumcheck_round_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint64_t n, const uint64_t r[3],
+                               uint64_t* a_out, uint64_t* b_out, uint64_t* partials, hipStream_t st,
+                               bool device_sum = true);
 void sumcheck_fold_launch(const uint64_t* in, bool in_base, uint64_t n, const uint64_t r[3],
                           uint64_t* out, hipStream_t st);
 
@@ -109,9 +112,11 @@ void term_accumulate_launch(uint64_t* d_acc, uint64_t n, uint32_t L, const uint6
 void mle_eval_launch(const uint64_t* f, bool f_base, uint64_t n, uint32_t L, const uint64_t* hi, const uint64_t* lo,
                      uint64_t* partials, hipStream_t st);
 // fold (a, b) of length n (>= 4) by r to length n/2, and (c0, c2) of the folded
-// pair into partials[0..6) (device); Ext3 inputs may be folded in place
-void fold_compute_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint64_t n, const uint64_t r[3],
-                         uint64_t* a_out, uint64_t* b_out, uint64_t* partials, hipStream_t st);
+// pair into partials[0..6) (device_sum), or per block as sumcheck_compute_launch;
+// Ext3 inputs may be folded in place.  Returns the number of blocks.
+unsigned fold_compute_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint64_t n, const uint64_t r[3],
+                             uint64_t* a_out, uint64_t* b_out, uint64_t* partials, hipStream_t st,
+                             bool device_sum = true);
 // out[k*row_u64 + c] = matrix[idx[k]*row_u64 + c]
 void gather_rows_launch(const uint64_t* matrix, uint64_t row_u64, const uint64_t* d_idx, uint64_t count,
                         uint64_t* out, hipStream_t st);

@@ -264,13 +264,14 @@ void mle_eval_launch(const uint64_t* f, bool f_base, uint64_t n, uint32_t L, con
     hipLaunchKernelGGL(red::sum_blocks_kernel<1>, dim3(1), dim3(256), 0, st, partials, g);
 }
 
-void fold_compute_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint64_t n, const uint64_t r[3],
-                         uint64_t* a_out, uint64_t* b_out, uint64_t* partials, hipStream_t st) {
+unsigned fold_compute_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint64_t n, const uint64_t r[3],
+                             uint64_t* a_out, uint64_t* b_out, uint64_t* partials, hipStream_t st, bool device_sum) {
     unsigned g = std::min<unsigned>(blocks_of(n / 4), SUMCHECK_MAX_BLOCKS);
     e3 rr = {r[0], r[1], r[2]};
     hipLaunchKernelGGL(fold_compute_kernel, dim3(g), dim3(256), 0, st, a, a_base ? 1 : 0, b, n / 4, rr, a_out, b_out,
                        partials);
-    hipLaunchKernelGGL(red::sum_blocks_kernel<2>, dim3(1), dim3(256), 0, st, partials, g);
+    if (device_sum) hipLaunchKernelGGL(red::sum_blocks_kernel<2>, dim3(1), dim3(256), 0, st, partials, g);
+    return g;
 }
 
 void gather_rows_launch(const uint64_t* matrix, uint64_t row_u64, const uint64_t* d_idx, uint64_t count, uint64_t* out,

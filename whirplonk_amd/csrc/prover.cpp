@@ -281,13 +281,13 @@ int terms_apply(whir_ctx* ctx, const Terms& T, uint64_t* d_acc, uint32_t nv, boo
 }
 
 // --------------------------------------------------------------- sumcheck
-int read_pair(whir_ctx* ctx, e3& c0, e3& c2) {
-    uint64_t h[6];
-    HIPCHK(hipMemcpyAsync(h, ctx->partials.p, 48, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    c0 = {h[0], h[1], h[2]};
-    c2 = {h[3], h[4], h[5]};
-    return 0;
+// (c0, c2) of the g block partials a launcher left in the mapped host buffer
+int read_pair(whir_ctx* ctx, unsigned g, e3& c0, e3& c2) {
+    e3 c[2];
+    int rc = ctx->host_sums(g, 2, c);
+    c0 = c[0];
+    c2 = c[1];
+    return rc;
 }
 
 // sumcheck::Config::prove over (a, b) of length n: a_out receives the folded
@@ -300,12 +300,12 @@ int sumcheck_prove(whir_prover* P, const uint64_t* a, bool a_base, uint64_t* a_o
     if (rounds == 0) return 0;
     Phase ph(P, PH_SUMCHECK);
     int rc;
-    if ((rc = ctx->ensure_partials())) return rc;
-    uint64_t* part = (uint64_t*)ctx->partials.p;
+    if ((rc = ctx->ensure_hpart())) return rc;
+    uint64_t* part = ctx->hpart_d;
     e3 c0, c2;
-    sumcheck_compute_launch(a, a_base, b, n, part, ctx->stream);
+    unsigned g = sumcheck_compute_launch(a, a_base, b, n, part, ctx->stream, false);
     HIPCHK(hipGetLastError());
-    if ((rc = read_pair(ctx, c0, c2))) return rc;
+    if ((rc = read_pair(ctx, g, c0, c2))) return rc;
     for (size_t r = 0; r < rounds; ++r) {
         e3 c1 = gl::e3_sub(gl::e3_sub(sum, gl::e3_add(c0, c0)), c2);
         P->ts.message_e3(c0);
@@ -316,9 +316,9 @@ int sumcheck_prove(whir_prover* P, const uint64_t* a, bool a_base, uint64_t* a_o
         uint64_t rr[3] = {rnd.c0, rnd.c1, rnd.c2};
         e3 nc0 = E3_ZERO, nc2 = E3_ZERO;
         if (r + 1 < rounds) {
-            fold_compute_launch(a, a_base, b, n, rr, a_out, b, part, ctx->stream);
+            g = fold_compute_launch(a, a_base, b, n, rr, a_out, b, part, ctx->stream, false);
             HIPCHK(hipGetLastError());
-            if ((rc = read_pair(ctx, nc0, nc2))) return rc;
+            if ((rc = read_pair(ctx, g, nc0, nc2))) return rc;
         } else {
             sumcheck_fold_launch(a, a_base, n, rr, a_out, ctx->stream);
             sumcheck_fold_launch(b, false, n, rr, b, ctx->stream);

@@ -148,20 +148,23 @@ unsigned grid_for(uint64_t work, unsigned cap) {
 
 }  // namespace
 
-void sumcheck_compute_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint64_t n, uint64_t* partials,
-                             hipStream_t st) {
+unsigned sumcheck_compute_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint64_t n, uint64_t* partials,
+                                 hipStream_t st, bool device_sum) {
     unsigned g = grid_for(n / 2, SUMCHECK_MAX_BLOCKS);
     hipLaunchKernelGGL(sumcheck_compute_kernel, dim3(g), dim3(256), 0, st, a, a_base ? 1 : 0, b, n / 2, partials);
-    hipLaunchKernelGGL(red::sum_blocks_kernel<2>, dim3(1), dim3(256), 0, st, partials, g);
+    if (device_sum) hipLaunchKernelGGL(red::sum_blocks_kernel<2>, dim3(1), dim3(256), 0, st, partials, g);
+    return g;
 }
 
-void sumcheck_round_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint64_t n, const uint64_t r[3],
-                           uint64_t* a_out, uint64_t* b_out, uint64_t* partials, hipStream_t st) {
+unsigned sumcheck_round_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint64_t n, const uint64_t r[3],
+                               uint64_t* a_out, uint64_t* b_out, uint64_t* partials, hipStream_t st,
+                               bool device_sum) {
     unsigned g = grid_for(n / 2, SUMCHECK_MAX_BLOCKS);
     e3 rr = {r[0], r[1], r[2]};
     hipLaunchKernelGGL(sumcheck_round_kernel, dim3(g), dim3(256), 0, st, a, a_base ? 1 : 0, b, n / 2, rr, a_out, b_out,
                        partials);
-    hipLaunchKernelGGL(red::sum_blocks_kernel<2>, dim3(1), dim3(256), 0, st, partials, g);
+    if (device_sum) hipLaunchKernelGGL(red::sum_blocks_kernel<2>, dim3(1), dim3(256), 0, st, partials, g);
+    return g;
 }
 
 void sumcheck_fold_launch(const uint64_t* in, bool in_base, uint64_t n, const uint64_t r[3], uint64_t* out,

@@ -267,6 +267,7 @@ int whir_ctx_destroy(whir_ctx* ctx) {
     }
     for (auto* p : ctx->t1024)
         if (p) (void)hipFree(p);
+    if (ctx->hpart) (void)hipHostFree(ctx->hpart);
     for (auto* e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
@@ -394,16 +395,14 @@ int whir_dev_sumcheck_compute(whir_ctx* ctx, const uint64_t* d_a, int a_is_base,
                               uint64_t* out_c0, uint64_t* out_c2) {
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     if (n % 2) return bad_arg("sumcheck length must be even");
-    int rc = ctx->ensure_partials();
+    int rc = ctx->ensure_hpart();
     if (rc) return rc;
-    uint64_t* part = (uint64_t*)ctx->partials.p;
-    sumcheck_compute_launch(d_a, a_is_base != 0, d_b, n, part, ctx->stream);
+    unsigned blocks = sumcheck_compute_launch(d_a, a_is_base != 0, d_b, n, ctx->hpart_d, ctx->stream, false);
     HIPCHK(hipGetLastError());
-    uint64_t h[6];
-    HIPCHK(hipMemcpyAsync(h, part, 48, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    std::memcpy(out_c0, h, 24);
-    std::memcpy(out_c2, h + 3, 24);
+    gl::e3 c[2];
+    if ((rc = ctx->host_sums(blocks, 2, c))) return rc;
+    std::memcpy(out_c0, &c[0], 24);
+    std::memcpy(out_c2, &c[1], 24);
     return 0;
 }
 
@@ -450,20 +449,19 @@ int whir_dev_sumcheck_round(whir_ctx* ctx, const uint64_t* d_a, int a_is_base, c
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     if (n % 2) return bad_arg("sumcheck length must be even");
     if (a_is_base && (const void*)d_a == (const void*)d_a_out) return bad_arg("base-field fold cannot run in place");
-    int rc = ctx->ensure_partials();
+    int rc = ctx->ensure_hpart();
     if (rc) return rc;
-    uint64_t* part = (uint64_t*)ctx->partials.p;
     if ((rc = ctx->mark(0))) return rc;
     // Each output index i < n/2 reads only inputs i and i + n/2, so writing
     // the folds over the inputs' lower halves in place is also safe.
-    sumcheck_round_launch(d_a, a_is_base != 0, d_b, n, r, d_a_out, d_b_out, part, ctx->stream);
+    unsigned blocks = sumcheck_round_launch(d_a, a_is_base != 0, d_b, n, r, d_a_out, d_b_out, ctx->hpart_d,
+                                            ctx->stream, false);
     HIPCHK(hipGetLastError());
     if ((rc = ctx->mark(1))) return rc;
-    uint64_t h[6];
-    HIPCHK(hipMemcpyAsync(h, part, 48, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    std::memcpy(out_c0, h, 24);
-    std::memcpy(out_c2, h + 3, 24);
+    gl::e3 c[2];
+    if ((rc = ctx->host_sums(blocks, 2, c))) return rc;
+    std::memcpy(out_c0, &c[0], 24);
+    std::memcpy(out_c2, &c[1], 24);
     return ctx->collect(0, 1, 3);
 }
 
