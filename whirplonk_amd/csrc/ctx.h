@@ -52,7 +52,7 @@ struct whir_ctx {
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
     std::recursive_mutex mu;
-    whir_gpu::DevBuf scratch, partials, zpow, hin, hout, hnodes, hidx, hcomp;
+    whir_gpu::DevBuf scratch, partials, hin, hout, hnodes, hidx, hcomp;
     // prove loop: PoW words, term descriptors and tables, opened rows, vector, covector
     whir_gpu::DevBuf powbuf, tdesc, ttab, oidx, orows;
     uint32_t pow_slot = 0;  // PoW result slot of the next search window

@@ -77,11 +77,16 @@ unsigned sumcheck_round_launch(const uint64_t* a, bool a_base, const uint64_t* b
 void sumcheck_fold_launch(const uint64_t* in, bool in_base, uint64_t n, const uint64_t r[3],
                           uint64_t* out, hipStream_t st);
 
-// sum_i f_i z^i over Ext3 (f base or Ext3 AoS); the result lands in out[0..3)
-// on the device.  zpow[l] = z^(2^l) for l < 64 (Ext3 AoS, 192 u64) on device.
+// sum_i f_i z^i over Ext3 (f base or Ext3 AoS): block b's share lands in
+// partials[3b..3b+3) and the caller sums the returned number of blocks.
+// zp.w[3l..3l+3) = z^(2^l) for l <= HORNER_MAX_LOGT, passed by value.
 constexpr int HORNER_MAX_BLOCKS = 1024;
-void horner_launch(const uint64_t* f, bool f_ext3, uint64_t n, const uint64_t* zpow,
-                   uint64_t* partials, uint64_t* out, hipStream_t st);
+constexpr uint32_t HORNER_MAX_LOGT = 18;
+struct HornerPowers {
+    uint64_t w[3 * (HORNER_MAX_LOGT + 1)];
+};
+unsigned horner_launch(const uint64_t* f, bool f_ext3, uint64_t n, const HornerPowers& zp, uint64_t* partials,
+                       hipStream_t st);
 
 // ------------------------------------------------------------- prove loop
 // Scan nonces base + k, k < blocks*256*per_thread, for the smallest one whose
@@ -108,9 +113,10 @@ void term_tables_launch(const uint64_t* d_desc, const uint64_t* d_coords, uint32
 void term_accumulate_launch(uint64_t* d_acc, uint64_t n, uint32_t L, const uint64_t* hi_e, const uint64_t* lo_e,
                             uint32_t te, const uint64_t* hi_b, const uint64_t* lo_b, uint32_t tb, bool accumulate,
                             hipStream_t st);
-// sum_i f_i hi[i >> L] lo[i & mask] into partials[0..3) (device)
-void mle_eval_launch(const uint64_t* f, bool f_base, uint64_t n, uint32_t L, const uint64_t* hi, const uint64_t* lo,
-                     uint64_t* partials, hipStream_t st);
+// sum_i f_i hi[i >> L] lo[i & mask] into partials[0..3) (device_sum), or
+// per block (partials[3b..3b+3)); returns the number of blocks
+unsigned mle_eval_launch(const uint64_t* f, bool f_base, uint64_t n, uint32_t L, const uint64_t* hi,
+                         const uint64_t* lo, uint64_t* partials, hipStream_t st, bool device_sum = true);
 // fold (a, b) of length n (>= 4) by r to length n/2, and (c0, c2) of the folded
 // pair into partials[0..6) (device_sum), or per block as sumcheck_compute_launch;
 // Ext3 inputs may be folded in place.  Returns the number of blocks.

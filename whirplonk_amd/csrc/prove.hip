@@ -257,11 +257,12 @@ void term_accumulate_launch(uint64_t* d_acc, uint64_t n, uint32_t L, const uint6
                        lo_b, tb, accumulate ? 1 : 0);
 }
 
-void mle_eval_launch(const uint64_t* f, bool f_base, uint64_t n, uint32_t L, const uint64_t* hi, const uint64_t* lo,
-                     uint64_t* partials, hipStream_t st) {
+unsigned mle_eval_launch(const uint64_t* f, bool f_base, uint64_t n, uint32_t L, const uint64_t* hi,
+                         const uint64_t* lo, uint64_t* partials, hipStream_t st, bool device_sum) {
     unsigned g = std::min<unsigned>(blocks_of(n), SUMCHECK_MAX_BLOCKS);
     hipLaunchKernelGGL(mle_eval_kernel, dim3(g), dim3(256), 0, st, f, f_base ? 1 : 0, n, L, hi, lo, partials);
-    hipLaunchKernelGGL(red::sum_blocks_kernel<1>, dim3(1), dim3(256), 0, st, partials, g);
+    if (device_sum) hipLaunchKernelGGL(red::sum_blocks_kernel<1>, dim3(1), dim3(256), 0, st, partials, g);
+    return g;
 }
 
 unsigned fold_compute_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint64_t n, const uint64_t r[3],

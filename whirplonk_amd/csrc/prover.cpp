@@ -920,7 +920,7 @@ int whir_dev_mle_evaluate(whir_ctx* ctx, const uint64_t* d_f, int f_is_base, siz
     const uint64_t H = 1ull << (nv - L), LO = 1ull << L;
     if ((rc = ctx->ensure(ctx->tdesc, (T.desc_e.size() + T.coords.size() + 4) * 8))) return rc;
     if ((rc = ctx->ensure(ctx->ttab, (H + LO) * 24 + 256))) return rc;
-    if ((rc = ctx->ensure_partials())) return rc;
+    if ((rc = ctx->ensure_hpart())) return rc;
     std::vector<uint64_t> all(T.desc_e);
     all.insert(all.end(), T.coords.begin(), T.coords.end());
     uint64_t* dd = (uint64_t*)ctx->tdesc.p;
@@ -928,10 +928,12 @@ int whir_dev_mle_evaluate(whir_ctx* ctx, const uint64_t* d_f, int f_is_base, siz
     uint64_t* hi = (uint64_t*)ctx->ttab.p;
     uint64_t* lo = hi + 3 * H;
     term_tables_launch(dd, dd + T.desc_e.size(), 1, nv, L, hi, lo, false, ctx->stream);
-    mle_eval_launch(d_f, f_is_base != 0, 1ull << nv, L, hi, lo, (uint64_t*)ctx->partials.p, ctx->stream);
+    unsigned blocks = mle_eval_launch(d_f, f_is_base != 0, 1ull << nv, L, hi, lo, ctx->hpart_d, ctx->stream, false);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(out, ctx->partials.p, 24, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    e3 v;
+    // (also the stream sync that keeps `all` alive until its copy has run)
+    if ((rc = ctx->host_sums(blocks, 1, &v))) return rc;
+    std::memcpy(out, &v, 24);
     return 0;
 }
 

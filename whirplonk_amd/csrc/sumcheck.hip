@@ -107,15 +107,20 @@ __global__ __launch_bounds__(256) void sumcheck_fold_kernel(const uint64_t* __re
 }
 
 // Thread t of T = 2^logT threads evaluates the interleaved sub-polynomial
-// sum_q f[t + T q] Z^q (Z = z^T) by Horner, scales by z^t, and the grid sums.
+// sum_q f[t + T q] Z^q (Z = z^T) by Horner, scales by z^t, and each block's
+// total goes to partials[block].  The powers z^(2^l), l <= logT <= HORNER_MAX_LOGT,
+// come by value in the kernel arguments (read with scalar loads).
 __global__ __launch_bounds__(256) void horner_kernel(const uint64_t* __restrict__ f, int f_ext3, uint64_t n,
-                                                     const uint64_t* __restrict__ zpow, uint32_t logT,
+                                                     HornerPowers zp, uint32_t logT,
                                                      uint64_t* __restrict__ partials) {
     const uint64_t T = 1ull << logT;
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     e3 acc[1] = {{0, 0, 0}};
     if (t < n) {
-        const e3 Z = load_e3(zpow, logT);
+        e3 Z = {0, 0, 0};
+#pragma unroll
+        for (uint32_t l = 0; l <= HORNER_MAX_LOGT; ++l)
+            if (l == logT) Z = {zp.w[3 * l], zp.w[3 * l + 1], zp.w[3 * l + 2]};
         uint64_t nq = (n - t + T - 1) / T;
         for (uint64_t q = nq; q-- > 0;) {
             uint64_t i = t + T * q;
@@ -126,18 +131,12 @@ __global__ __launch_bounds__(256) void horner_kernel(const uint64_t* __restrict_
                 acc[0].c0 = gl::add(acc[0].c0, f[i]);
             }
         }
-        for (uint32_t l = 0; l < logT; ++l)
-            if ((t >> l) & 1) acc[0] = gl::e3_mul(acc[0], load_e3(zpow, l));
+#pragma unroll
+        for (uint32_t l = 0; l < HORNER_MAX_LOGT; ++l)
+            if (l < logT && ((t >> l) & 1)) acc[0] = gl::e3_mul(acc[0], e3{zp.w[3 * l], zp.w[3 * l + 1], zp.w[3 * l + 2]});
     }
     block_sum<1>(acc);
     if (threadIdx.x == 0) store_e3(partials, blockIdx.x, acc[0]);
-}
-
-__global__ __launch_bounds__(256) void reduce_sum_kernel(const uint64_t* partials, uint32_t nblocks, uint64_t* out) {
-    e3 acc[1] = {{0, 0, 0}};
-    for (uint32_t k = threadIdx.x; k < nblocks; k += blockDim.x) acc[0] = gl::e3_add(acc[0], load_e3(partials, k));
-    block_sum<1>(acc);
-    if (threadIdx.x == 0) store_e3(out, 0, acc[0]);
 }
 
 unsigned grid_for(uint64_t work, unsigned cap) {
@@ -174,14 +173,14 @@ void sumcheck_fold_launch(const uint64_t* in, bool in_base, uint64_t n, const ui
     hipLaunchKernelGGL(sumcheck_fold_kernel, dim3(g), dim3(256), 0, st, in, in_base ? 1 : 0, n / 2, rr, out);
 }
 
-void horner_launch(const uint64_t* f, bool f_ext3, uint64_t n, const uint64_t* zpow, uint64_t* partials,
-                   uint64_t* out, hipStream_t st) {
+unsigned horner_launch(const uint64_t* f, bool f_ext3, uint64_t n, const HornerPowers& zp, uint64_t* partials,
+                       hipStream_t st) {
     // T = blocks*256 threads, a power of two, about 16+ terms per thread
     uint32_t logT = 8;
-    while (logT < 18 && (1ull << (logT + 4)) < n) ++logT;
+    while (logT < HORNER_MAX_LOGT && (1ull << (logT + 4)) < n) ++logT;
     unsigned blocks = (unsigned)((1ull << logT) / 256);
-    hipLaunchKernelGGL(horner_kernel, dim3(blocks), dim3(256), 0, st, f, f_ext3 ? 1 : 0, n, zpow, logT, partials);
-    hipLaunchKernelGGL(reduce_sum_kernel, dim3(1), dim3(256), 0, st, partials, blocks, out);
+    hipLaunchKernelGGL(horner_kernel, dim3(blocks), dim3(256), 0, st, f, f_ext3 ? 1 : 0, n, zp, logT, partials);
+    return blocks;
 }
 
 }  // namespace whir_gpu

@@ -258,7 +258,7 @@ int whir_ctx_destroy(whir_ctx* ctx) {
     if (!ctx) return 0;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    for (DevBuf* b : {&ctx->scratch, &ctx->partials, &ctx->zpow, &ctx->hin, &ctx->hout, &ctx->hnodes, &ctx->hidx,
+    for (DevBuf* b : {&ctx->scratch, &ctx->partials, &ctx->hin, &ctx->hout, &ctx->hnodes, &ctx->hidx,
                       &ctx->hcomp, &ctx->powbuf, &ctx->tdesc, &ctx->ttab, &ctx->oidx, &ctx->orows})
         if (b->p) (void)hipFree(b->p);
     for (auto& kv : ctx->tw) {
@@ -422,24 +422,21 @@ int whir_dev_horner(whir_ctx* ctx, const uint64_t* d_f, int f_is_ext3, size_t n,
         out[0] = out[1] = out[2] = 0;
         return 0;
     }
-    int rc = ctx->ensure(ctx->zpow, 192 * 8 + 3 * 8);
+    int rc = ctx->ensure_hpart();
     if (rc) return rc;
-    rc = ctx->ensure_partials();
-    if (rc) return rc;
-    uint64_t h[192];
+    HornerPowers zp;
     gl::e3 p = {z[0], z[1], z[2]};
-    for (int l = 0; l < 64; ++l) {
-        h[3 * l] = p.c0;
-        h[3 * l + 1] = p.c1;
-        h[3 * l + 2] = p.c2;
+    for (uint32_t l = 0; l <= HORNER_MAX_LOGT; ++l) {
+        zp.w[3 * l] = p.c0;
+        zp.w[3 * l + 1] = p.c1;
+        zp.w[3 * l + 2] = p.c2;
         p = gl::e3_mul(p, p);
     }
-    uint64_t* dz = (uint64_t*)ctx->zpow.p;
-    HIPCHK(hipMemcpyAsync(dz, h, sizeof h, hipMemcpyHostToDevice, ctx->stream));
-    horner_launch(d_f, f_is_ext3 != 0, n, dz, (uint64_t*)ctx->partials.p, dz + 192, ctx->stream);
+    unsigned blocks = horner_launch(d_f, f_is_ext3 != 0, n, zp, ctx->hpart_d, ctx->stream);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(out, dz + 192, 24, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    gl::e3 v;
+    if ((rc = ctx->host_sums(blocks, 1, &v))) return rc;
+    std::memcpy(out, &v, 24);
     return 0;
 }
 
