@@ -54,7 +54,7 @@ struct whir_ctx {
     std::recursive_mutex mu;
     whir_gpu::DevBuf scratch, partials, hin, hout, hnodes, hidx, hcomp;
     // prove loop: PoW words, term descriptors and tables, opened rows, vector, covector
-    whir_gpu::DevBuf powbuf, tdesc, ttab, oidx, orows;
+    whir_gpu::DevBuf powbuf, tdesc, ttab;
     uint32_t pow_slot = 0;  // PoW result slot of the next search window
     // when set, LDE passes use this caller-owned scratch instead of `scratch`
     void* ext_scratch = nullptr;
@@ -126,6 +126,31 @@ struct whir_ctx {
     }
     uint64_t* hpart = nullptr;   // host address
     uint64_t* hpart_d = nullptr;  // device address of the same memory
+
+    // Grow-only mapped staging for openings: the gather kernels read the
+    // indices from it and write the rows and hashes into it.
+    void* hio = nullptr;
+    void* hio_d = nullptr;
+    size_t hio_bytes = 0;
+    int ensure_hio(size_t bytes) {
+        if (hio && hio_bytes >= bytes) return 0;
+        if (hio) {
+            HIPCHK(hipStreamSynchronize(stream));
+            HIPCHK(hipHostFree(hio));
+            hio = hio_d = nullptr;
+            hio_bytes = 0;
+        }
+        size_t want = std::max<size_t>(bytes, 64 << 10);
+        HIPCHK(hipHostMalloc(&hio, want, hipHostMallocMapped | hipHostMallocCoherent));
+        hipError_t e = hipHostGetDevicePointer(&hio_d, hio, 0);
+        if (e != hipSuccess) {
+            (void)hipHostFree(hio);
+            hio = nullptr;
+            return whir_gpu::fail("hipHostGetDevicePointer", e);
+        }
+        hio_bytes = want;
+        return 0;
+    }
     int ensure_hpart() {
         if (hpart) return 0;
         void* h = nullptr;

@@ -54,7 +54,8 @@ void leaf_hash_launch(const uint64_t* matrix, uint64_t rows, uint32_t row_u64, u
 // out[i] = BLAKE3(in[i*size .. (i+1)*size]) for arbitrary byte size.
 void hash_many_launch(const uint8_t* in, uint64_t size, uint64_t count, uint8_t* out, hipStream_t st);
 // Build the flat tree in nodes (leaves already in nodes[0..n), padded to 2^L).
-void merkle_build_launch(uint8_t* nodes, uint64_t n_leaves, hipStream_t st);
+// With n_leaves >= 2 and root_copy set, the root is also written there.
+void merkle_build_launch(uint8_t* nodes, uint64_t n_leaves, hipStream_t st, uint8_t* root_copy = nullptr);
 // out[i] = nodes[idx[i]] (32 bytes each)
 void gather_hashes_launch(const uint8_t* nodes, const uint64_t* idx, uint64_t count, uint8_t* out, hipStream_t st);
 void fill_zero_launch(uint8_t* p, uint64_t bytes, hipStream_t st);
@@ -90,14 +91,16 @@ unsigned horner_launch(const uint64_t* f, bool f_ext3, uint64_t n, const HornerP
 
 // ------------------------------------------------------------- prove loop
 // Scan nonces base + k, k < blocks*256*per_thread, for the smallest one whose
-// BLAKE3(ch || LE64(nonce) || 0^24) read as LE64 is <= threshold; *best
-// (~0 on entry) takes the minimum hit, and *reset is set to ~0 for the next
-// window (results alternate between two slots).
+// BLAKE3(ch || LE64(nonce) || 0^24) read as LE64 is <= threshold.  *best
+// (~0 on entry) is the running minimum that lets lanes stop early, *reset is
+// set to ~0 for the next window (the slots alternate), and block_min[b] gets
+// block b's smallest hit or ~0: the search's answer is the minimum over blocks.
 struct PowChallenge {
     uint32_t w[8];
 };
 void pow_search_launch(const PowChallenge& ch, uint64_t threshold, uint64_t base, uint32_t blocks,
-                       uint32_t per_thread, unsigned long long* d_best, unsigned long long* d_reset, hipStream_t st);
+                       uint32_t per_thread, unsigned long long* d_best, unsigned long long* d_reset,
+                       uint64_t* block_min, hipStream_t st);
 void set_u64_launch(uint64_t* p, uint64_t v, hipStream_t st);
 
 // Weight terms sum_t s_t w_t(i) over i < 2^nv, factored as hi_t[i >> L] * lo_t[i & (2^L-1)].

@@ -135,7 +135,7 @@ __global__ __launch_bounds__(256) void merkle_2level_kernel(uint8_t* __restrict_
 // lane plus two barriers instead of a round trip through the cache; every
 // parent is also written to the flat node array.
 __global__ __launch_bounds__(1024) void merkle_top_kernel(uint8_t* __restrict__ nodes, uint64_t child_off,
-                                                          uint64_t child_width) {
+                                                          uint64_t child_width, uint8_t* root_copy) {
     __shared__ uint32_t lvl[TOP_WIDTH * 8];
     for (uint64_t i = threadIdx.x; i < child_width; i += blockDim.x) {
         const uint4* src = reinterpret_cast<const uint4*>(nodes + 32 * (child_off + i));
@@ -166,6 +166,7 @@ __global__ __launch_bounds__(1024) void merkle_top_kernel(uint8_t* __restrict__ 
         child_off = parent_off;
         child_width = width;
     }
+    if (root_copy && threadIdx.x == 0) store_hash(root_copy, lvl);
 }
 
 __global__ __launch_bounds__(256) void gather_hashes_kernel(const uint8_t* __restrict__ nodes,
@@ -219,7 +220,7 @@ void copy_launch(void* dst, const void* src, uint64_t bytes, hipStream_t st) {
                        reinterpret_cast<const uint4*>(src), n16);
 }
 
-void merkle_build_launch(uint8_t* nodes, uint64_t n_leaves, hipStream_t st) {
+void merkle_build_launch(uint8_t* nodes, uint64_t n_leaves, hipStream_t st, uint8_t* root_copy) {
     uint64_t width = 1;
     while (width < n_leaves) width <<= 1;
     if (n_leaves < width) fill_zero_launch(nodes + 32 * n_leaves, 32 * (width - n_leaves), st);
@@ -240,7 +241,8 @@ void merkle_build_launch(uint8_t* nodes, uint64_t n_leaves, hipStream_t st) {
         off += width;
         width = parents;
     }
-    if (width > 1) hipLaunchKernelGGL(merkle_top_kernel, dim3(1), dim3(TOP_WIDTH / 2), 0, st, nodes, off, width);
+    if (width > 1)
+        hipLaunchKernelGGL(merkle_top_kernel, dim3(1), dim3(TOP_WIDTH / 2), 0, st, nodes, off, width, root_copy);
 }
 
 void gather_hashes_launch(const uint8_t* nodes, const uint64_t* idx, uint64_t count, uint8_t* out, hipStream_t st) {

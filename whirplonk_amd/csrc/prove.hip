@@ -43,12 +43,17 @@ using red::block_sum;
 using red::store_block_totals;
 
 // ------------------------------------------------------------------ PoW
+// Lanes walk their nonces in increasing order; `best` (device) is the running
+// minimum hit that lets lanes stop early, and each block also writes the
+// minimum of its own hits (or ~0) to block_min[blockIdx.x] for the host.
 __global__ __launch_bounds__(256) void pow_search_kernel(PowChallenge ch, uint64_t threshold, uint64_t base,
                                                          uint32_t per_thread, unsigned long long* best,
-                                                         unsigned long long* reset) {
-    // the other result slot is cleared here for the next window (the host has
-    // already read it)
+                                                         unsigned long long* reset, uint64_t* block_min) {
+    __shared__ unsigned long long smin;
+    if (threadIdx.x == 0) smin = ~0ull;
+    // the other early-exit slot is cleared here for the next window
     if (blockIdx.x == 0 && threadIdx.x == 0) *reset = ~0ull;
+    __syncthreads();
     uint32_t m[16];
 #pragma unroll
     for (int i = 0; i < 8; ++i) m[i] = ch.w[i];
@@ -70,9 +75,12 @@ __global__ __launch_bounds__(256) void pow_search_kernel(PowChallenge ch, uint64
         uint64_t h = ((uint64_t)out[1] << 32) | out[0];
         if (h <= threshold) {
             atomicMin(best, (unsigned long long)nonce);
+            atomicMin(&smin, (unsigned long long)nonce);
             break;
         }
     }
+    __syncthreads();
+    if (threadIdx.x == 0) block_min[blockIdx.x] = smin;
 }
 
 // ------------------------------------------------------------ term tables
@@ -233,9 +241,10 @@ unsigned blocks_of(uint64_t work) { return (unsigned)((work + 255) / 256); }
 }  // namespace
 
 void pow_search_launch(const PowChallenge& ch, uint64_t threshold, uint64_t base, uint32_t blocks,
-                       uint32_t per_thread, unsigned long long* d_best, unsigned long long* d_reset, hipStream_t st) {
+                       uint32_t per_thread, unsigned long long* d_best, unsigned long long* d_reset, uint64_t* block_min,
+                       hipStream_t st) {
     hipLaunchKernelGGL(pow_search_kernel, dim3(blocks), dim3(256), 0, st, ch, threshold, base, per_thread, d_best,
-                       d_reset);
+                       d_reset, block_min);
 }
 
 void set_u64_launch(uint64_t* p, uint64_t v, hipStream_t st) {

@@ -165,6 +165,7 @@ int sync(whir_prover* P) {
 // challenge, find the smallest nonce, send it as 8 LE bytes.
 int pow_device(whir_ctx* ctx, const uint8_t ch[32], uint64_t threshold, uint64_t* nonce, size_t* scanned) {
     int rc;
+    if ((rc = ctx->ensure_hpart())) return rc;
     if (!ctx->powbuf.p) {
         if ((rc = ctx->ensure(ctx->powbuf, 64))) return rc;
         set_u64_launch((uint64_t*)ctx->powbuf.p, ~0ull, ctx->stream);
@@ -188,11 +189,12 @@ int pow_device(whir_ctx* ctx, const uint8_t ch[32], uint64_t threshold, uint64_t
     for (uint64_t base = 0;; base += window) {
         const uint32_t k = ctx->pow_slot;
         ctx->pow_slot ^= 1;
-        pow_search_launch(pc, threshold, base, blocks, per_thread, slots + k, slots + (k ^ 1), ctx->stream);
+        pow_search_launch(pc, threshold, base, blocks, per_thread, slots + k, slots + (k ^ 1), ctx->hpart_d,
+                          ctx->stream);
         HIPCHK(hipGetLastError());
-        uint64_t best = ~0ull;
-        HIPCHK(hipMemcpyAsync(&best, slots + k, 8, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
+        uint64_t best = ~0ull;  // the block minima, in mapped host memory
+        for (uint32_t b = 0; b < blocks; ++b) best = std::min<uint64_t>(best, ctx->hpart[b]);
         if (best != ~0ull) {
             if (scanned) *scanned += best + 1;  // the serial scan's work
             *nonce = best;
@@ -390,23 +392,24 @@ int open_witness(whir_prover* P, const whir_host::IrsConfig& c, const DevWitness
     size_t nh = whir_merkle_hint_node_indices(w.rows, idx64.data(), k, nullptr);
     std::vector<uint64_t> hidx(nh);
     whir_merkle_hint_node_indices(w.rows, idx64.data(), k, hidx.data());
-    if ((rc = ctx->ensure(ctx->oidx, (k + nh) * 8 + 8))) return rc;
-    if ((rc = ctx->ensure(ctx->orows, (k * ru + nh * 4) * 8 + 8))) return rc;
-    uint64_t* didx = (uint64_t*)ctx->oidx.p;
-    uint64_t* drows = (uint64_t*)ctx->orows.p;
-    std::vector<uint64_t> up(idx64);
-    up.insert(up.end(), hidx.begin(), hidx.end());
-    if (!up.empty()) HIPCHK(hipMemcpyAsync(didx, up.data(), up.size() * 8, hipMemcpyHostToDevice, ctx->stream));
-    gather_rows_launch(w.matrix, ru, didx, k, drows, ctx->stream);
-    if (nh) gather_hashes_launch(w.nodes, didx + k, nh, (uint8_t*)(drows + k * ru), ctx->stream);
+    // Indices in, rows and hashes out, all through the mapped staging buffer:
+    // the gather kernels read the indices over PCIe and write their output
+    // straight to host memory, so the opening costs no copies.
+    // Layout (u64): rows [0, k*ru), hashes from the next even offset (their
+    // 16-byte stores stay aligned), then the row and node indices.
+    const size_t hoff = (k * ru + 1) & ~size_t{1}, ioff = hoff + 4 * nh;
+    if ((rc = ctx->ensure_hio((ioff + k + nh) * 8))) return rc;
+    uint64_t* back = (uint64_t*)ctx->hio;
+    uint64_t* back_d = (uint64_t*)ctx->hio_d;
+    std::memcpy(back + ioff, idx64.data(), k * 8);
+    std::memcpy(back + ioff + k, hidx.data(), nh * 8);
+    gather_rows_launch(w.matrix, ru, back_d + ioff, k, back_d, ctx->stream);
+    if (nh) gather_hashes_launch(w.nodes, back_d + ioff + k, nh, (uint8_t*)(back_d + hoff), ctx->stream);
     HIPCHK(hipGetLastError());
-    std::vector<uint64_t> back(k * ru + nh * 4);
-    if (!back.empty())
-        HIPCHK(hipMemcpyAsync(back.data(), drows, back.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
-    o.rows.assign(back.begin(), back.begin() + k * ru);
+    o.rows.assign(back, back + k * ru);
     P->ts.hint_vector(o.rows.data(), k * w.num_cols, w.ext ? 24 : 8);
-    for (size_t h = 0; h < nh; ++h) P->ts.hint(back.data() + k * ru + 4 * h, 32);
+    for (size_t h = 0; h < nh; ++h) P->ts.hint(back + hoff + 4 * h, 32);
     return 0;
 }
 

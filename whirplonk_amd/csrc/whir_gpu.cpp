@@ -191,10 +191,17 @@ int whir_gpu::commit_device(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_
     size_t row_u64 = num_polys * depth * (ext ? 3 : 1);
     if (!fused) leaf_hash_launch(d_matrix, cw, (uint32_t)row_u64, d_nodes, ctx->stream);
     if ((rc = ctx->mark(2))) return rc;
-    merkle_build_launch(d_nodes, cw, ctx->stream);
+    // from two leaves up the tree's last kernel also writes the root to mapped
+    // host memory, so reading it costs no copy
+    const bool mapped_root = out_root && cw >= 2;
+    if (mapped_root && (rc = ctx->ensure_hpart())) return rc;
+    merkle_build_launch(d_nodes, cw, ctx->stream, mapped_root ? (uint8_t*)ctx->hpart_d : nullptr);
     HIPCHK(hipGetLastError());
     if ((rc = ctx->mark(3))) return rc;
-    if (out_root) {
+    if (mapped_root) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        std::memcpy(out_root, ctx->hpart, 32);
+    } else if (out_root) {
         size_t nn = whir_merkle_num_nodes(cw);
         HIPCHK(hipMemcpyAsync(out_root, d_nodes + 32 * (nn - 1), 32, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -259,7 +266,7 @@ int whir_ctx_destroy(whir_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (DevBuf* b : {&ctx->scratch, &ctx->partials, &ctx->hin, &ctx->hout, &ctx->hnodes, &ctx->hidx,
-                      &ctx->hcomp, &ctx->powbuf, &ctx->tdesc, &ctx->ttab, &ctx->oidx, &ctx->orows})
+                      &ctx->hcomp, &ctx->powbuf, &ctx->tdesc, &ctx->ttab})
         if (b->p) (void)hipFree(b->p);
     for (auto& kv : ctx->tw) {
         (void)hipFree(kv.second.lo);
@@ -268,6 +275,7 @@ int whir_ctx_destroy(whir_ctx* ctx) {
     for (auto* p : ctx->t1024)
         if (p) (void)hipFree(p);
     if (ctx->hpart) (void)hipHostFree(ctx->hpart);
+    if (ctx->hio) (void)hipHostFree(ctx->hio);
     for (auto* e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
