@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <map>
 #include <mutex>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -126,6 +127,31 @@ struct whir_ctx {
     }
     uint64_t* hpart = nullptr;   // host address
     uint64_t* hpart_d = nullptr;  // device address of the same memory
+
+    // Pinned staging of the term descriptors: the copy out of it runs
+    // asynchronously; before the host rewrites it, it waits for the event
+    // recorded after the previous copy.
+    void* hdesc = nullptr;
+    size_t hdesc_bytes = 0;
+    hipEvent_t hdesc_ev = nullptr;
+    int stage_desc(const std::vector<uint64_t>& v, void* d_dst) {
+        const size_t bytes = v.size() * 8;
+        if (!bytes) return 0;
+        if (hdesc_ev) HIPCHK(hipEventSynchronize(hdesc_ev));
+        if (hdesc_bytes < bytes) {
+            if (hdesc) HIPCHK(hipHostFree(hdesc));
+            hdesc = nullptr;
+            hdesc_bytes = 0;
+            size_t want = std::max<size_t>(bytes, 64 << 10);
+            HIPCHK(hipHostMalloc(&hdesc, want, hipHostMallocDefault));
+            hdesc_bytes = want;
+        }
+        std::memcpy(hdesc, v.data(), bytes);
+        HIPCHK(hipMemcpyAsync(d_dst, hdesc, bytes, hipMemcpyHostToDevice, stream));
+        if (!hdesc_ev) HIPCHK(hipEventCreateWithFlags(&hdesc_ev, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(hdesc_ev, stream));
+        return 0;
+    }
 
     // Grow-only mapped staging for openings: the gather kernels read the
     // indices from it and write the rows and hashes into it.

@@ -75,6 +75,10 @@ unsigned sumcheck_compute_launch(const uint64_t* a, bool a_base, const uint64_t*
 unsigned sumcheck_round_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint64_t n, const uint64_t r[3],
                                uint64_t* a_out, uint64_t* b_out, uint64_t* partials, hipStream_t st,
                                bool device_sum = true);
+// the last round of a sumcheck: a (base or Ext3) into a_out and Ext3 b into
+// b_out, both folded by r in one launch
+void sumcheck_fold2_launch(const uint64_t* a, bool a_base, uint64_t* a_out, const uint64_t* b, uint64_t* b_out,
+                           uint64_t n, const uint64_t r[3], hipStream_t st);
 void sumcheck_fold_launch(const uint64_t* in, bool in_base, uint64_t n, const uint64_t r[3],
                           uint64_t* out, hipStream_t st);
 
@@ -108,10 +112,12 @@ void set_u64_launch(uint64_t* p, uint64_t v, hipStream_t st);
 // z (Ext3, power terms), z^(2^L) (Ext3, power terms), offset of the eq point
 // in the coordinate array (Ext3 elements, nv coordinates, first = top bit).
 constexpr uint32_t TERM_WORDS = 11;
-// hi: nterms * 2^(nv-L) Ext3; lo: nterms * 2^L Ext3, or u64 when lo_base
-// (power terms with z in the base field).
-void term_tables_launch(const uint64_t* d_desc, const uint64_t* d_coords, uint32_t nterms, uint32_t nv, uint32_t L,
-                        uint64_t* d_hi, uint64_t* d_lo, bool lo_base, hipStream_t st);
+// Both groups in one launch.  Ext3 group: hi_e ne * 2^(nv-L) Ext3, lo_e
+// ne * 2^L Ext3; base group (power terms with z in the base field): hi_b
+// nb * 2^(nv-L) Ext3, lo_b nb * 2^L u64.
+void term_tables_launch(const uint64_t* d_desc_e, uint32_t ne, const uint64_t* d_desc_b, uint32_t nb,
+                        const uint64_t* d_coords, uint32_t nv, uint32_t L, uint64_t* hi_e, uint64_t* lo_e,
+                        uint64_t* hi_b, uint64_t* lo_b, hipStream_t st);
 // acc[i] = (accumulate ? acc[i] : 0) + sum over both term groups
 void term_accumulate_launch(uint64_t* d_acc, uint64_t n, uint32_t L, const uint64_t* hi_e, const uint64_t* lo_e,
                             uint32_t te, const uint64_t* hi_b, const uint64_t* lo_b, uint32_t tb, bool accumulate,
@@ -126,6 +132,11 @@ unsigned mle_eval_launch(const uint64_t* f, bool f_base, uint64_t n, uint32_t L,
 unsigned fold_compute_launch(const uint64_t* a, bool a_base, const uint64_t* b, uint64_t n, const uint64_t r[3],
                              uint64_t* a_out, uint64_t* b_out, uint64_t* partials, hipStream_t st,
                              bool device_sum = true);
+// One opening: rows[k*row_u64 + c] = matrix[ridx[k]*row_u64 + c] and the node
+// hashes hashes[32h..] = nodes[32*hidx[h]..] (16-byte aligned), one launch
+void gather_open_launch(const uint64_t* matrix, uint64_t row_u64, const uint64_t* d_ridx, uint64_t count,
+                        uint64_t* rows, const uint8_t* nodes, const uint64_t* d_hidx, uint64_t nh, uint8_t* hashes,
+                        hipStream_t st);
 // out[k*row_u64 + c] = matrix[idx[k]*row_u64 + c]
 void gather_rows_launch(const uint64_t* matrix, uint64_t row_u64, const uint64_t* d_idx, uint64_t count,
                         uint64_t* out, hipStream_t st);

@@ -89,13 +89,20 @@ __global__ __launch_bounds__(256) void pow_search_kernel(PowChallenge ch, uint64
 enum : uint64_t { TERM_EQ = 0, TERM_POW = 1 };
 
 // hi tables: H entries per term (Ext3); lo tables: 2^L entries per term, Ext3
-// for eq terms and Ext3 powers, u64 for base-field powers (lo_base != 0).
-__global__ __launch_bounds__(256) void term_tables_kernel(const uint64_t* __restrict__ desc,
+// for eq terms and Ext3 powers, u64 for base-field powers.  Both term groups
+// in one launch: blockIdx.y < ne is term y of the Ext3 group (desc_e, hi_e,
+// lo_e), the rest the base-field power group (desc_b, hi_b, lo_b).
+__global__ __launch_bounds__(256) void term_tables_kernel(const uint64_t* __restrict__ desc_e,
+                                                          const uint64_t* __restrict__ desc_b,
                                                           const uint64_t* __restrict__ coords, uint32_t nv,
-                                                          uint32_t L, uint64_t* __restrict__ hi,
-                                                          uint64_t* __restrict__ lo, int lo_base) {
-    const uint32_t t = blockIdx.y;
-    const uint64_t* d = desc + (uint64_t)TERM_WORDS * t;
+                                                          uint32_t L, uint32_t ne, uint64_t* __restrict__ hi_e,
+                                                          uint64_t* __restrict__ lo_e, uint64_t* __restrict__ hi_b,
+                                                          uint64_t* __restrict__ lo_b) {
+    const int lo_base = blockIdx.y >= ne;
+    const uint32_t t = lo_base ? blockIdx.y - ne : blockIdx.y;
+    const uint64_t* d = (lo_base ? desc_b : desc_e) + (uint64_t)TERM_WORDS * t;
+    uint64_t* hi = lo_base ? hi_b : hi_e;
+    uint64_t* lo = lo_base ? lo_b : lo_e;
     const uint64_t H = 1ull << (nv - L), LO = 1ull << L;
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const e3 s = {d[1], d[2], d[3]};
@@ -220,6 +227,25 @@ __global__ __launch_bounds__(256) void gather_rows_kernel(const uint64_t* __rest
     out[k] = matrix[idx[r] * row_u64 + c];
 }
 
+// An opening in one launch: y = 0 gathers the rows (as gather_rows_kernel),
+// y = 1 the 32-byte node hashes (16 bytes per lane).
+__global__ __launch_bounds__(256) void gather_open_kernel(const uint64_t* __restrict__ matrix, uint64_t row_u64,
+                                                          const uint64_t* __restrict__ ridx, uint64_t count,
+                                                          uint64_t* __restrict__ rows, const uint8_t* __restrict__ nodes,
+                                                          const uint64_t* __restrict__ hidx, uint64_t nh,
+                                                          uint8_t* __restrict__ hashes) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.y == 0) {
+        if (k >= count * row_u64) return;
+        const uint64_t r = k / row_u64, c = k - r * row_u64;
+        rows[k] = matrix[ridx[r] * row_u64 + c];
+    } else {
+        if (k >= 2 * nh) return;
+        const uint4* s = reinterpret_cast<const uint4*>(nodes + 32 * hidx[k >> 1]);
+        reinterpret_cast<uint4*>(hashes)[k] = s[k & 1];
+    }
+}
+
 // ------------------------------------------------------------ vector RLC
 // out[i] = f_0[i] + sum_{k>=1} c_k f_k[i], f_k base-field vectors of length n
 __global__ __launch_bounds__(256) void rlc_combine_kernel(const uint64_t* __restrict__ f, uint64_t n, uint32_t nvec,
@@ -251,12 +277,13 @@ void set_u64_launch(uint64_t* p, uint64_t v, hipStream_t st) {
     hipLaunchKernelGGL(set_u64_kernel, dim3(1), dim3(64), 0, st, p, v);
 }
 
-void term_tables_launch(const uint64_t* d_desc, const uint64_t* d_coords, uint32_t nterms, uint32_t nv, uint32_t L,
-                        uint64_t* d_hi, uint64_t* d_lo, bool lo_base, hipStream_t st) {
-    if (!nterms) return;
+void term_tables_launch(const uint64_t* d_desc_e, uint32_t ne, const uint64_t* d_desc_b, uint32_t nb,
+                        const uint64_t* d_coords, uint32_t nv, uint32_t L, uint64_t* hi_e, uint64_t* lo_e,
+                        uint64_t* hi_b, uint64_t* lo_b, hipStream_t st) {
+    if (!ne && !nb) return;
     uint64_t span = std::max<uint64_t>(1ull << (nv - L), 1ull << L);
-    hipLaunchKernelGGL(term_tables_kernel, dim3(blocks_of(span), nterms), dim3(256), 0, st, d_desc, d_coords, nv, L,
-                       d_hi, d_lo, lo_base ? 1 : 0);
+    hipLaunchKernelGGL(term_tables_kernel, dim3(blocks_of(span), ne + nb), dim3(256), 0, st, d_desc_e, d_desc_b,
+                       d_coords, nv, L, ne, hi_e, lo_e, hi_b, lo_b);
 }
 
 void term_accumulate_launch(uint64_t* d_acc, uint64_t n, uint32_t L, const uint64_t* hi_e, const uint64_t* lo_e,
@@ -289,6 +316,15 @@ void gather_rows_launch(const uint64_t* matrix, uint64_t row_u64, const uint64_t
     if (!count || !row_u64) return;
     hipLaunchKernelGGL(gather_rows_kernel, dim3(blocks_of(count * row_u64)), dim3(256), 0, st, matrix, row_u64, d_idx,
                        count, out);
+}
+
+void gather_open_launch(const uint64_t* matrix, uint64_t row_u64, const uint64_t* d_ridx, uint64_t count,
+                        uint64_t* rows, const uint8_t* nodes, const uint64_t* d_hidx, uint64_t nh, uint8_t* hashes,
+                        hipStream_t st) {
+    const uint64_t span = std::max<uint64_t>(count * row_u64, 2 * nh);
+    if (!span) return;
+    hipLaunchKernelGGL(gather_open_kernel, dim3(blocks_of(span), 2), dim3(256), 0, st, matrix, row_u64, d_ridx, count,
+                       rows, nodes, d_hidx, nh, hashes);
 }
 
 void rlc_combine_launch(const uint64_t* f, uint64_t n, uint32_t nvec, const uint64_t* d_coef, uint64_t* out,

@@ -267,18 +267,15 @@ int terms_apply(whir_ctx* ctx, const Terms& T, uint64_t* d_acc, uint32_t nv, boo
     all.insert(all.end(), T.desc_e.begin(), T.desc_e.end());
     all.insert(all.end(), T.desc_b.begin(), T.desc_b.end());
     all.insert(all.end(), T.coords.begin(), T.coords.end());
-    if (!all.empty()) HIPCHK(hipMemcpyAsync(dd, all.data(), all.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = ctx->stage_desc(all, dd))) return rc;  // asynchronous: no stream sync here
     uint64_t* tab = (uint64_t*)ctx->ttab.p;
     uint64_t* hi_e = tab;
     uint64_t* lo_e = hi_e + he / 8;
     uint64_t* hi_b = lo_e + le / 8;
     uint64_t* lo_b = hi_b + hb / 8;
-    term_tables_launch(de, dc, T.ne, nv, L, hi_e, lo_e, false, ctx->stream);
-    term_tables_launch(db, dc, T.nb, nv, L, hi_b, lo_b, true, ctx->stream);
+    term_tables_launch(de, T.ne, db, T.nb, dc, nv, L, hi_e, lo_e, hi_b, lo_b, ctx->stream);
     term_accumulate_launch(d_acc, 1ull << nv, L, hi_e, lo_e, T.ne, hi_b, lo_b, T.nb, accumulate, ctx->stream);
     HIPCHK(hipGetLastError());
-    // the host staging vector goes out of scope: make sure the copy consumed it
-    HIPCHK(hipStreamSynchronize(ctx->stream));
     return 0;
 }
 
@@ -322,8 +319,7 @@ int sumcheck_prove(whir_prover* P, const uint64_t* a, bool a_base, uint64_t* a_o
             HIPCHK(hipGetLastError());
             if ((rc = read_pair(ctx, g, nc0, nc2))) return rc;
         } else {
-            sumcheck_fold_launch(a, a_base, n, rr, a_out, ctx->stream);
-            sumcheck_fold_launch(b, false, n, rr, b, ctx->stream);
+            sumcheck_fold2_launch(a, a_base, a_out, b, b, n, rr, ctx->stream);
             HIPCHK(hipGetLastError());
         }
         sum = gl::e3_add(gl::e3_mul(gl::e3_add(gl::e3_mul(c2, rnd), c1), rnd), c0);
@@ -403,8 +399,8 @@ int open_witness(whir_prover* P, const whir_host::IrsConfig& c, const DevWitness
     uint64_t* back_d = (uint64_t*)ctx->hio_d;
     std::memcpy(back + ioff, idx64.data(), k * 8);
     std::memcpy(back + ioff + k, hidx.data(), nh * 8);
-    gather_rows_launch(w.matrix, ru, back_d + ioff, k, back_d, ctx->stream);
-    if (nh) gather_hashes_launch(w.nodes, back_d + ioff + k, nh, (uint8_t*)(back_d + hoff), ctx->stream);
+    gather_open_launch(w.matrix, ru, back_d + ioff, k, back_d, w.nodes, back_d + ioff + k, nh,
+                       (uint8_t*)(back_d + hoff), ctx->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->stream));
     o.rows.assign(back, back + k * ru);
@@ -927,10 +923,10 @@ int whir_dev_mle_evaluate(whir_ctx* ctx, const uint64_t* d_f, int f_is_base, siz
     std::vector<uint64_t> all(T.desc_e);
     all.insert(all.end(), T.coords.begin(), T.coords.end());
     uint64_t* dd = (uint64_t*)ctx->tdesc.p;
-    HIPCHK(hipMemcpyAsync(dd, all.data(), all.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = ctx->stage_desc(all, dd))) return rc;
     uint64_t* hi = (uint64_t*)ctx->ttab.p;
     uint64_t* lo = hi + 3 * H;
-    term_tables_launch(dd, dd + T.desc_e.size(), 1, nv, L, hi, lo, false, ctx->stream);
+    term_tables_launch(dd, 1, nullptr, 0, dd + T.desc_e.size(), nv, L, hi, lo, nullptr, nullptr, ctx->stream);
     unsigned blocks = mle_eval_launch(d_f, f_is_base != 0, 1ull << nv, L, hi, lo, ctx->hpart_d, ctx->stream, false);
     HIPCHK(hipGetLastError());
     e3 v;

@@ -89,8 +89,16 @@ __global__ __launch_bounds__(256) void sumcheck_round_kernel(const uint64_t* a, 
     store_block_totals<2>(acc, partials);
 }
 
+// blockIdx.y = 1 folds the second (Ext3) vector in2 into out2 in the same launch
 __global__ __launch_bounds__(256) void sumcheck_fold_kernel(const uint64_t* __restrict__ in, int in_base,
-                                                            uint64_t half, e3 r, uint64_t* __restrict__ out) {
+                                                            uint64_t half, e3 r, uint64_t* __restrict__ out,
+                                                            const uint64_t* __restrict__ in2,
+                                                            uint64_t* __restrict__ out2) {
+    if (blockIdx.y == 1) {
+        in = in2;
+        out = out2;
+        in_base = 0;
+    }
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < half; i += stride) {
         e3 v;
@@ -170,7 +178,16 @@ void sumcheck_fold_launch(const uint64_t* in, bool in_base, uint64_t n, const ui
                           hipStream_t st) {
     unsigned g = grid_for(n / 2, 8192);
     e3 rr = {r[0], r[1], r[2]};
-    hipLaunchKernelGGL(sumcheck_fold_kernel, dim3(g), dim3(256), 0, st, in, in_base ? 1 : 0, n / 2, rr, out);
+    hipLaunchKernelGGL(sumcheck_fold_kernel, dim3(g), dim3(256), 0, st, in, in_base ? 1 : 0, n / 2, rr, out,
+                       (const uint64_t*)nullptr, (uint64_t*)nullptr);
+}
+
+void sumcheck_fold2_launch(const uint64_t* a, bool a_base, uint64_t* a_out, const uint64_t* b, uint64_t* b_out,
+                           uint64_t n, const uint64_t r[3], hipStream_t st) {
+    unsigned g = grid_for(n / 2, 8192);
+    e3 rr = {r[0], r[1], r[2]};
+    hipLaunchKernelGGL(sumcheck_fold_kernel, dim3(g, 2), dim3(256), 0, st, a, a_base ? 1 : 0, n / 2, rr, a_out, b,
+                       b_out);
 }
 
 unsigned horner_launch(const uint64_t* f, bool f_ext3, uint64_t n, const HornerPowers& zp, uint64_t* partials,

@@ -276,6 +276,8 @@ int whir_ctx_destroy(whir_ctx* ctx) {
         if (p) (void)hipFree(p);
     if (ctx->hpart) (void)hipHostFree(ctx->hpart);
     if (ctx->hio) (void)hipHostFree(ctx->hio);
+    if (ctx->hdesc) (void)hipHostFree(ctx->hdesc);
+    if (ctx->hdesc_ev) (void)hipEventDestroy(ctx->hdesc_ev);
     for (auto* e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
