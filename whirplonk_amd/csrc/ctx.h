@@ -125,6 +125,9 @@ struct whir_ctx {
         }
         return 0;
     }
+    // u64 entries: the sumcheck partials (6 per block) and one PoW minimum per
+    // block of the widest search window (2^22 lanes / 256)
+    static constexpr size_t HPART_U64 = 16384;
     uint64_t* hpart = nullptr;   // host address
     uint64_t* hpart_d = nullptr;  // device address of the same memory
 
@@ -180,7 +183,7 @@ struct whir_ctx {
     int ensure_hpart() {
         if (hpart) return 0;
         void* h = nullptr;
-        HIPCHK(hipHostMalloc(&h, 6 * whir_gpu::SUMCHECK_MAX_BLOCKS * 8, hipHostMallocMapped | hipHostMallocCoherent));
+        HIPCHK(hipHostMalloc(&h, HPART_U64 * 8, hipHostMallocMapped | hipHostMallocCoherent));
         void* d = nullptr;
         hipError_t e = hipHostGetDevicePointer(&d, h, 0);
         if (e != hipSuccess) {

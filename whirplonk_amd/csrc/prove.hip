@@ -44,7 +44,7 @@ using red::store_block_totals;
 
 // ------------------------------------------------------------------ PoW
 // Lanes walk their nonces in increasing order; `best` (device) is the running
-// minimum hit that lets lanes stop early, and each block also writes the
+// minimum hit that lets lanes and whole blocks stop early, and each block also writes the
 // minimum of its own hits (or ~0) to block_min[blockIdx.x] for the host.
 __global__ __launch_bounds__(256) void pow_search_kernel(PowChallenge ch, uint64_t threshold, uint64_t base,
                                                          uint32_t per_thread, unsigned long long* best,
@@ -53,7 +53,17 @@ __global__ __launch_bounds__(256) void pow_search_kernel(PowChallenge ch, uint64
     if (threadIdx.x == 0) smin = ~0ull;
     // the other early-exit slot is cleared here for the next window
     if (blockIdx.x == 0 && threadIdx.x == 0) *reset = ~0ull;
+    // A block dispatched after a hit below its first nonce was recorded has
+    // nothing to add: with more blocks than the GPU holds at once, the later
+    // ones mostly return here.
+    __shared__ int skip;
+    if (threadIdx.x == 0)
+        skip = __hip_atomic_load(best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < base + (uint64_t)blockIdx.x * blockDim.x;
     __syncthreads();
+    if (skip) {
+        if (threadIdx.x == 0) block_min[blockIdx.x] = ~0ull;
+        return;
+    }
     uint32_t m[16];
 #pragma unroll
     for (int i = 0; i < 8; ++i) m[i] = ch.w[i];

@@ -177,13 +177,15 @@ int pow_device(whir_ctx* ctx, const uint8_t ch[32], uint64_t threshold, uint64_t
     std::memcpy(pc.w, ch, 32);
     // Window of candidates per launch: 2^(floor(d) + 2) for difficulty d =
     // 64 - log2(threshold) (2 to 4 times the expected work), 2^8 to 2^26
-    // nonces, one nonce per lane up to 2^20 lanes.  One wide step finishes
-    // sooner than a narrower grid walking several nonces per lane, and a
-    // window holds a hit with probability at least 1 - e^-2.
+    // nonces, one nonce per lane up to 2^22 lanes (blocks dispatched after a
+    // lower hit is known return at once).  One wide step finishes sooner than
+    // a narrower grid walking several nonces per lane, and a window holds a
+    // hit with probability at least 1 - e^-2.
     uint32_t lg_expect = 0;
     while (lg_expect < 64 && (threshold >> (63 - lg_expect)) == 0) ++lg_expect;  // floor(d)
     const uint32_t lg_window = std::min<uint32_t>(26, std::max<uint32_t>(8, lg_expect + 2));
-    const uint32_t lg_lanes = std::min<uint32_t>(lg_window, 20);
+    const uint32_t lg_lanes = std::min<uint32_t>(lg_window, 22);
+    static_assert((size_t{1} << (22 - 8)) <= whir_ctx::HPART_U64, "one block minimum per block");
     const uint32_t blocks = 1u << (lg_lanes - 8), per_thread = 1u << (lg_window - lg_lanes);
     const uint64_t window = 1ull << lg_window;
     for (uint64_t base = 0;; base += window) {
