@@ -27,8 +27,9 @@ Beside the headline line's workload it measures, in the same JSON line:
     one graph launch, against one-at-a-time commits and the CPU roots;
   * served: commit+prove requests (sizes 2^16..2^24 with weights
     .3/.3/.2/.15/.05, rate 1/4, 3 MLE constraints) through the serving loop:
-    capacity, Poisson arrivals at a fixed offered rate with p50/p99 latency,
-    and the plain one-at-a-time GPU path on the same requests.
+    capacity, Poisson arrivals at a fixed offered rate (default 0.8x the
+    measured plain-path rate) with p50/p99 latency, and the plain
+    one-at-a-time GPU path on the same requests.
 
     python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--legs commit,prove,batched,served]
 """
@@ -335,10 +336,16 @@ def served_leg(n_req: int, offered: float, slots: int, budget_gb: float, cpu: bo
             ids = [srv.submit(1 << nv, pools[nv][0][0].array, pools[nv][0][1], borrow=True) for _ in range(slots)]
             [srv.wait(i) for i in ids]
         # capacity: everything submitted at once
+        st0 = srv.stats()
         t0 = time.perf_counter()
         ids = [srv.submit(1 << nv, pin.array, pts, borrow=True) for nv, (pin, pts) in reqs]
         res = [srv.wait(i) for i in ids]
         cap_s = time.perf_counter() - t0
+        st1 = srv.stats()
+        pow_launches = st1["pow_launches"] - st0["pow_launches"]
+        pow_searches = st1["pow_searches"] - st0["pow_searches"]
+        if offered <= 0:  # 0.8x the measured plain-path capacity
+            offered = round(0.8 * n_req / plain_s, 1)
         same_cap = all(res[i][:2] == ref[i] for i in range(n_req))
         # Poisson arrivals at the fixed offered rate
         arr = np.cumsum(np.random.default_rng(SERVE_SEED).exponential(1.0 / offered, n_req))
@@ -373,6 +380,7 @@ def served_leg(n_req: int, offered: float, slots: int, budget_gb: float, cpu: bo
         "proofs_per_s": round(n_req / run_s, 2),
         "p50_ms": round(lat[len(lat) // 2], 3), "p99_ms": round(lat[min(len(lat) - 1, int(0.99 * len(lat)))], 3),
         "peak_hbm_bytes": st["peak"],
+        "pow_searches_per_launch": round(pow_searches / max(pow_launches, 1), 2),
         "bit_exact_vs_plain": bool(same_cap and same_p),
         "bit_exact_vs_cpu": None,
     })
@@ -456,7 +464,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--legs", default="commit,prove,batched,served")
     ap.add_argument("--requests", type=int, default=2000, help="served requests")
-    ap.add_argument("--offered", type=float, default=250.0, help="served offered rate (requests/s, Poisson)")
+    ap.add_argument("--offered", type=float, default=0.0,
+                    help="served offered rate (requests/s, Poisson); 0: 0.8x the measured plain-path rate")
     ap.add_argument("--slots", type=int, default=16)
     ap.add_argument("--budget-gb", type=float, default=64.0)
     ap.add_argument("--calibrate-copy", action="store_true",

@@ -399,8 +399,8 @@ int whir_host_free(void* ptr);
  * queue wait, service time and end-to-end latency in ms. */
 int whir_server_wait(whir_server* server, uint64_t id, uint8_t* narg, size_t narg_cap, size_t* narg_len,
                      uint8_t* hints, size_t hints_cap, size_t* hints_len, double* out_ms);
-/* out[7]: completed, running, queued, arena bytes in use, peak bytes in use,
- * budget, slots. */
+/* out[9]: completed, running, queued, arena bytes in use, peak bytes in use,
+ * budget, slots, batched proof-of-work launches, searches they ran. */
 int whir_server_stats(whir_server* server, uint64_t* out);
 
 /* --------------------------------------------------------- stage timing */

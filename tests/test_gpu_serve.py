@@ -60,6 +60,8 @@ def test_mix_equals_alone(slots):
         st = srv.stats()
         assert st["completed"] == len(inputs)
         assert st["peak"] <= budget and st["in_use"] == 0
+        # every proof-of-work search went through the batcher
+        assert st["pow_searches"] >= st["pow_launches"] > 0
 
 
 def test_served_vs_reference():

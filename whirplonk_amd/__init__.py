@@ -730,7 +730,7 @@ class Server:
                                        ctypes.byref(nh), ms))
         return a[:na.value].tobytes(), b[:nh.value].tobytes(), (ms[0], ms[1], ms[2])
 
-    STATS = ("completed", "running", "queued", "in_use", "peak", "budget", "slots")
+    STATS = ("completed", "running", "queued", "in_use", "peak", "budget", "slots", "pow_launches", "pow_searches")
 
     def stats(self) -> dict:
         out = np.zeros(len(self.STATS), dtype=np.uint64)

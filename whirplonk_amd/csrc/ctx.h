@@ -40,6 +40,22 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// One proof-of-work search and its current window.
+constexpr uint32_t POW_MAX_LG_LANES = 22;
+struct PowJob {
+    uint8_t ch[32];
+    uint64_t threshold = 0, base = 0, window = 0;
+    uint32_t blocks = 0, per_thread = 0;
+};
+void pow_job_init(PowJob& j, const uint8_t ch[32], uint64_t threshold);
+// One launch over jobs[0..n) (n <= POW_BATCH_MAX, their blocks <= cap entries
+// of the mapped block-minimum buffer hmin / hmin_d).  found[k], nonce[k] get
+// job k's answer; a job without a hit moves to its next window.  slots holds
+// two sets of POW_BATCH_MAX device early-exit slots (~0 at start); `set`
+// alternates between them.  Waits for the stream.
+int pow_run(hipStream_t st, unsigned long long* slots, int& set, const uint64_t* hmin, uint64_t* hmin_d, size_t cap,
+            PowJob* const* jobs, int n, bool* found, uint64_t* nonce);
+
 }  // namespace whir_gpu
 
 #define HIPCHK(expr)                                              \
@@ -56,7 +72,7 @@ struct whir_ctx {
     whir_gpu::DevBuf scratch, partials, hin, hout, hnodes, hidx, hcomp;
     // prove loop: PoW words, term descriptors and tables, opened rows, vector, covector
     whir_gpu::DevBuf powbuf, tdesc, ttab;
-    uint32_t pow_slot = 0;  // PoW result slot of the next search window
+    int pow_slot = 0;  // early-exit slot set of the next PoW launch
     // when set, LDE passes use this caller-owned scratch instead of `scratch`
     void* ext_scratch = nullptr;
     size_t ext_scratch_bytes = 0;
@@ -128,6 +144,12 @@ struct whir_ctx {
     // u64 entries: the sumcheck partials (6 per block) and one PoW minimum per
     // block of the widest search window (2^22 lanes / 256)
     static constexpr size_t HPART_U64 = 16384;
+    static_assert((size_t{1} << (whir_gpu::POW_MAX_LG_LANES - 8)) <= HPART_U64, "one PoW minimum per block");
+
+    // A serving loop may take over the nonce searches (to batch them across
+    // its requests): the hook returns the nonce for (challenge, threshold).
+    int (*pow_hook)(void* arg, const uint8_t* ch, uint64_t threshold, uint64_t* nonce) = nullptr;
+    void* pow_hook_arg = nullptr;
     uint64_t* hpart = nullptr;   // host address
     uint64_t* hpart_d = nullptr;  // device address of the same memory
 

@@ -94,18 +94,25 @@ unsigned horner_launch(const uint64_t* f, bool f_ext3, uint64_t n, const HornerP
                        hipStream_t st);
 
 // ------------------------------------------------------------- prove loop
-// Scan nonces base + k, k < blocks*256*per_thread, for the smallest one whose
-// BLAKE3(ch || LE64(nonce) || 0^24) read as LE64 is <= threshold.  *best
-// (~0 on entry) is the running minimum that lets lanes stop early, *reset is
-// set to ~0 for the next window (the slots alternate), and block_min[b] gets
-// block b's smallest hit or ~0: the search's answer is the minimum over blocks.
-struct PowChallenge {
-    uint32_t w[8];
+// Up to POW_BATCH_MAX nonce searches in one launch.  Job k scans nonces
+// base + i, i < (block0_{k+1} - block0_k) * 256 * per_thread, for the smallest
+// one whose BLAKE3(ch || LE64(nonce) || 0^24) read as LE64 is <= threshold.
+// best[k] (~0 on entry) is job k's running minimum that lets lanes stop early;
+// reset[0..POW_BATCH_MAX) (the other slot set) is set to ~0 for the next
+// launch; block_min[b] gets block b's smallest hit or ~0.
+constexpr uint32_t POW_BATCH_MAX = 16;
+struct PowJobDesc {
+    uint32_t ch[8];
+    uint64_t threshold, base;
+    uint32_t block0, per_thread;
 };
-void pow_search_launch(const PowChallenge& ch, uint64_t threshold, uint64_t base, uint32_t blocks,
-                       uint32_t per_thread, unsigned long long* d_best, unsigned long long* d_reset,
-                       uint64_t* block_min, hipStream_t st);
-void set_u64_launch(uint64_t* p, uint64_t v, hipStream_t st);
+struct PowBatchArgs {
+    PowJobDesc job[POW_BATCH_MAX];
+    uint32_t njobs, total_blocks;
+};
+void pow_batch_launch(const PowBatchArgs& a, unsigned long long* d_best, unsigned long long* d_reset,
+                      uint64_t* block_min, hipStream_t st);
+void fill_u64_launch(uint64_t* p, uint64_t n, uint64_t v, hipStream_t st);
 
 // Weight terms sum_t s_t w_t(i) over i < 2^nv, factored as hi_t[i >> L] * lo_t[i & (2^L-1)].
 // Descriptor of a term, TERM_WORDS u64: kind (0 eq, 1 power), s (Ext3),

@@ -43,22 +43,41 @@ using red::block_sum;
 using red::store_block_totals;
 
 // ------------------------------------------------------------------ PoW
-// Lanes walk their nonces in increasing order; `best` (device) is the running
-// minimum hit that lets lanes and whole blocks stop early, and each block also writes the
-// minimum of its own hits (or ~0) to block_min[blockIdx.x] for the host.
-__global__ __launch_bounds__(256) void pow_search_kernel(PowChallenge ch, uint64_t threshold, uint64_t base,
-                                                         uint32_t per_thread, unsigned long long* best,
-                                                         unsigned long long* reset, uint64_t* block_min) {
+// Up to POW_BATCH_MAX independent searches in one launch (a single search is
+// a batch of one).  Job k owns blocks [block0_k, block0_{k+1}); its lanes
+// walk nonces base_k + lane, + lanes_k, ... in increasing order.  best[k]
+// (device) is job k's running minimum hit, which lets lanes and whole blocks
+// stop early; each block also writes the minimum of its own hits (or ~0) to
+// block_min[blockIdx.x], and the host takes the minimum over a job's blocks.
+__global__ __launch_bounds__(256) void pow_batch_kernel(PowBatchArgs a, unsigned long long* best,
+                                                        unsigned long long* reset, uint64_t* block_min) {
     __shared__ unsigned long long smin;
-    if (threadIdx.x == 0) smin = ~0ull;
-    // the other early-exit slot is cleared here for the next window
-    if (blockIdx.x == 0 && threadIdx.x == 0) *reset = ~0ull;
-    // A block dispatched after a hit below its first nonce was recorded has
-    // nothing to add: with more blocks than the GPU holds at once, the later
-    // ones mostly return here.
     __shared__ int skip;
-    if (threadIdx.x == 0)
-        skip = __hip_atomic_load(best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < base + (uint64_t)blockIdx.x * blockDim.x;
+    // the other set of early-exit slots is cleared for the next launch
+    if (blockIdx.x == 0 && threadIdx.x < POW_BATCH_MAX) reset[threadIdx.x] = ~0ull;
+    // this block's job (block-uniform; constant indices keep the arguments in
+    // scalar registers)
+    uint32_t jb = 0;
+#pragma unroll
+    for (uint32_t k = 1; k < POW_BATCH_MAX; ++k)
+        if (k < a.njobs && blockIdx.x >= a.job[k].block0) jb = k;
+    PowJobDesc J = a.job[0];
+    uint32_t next0 = a.total_blocks;
+#pragma unroll
+    for (uint32_t k = 0; k < POW_BATCH_MAX; ++k) {
+        if (k == jb) J = a.job[k];
+        if (k == jb + 1 && k < a.njobs) next0 = a.job[k].block0;
+    }
+    unsigned long long* bj = best + jb;
+    const uint64_t lanes = (uint64_t)(next0 - J.block0) * blockDim.x;
+    const uint64_t first = J.base + (uint64_t)(blockIdx.x - J.block0) * blockDim.x;
+    if (threadIdx.x == 0) {
+        smin = ~0ull;
+        // a block dispatched after a hit below its first nonce has nothing to
+        // add: with more blocks than the GPU holds at once, the later ones
+        // mostly return here
+        skip = __hip_atomic_load(bj, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < first;
+    }
     __syncthreads();
     if (skip) {
         if (threadIdx.x == 0) block_min[blockIdx.x] = ~0ull;
@@ -66,25 +85,23 @@ __global__ __launch_bounds__(256) void pow_search_kernel(PowChallenge ch, uint64
     }
     uint32_t m[16];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) m[i] = ch.w[i];
+    for (int i = 0; i < 8; ++i) m[i] = J.ch[i];
 #pragma unroll
     for (int i = 10; i < 16; ++i) m[i] = 0;
     uint32_t cv[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) cv[i] = b3::iv(i);
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    uint64_t nonce = base + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (uint32_t k = 0; k < per_thread; ++k, nonce += stride) {
-        // Each lane visits its nonces in increasing order, so once the best
-        // hit so far is below the lane's next nonce, the lane cannot improve it.
-        if (k && nonce > __hip_atomic_load(best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+    uint64_t nonce = first + threadIdx.x;
+    for (uint32_t k = 0; k < J.per_thread; ++k, nonce += lanes) {
+        // once a hit below the lane's next nonce is known, the lane cannot improve it
+        if (k && nonce > __hip_atomic_load(bj, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
         m[8] = (uint32_t)nonce;
         m[9] = (uint32_t)(nonce >> 32);
         uint32_t out[8];
         b3::compress(cv, m, 0, 64, b3::CHUNK_START | b3::CHUNK_END | b3::ROOT, out);
         uint64_t h = ((uint64_t)out[1] << 32) | out[0];
-        if (h <= threshold) {
-            atomicMin(best, (unsigned long long)nonce);
+        if (h <= J.threshold) {
+            atomicMin(bj, (unsigned long long)nonce);
             atomicMin(&smin, (unsigned long long)nonce);
             break;
         }
@@ -268,23 +285,23 @@ __global__ __launch_bounds__(256) void rlc_combine_kernel(const uint64_t* __rest
     st3(out, i, acc);
 }
 
-__global__ void set_u64_kernel(uint64_t* p, uint64_t v) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+__global__ void fill_u64_kernel(uint64_t* p, uint64_t n, uint64_t v) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 unsigned blocks_of(uint64_t work) { return (unsigned)((work + 255) / 256); }
 
 }  // namespace
 
-void pow_search_launch(const PowChallenge& ch, uint64_t threshold, uint64_t base, uint32_t blocks,
-                       uint32_t per_thread, unsigned long long* d_best, unsigned long long* d_reset, uint64_t* block_min,
-                       hipStream_t st) {
-    hipLaunchKernelGGL(pow_search_kernel, dim3(blocks), dim3(256), 0, st, ch, threshold, base, per_thread, d_best,
-                       d_reset, block_min);
+void pow_batch_launch(const PowBatchArgs& a, unsigned long long* d_best, unsigned long long* d_reset,
+                      uint64_t* block_min, hipStream_t st) {
+    hipLaunchKernelGGL(pow_batch_kernel, dim3(a.total_blocks), dim3(256), 0, st, a, d_best, d_reset, block_min);
 }
 
-void set_u64_launch(uint64_t* p, uint64_t v, hipStream_t st) {
-    hipLaunchKernelGGL(set_u64_kernel, dim3(1), dim3(64), 0, st, p, v);
+void fill_u64_launch(uint64_t* p, uint64_t n, uint64_t v, hipStream_t st) {
+    if (!n) return;
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(blocks_of(n)), dim3(256), 0, st, p, n, v);
 }
 
 void term_tables_launch(const uint64_t* d_desc_e, uint32_t ne, const uint64_t* d_desc_b, uint32_t nb,

@@ -165,44 +165,31 @@ int sync(whir_prover* P) {
 // challenge, find the smallest nonce, send it as 8 LE bytes.
 int pow_device(whir_ctx* ctx, const uint8_t ch[32], uint64_t threshold, uint64_t* nonce, size_t* scanned) {
     int rc;
+    if (ctx->pow_hook) {  // a serving loop batches the searches of its requests
+        if ((rc = ctx->pow_hook(ctx->pow_hook_arg, ch, threshold, nonce))) return rc;
+        if (scanned) *scanned += *nonce + 1;
+        return 0;
+    }
     if ((rc = ctx->ensure_hpart())) return rc;
     if (!ctx->powbuf.p) {
-        if ((rc = ctx->ensure(ctx->powbuf, 64))) return rc;
-        set_u64_launch((uint64_t*)ctx->powbuf.p, ~0ull, ctx->stream);
-        set_u64_launch((uint64_t*)ctx->powbuf.p + 1, ~0ull, ctx->stream);
+        if ((rc = ctx->ensure(ctx->powbuf, 2 * POW_BATCH_MAX * 8))) return rc;
+        fill_u64_launch((uint64_t*)ctx->powbuf.p, 2 * POW_BATCH_MAX, ~0ull, ctx->stream);
         ctx->pow_slot = 0;
     }
-    unsigned long long* slots = (unsigned long long*)ctx->powbuf.p;  // two result slots
-    PowChallenge pc;
-    std::memcpy(pc.w, ch, 32);
-    // Window of candidates per launch: 2^(floor(d) + 2) for difficulty d =
-    // 64 - log2(threshold) (2 to 4 times the expected work), 2^8 to 2^26
-    // nonces, one nonce per lane up to 2^22 lanes (blocks dispatched after a
-    // lower hit is known return at once).  One wide step finishes sooner than
-    // a narrower grid walking several nonces per lane, and a window holds a
-    // hit with probability at least 1 - e^-2.
-    uint32_t lg_expect = 0;
-    while (lg_expect < 64 && (threshold >> (63 - lg_expect)) == 0) ++lg_expect;  // floor(d)
-    const uint32_t lg_window = std::min<uint32_t>(26, std::max<uint32_t>(8, lg_expect + 2));
-    const uint32_t lg_lanes = std::min<uint32_t>(lg_window, 22);
-    static_assert((size_t{1} << (22 - 8)) <= whir_ctx::HPART_U64, "one block minimum per block");
-    const uint32_t blocks = 1u << (lg_lanes - 8), per_thread = 1u << (lg_window - lg_lanes);
-    const uint64_t window = 1ull << lg_window;
-    for (uint64_t base = 0;; base += window) {
-        const uint32_t k = ctx->pow_slot;
-        ctx->pow_slot ^= 1;
-        pow_search_launch(pc, threshold, base, blocks, per_thread, slots + k, slots + (k ^ 1), ctx->hpart_d,
-                          ctx->stream);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        uint64_t best = ~0ull;  // the block minima, in mapped host memory
-        for (uint32_t b = 0; b < blocks; ++b) best = std::min<uint64_t>(best, ctx->hpart[b]);
-        if (best != ~0ull) {
-            if (scanned) *scanned += best + 1;  // the serial scan's work
-            *nonce = best;
+    PowJob job;
+    pow_job_init(job, ch, threshold);
+    PowJob* jp = &job;
+    for (;;) {
+        bool found = false;
+        uint64_t n = 0;
+        if ((rc = pow_run(ctx->stream, (unsigned long long*)ctx->powbuf.p, ctx->pow_slot, ctx->hpart, ctx->hpart_d,
+                          whir_ctx::HPART_U64, &jp, 1, &found, &n)))
+            return rc;
+        if (found) {
+            if (scanned) *scanned += n + 1;  // the serial scan's work
+            *nonce = n;
             return 0;
         }
-        if (base > ~0ull - 2 * window) return bad_arg("proof of work found no nonce");
     }
 }
 

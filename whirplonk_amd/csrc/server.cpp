@@ -8,8 +8,12 @@
 // own context and HIP stream, so admitted requests run concurrently on the
 // device: their kernels, copies and host transcript work overlap, and the
 // codewords, trees and vectors of each request stay resident in its arena
-// block across all WHIR rounds.  Results are byte-identical to the plain
-// single-request path (the same prover code runs on the same inputs).
+// block across all WHIR rounds.  Same-stage work is batched across requests
+// where it has no data dependence on a request's device state: every
+// proof-of-work search of the running requests goes through one batcher that
+// launches the searches pending at that moment as one kernel.  Results are
+// byte-identical to the plain single-request path (the same prover code runs
+// on the same inputs; a search's nonce does not depend on its batch).
 #include "../../include/whir_gpu.h"
 
 #include <hip/hip_runtime.h>
@@ -97,6 +101,128 @@ struct Request {
     Clock::time_point t_submit, t_admit, t_done;
 };
 
+// Proof-of-work searches of all running requests, batched: a worker that
+// reaches a search queues it; if one of the LANES launch paths is free it
+// becomes the leader and launches everything queued (up to POW_BATCH_MAX
+// jobs) as one kernel on that path's stream, while searches that arrive
+// meanwhile queue for the next launch.  No request waits for a batch to fill: a lone search launches
+// at once, and under load the batches form by themselves.  A search is a pure
+// function of (challenge, threshold), so its nonce is the one it gets alone.
+class PowBatcher {
+public:
+    static constexpr int LANES = 2;  // launches in flight at once
+
+    int init(int device) {
+        HIPCHK(hipSetDevice(device));
+        cap_ = (size_t)POW_BATCH_MAX << (POW_MAX_LG_LANES - 8);  // one minimum per block
+        for (Lane& l : lanes_) {
+            HIPCHK(hipStreamCreateWithFlags(&l.st, hipStreamNonBlocking));
+            HIPCHK(hipMalloc((void**)&l.slots, 2 * POW_BATCH_MAX * 8));
+            fill_u64_launch((uint64_t*)l.slots, 2 * POW_BATCH_MAX, ~0ull, l.st);
+            void* h = nullptr;
+            HIPCHK(hipHostMalloc(&h, cap_ * 8, hipHostMallocMapped | hipHostMallocCoherent));
+            l.hmin = (uint64_t*)h;
+            HIPCHK(hipHostGetDevicePointer((void**)&l.hmin_d, h, 0));
+            HIPCHK(hipStreamSynchronize(l.st));
+        }
+        return 0;
+    }
+    ~PowBatcher() {
+        for (Lane& l : lanes_) {
+            if (l.st) (void)hipStreamSynchronize(l.st);
+            if (l.slots) (void)hipFree(l.slots);
+            if (l.hmin) (void)hipHostFree(l.hmin);
+            if (l.st) (void)hipStreamDestroy(l.st);
+        }
+    }
+    static int hook(void* self, const uint8_t* ch, uint64_t threshold, uint64_t* nonce) {
+        return static_cast<PowBatcher*>(self)->search(ch, threshold, nonce);
+    }
+    void stats(uint64_t* launches, uint64_t* jobs) {
+        std::lock_guard<std::mutex> lk(mu_);
+        *launches = launches_;
+        *jobs = jobs_;
+    }
+
+private:
+    struct Waiter {
+        PowJob job;
+        bool done = false;
+        int rc = 0;
+        uint64_t nonce = 0;
+    };
+    // one launch path: its stream, early-exit slot sets and block minima
+    struct Lane {
+        hipStream_t st = nullptr;
+        unsigned long long* slots = nullptr;
+        int set = 0;
+        uint64_t* hmin = nullptr;
+        uint64_t* hmin_d = nullptr;
+        bool busy = false;
+    };
+
+    int search(const uint8_t* ch, uint64_t threshold, uint64_t* nonce) {
+        Waiter w;
+        pow_job_init(w.job, ch, threshold);
+        std::unique_lock<std::mutex> lk(mu_);
+        queue_.push_back(&w);
+        while (!w.done) {
+            Lane* lane = nullptr;
+            for (Lane& l : lanes_)
+                if (!l.busy) lane = &l;
+            if (!lane || queue_.empty()) {
+                cv_.wait(lk);
+                continue;
+            }
+            // lead one launch over the queue
+            lane->busy = true;
+            std::vector<Waiter*> batch;
+            size_t blocks = 0;
+            while (!queue_.empty() && batch.size() < POW_BATCH_MAX && blocks + queue_.front()->job.blocks <= cap_) {
+                blocks += queue_.front()->job.blocks;
+                batch.push_back(queue_.front());
+                queue_.pop_front();
+            }
+            lk.unlock();
+            std::vector<PowJob*> jobs;
+            for (Waiter* x : batch) jobs.push_back(&x->job);
+            bool found[POW_BATCH_MAX];
+            uint64_t got[POW_BATCH_MAX];
+            int rc = pow_run(lane->st, lane->slots, lane->set, lane->hmin, lane->hmin_d, cap_, jobs.data(),
+                             (int)jobs.size(), found, got);
+            std::string err = rc ? last_error() : std::string();
+            lk.lock();
+            lane->busy = false;
+            ++launches_;
+            jobs_ += batch.size();
+            for (size_t k = 0; k < batch.size(); ++k) {
+                Waiter* x = batch[k];
+                if (rc) {
+                    x->rc = rc;
+                    x->done = true;
+                } else if (found[k]) {
+                    x->nonce = got[k];
+                    x->done = true;
+                } else {
+                    queue_.push_front(x);  // its next window goes first
+                }
+            }
+            if (rc) last_error() = err;
+            cv_.notify_all();
+        }
+        if (w.rc) return w.rc;
+        *nonce = w.nonce;
+        return 0;
+    }
+
+    Lane lanes_[LANES];
+    size_t cap_ = 0;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<Waiter*> queue_;
+    uint64_t launches_ = 0, jobs_ = 0;
+};
+
 struct Slot {
     whir_ctx* ctx = nullptr;
     uint8_t* pinned = nullptr;
@@ -118,6 +244,7 @@ struct whir_server {
     std::condition_variable cv_work, cv_done;
     std::deque<Request*> queue;
     std::map<uint64_t, std::unique_ptr<Request>> reqs;
+    PowBatcher pow;
     uint64_t next_id = 1;
     size_t completed = 0, running = 0;
     bool stop = false;
@@ -258,6 +385,10 @@ int whir_server_create(int device, const whir_params* params, uint32_t num_slots
     S->budget = (hbm_budget_bytes + 255) & ~size_t{255};
     HIPCHK(hipMalloc((void**)&S->region, S->budget));
     S->arena.init(S->region, S->budget);
+    if ((rc = S->pow.init(device))) {
+        (void)hipFree(S->region);
+        return rc;
+    }
     for (uint32_t i = 0; i < num_slots; ++i) {
         auto sl = std::make_unique<Slot>();
         if ((rc = whir_ctx_create(device, &sl->ctx))) {
@@ -265,6 +396,8 @@ int whir_server_create(int device, const whir_params* params, uint32_t num_slots
             (void)hipFree(S->region);
             return rc;
         }
+        sl->ctx->pow_hook = &PowBatcher::hook;
+        sl->ctx->pow_hook_arg = &S->pow;
         S->slots.push_back(std::move(sl));
     }
     for (auto& sl : S->slots) sl->th = std::thread(&whir_server::worker, S.get(), sl.get());
@@ -367,6 +500,7 @@ int whir_server_stats(whir_server* S, uint64_t* out) {
     out[4] = S->peak;
     out[5] = S->budget;
     out[6] = S->slots.size();
+    S->pow.stats(&out[7], &out[8]);
     return 0;
 }
 

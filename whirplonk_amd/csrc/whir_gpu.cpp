@@ -180,6 +180,62 @@ int whir_gpu::lde_device(whir_ctx* ctx, const LdeShape& s, const uint64_t* in, u
     return run_lde(ctx, s, in, out, leaves, fused);
 }
 
+void whir_gpu::pow_job_init(PowJob& j, const uint8_t ch[32], uint64_t threshold) {
+    std::memcpy(j.ch, ch, 32);
+    j.threshold = threshold;
+    j.base = 0;
+    // Window of candidates per launch: 2^(floor(d) + 2) for difficulty d =
+    // 64 - log2(threshold) (2 to 4 times the expected work), 2^8 to 2^26
+    // nonces, one nonce per lane up to 2^22 lanes (blocks dispatched after a
+    // lower hit is known return at once).  One wide step finishes sooner than
+    // a narrower grid walking several nonces per lane, and a window holds a
+    // hit with probability at least 1 - e^-2.
+    uint32_t lg_expect = 0;
+    while (lg_expect < 64 && (threshold >> (63 - lg_expect)) == 0) ++lg_expect;  // floor(d)
+    const uint32_t lg_window = std::min<uint32_t>(26, std::max<uint32_t>(8, lg_expect + 2));
+    const uint32_t lg_lanes = std::min<uint32_t>(lg_window, POW_MAX_LG_LANES);
+    j.blocks = 1u << (lg_lanes - 8);
+    j.per_thread = 1u << (lg_window - lg_lanes);
+    j.window = 1ull << lg_window;
+}
+
+int whir_gpu::pow_run(hipStream_t st, unsigned long long* slots, int& set, const uint64_t* hmin, uint64_t* hmin_d,
+                      size_t cap, PowJob* const* jobs, int n, bool* found, uint64_t* nonce) {
+    if (n < 1 || n > (int)POW_BATCH_MAX) return bad_arg("bad PoW batch");
+    PowBatchArgs a{};
+    uint32_t b0 = 0;
+    for (int k = 0; k < n; ++k) {
+        const PowJob& j = *jobs[k];
+        PowJobDesc& d = a.job[k];
+        std::memcpy(d.ch, j.ch, 32);
+        d.threshold = j.threshold;
+        d.base = j.base;
+        d.block0 = b0;
+        d.per_thread = j.per_thread;
+        b0 += j.blocks;
+    }
+    if (b0 > cap) return bad_arg("PoW batch exceeds the block-minimum buffer");
+    a.njobs = (uint32_t)n;
+    a.total_blocks = b0;
+    const int k = set;
+    set ^= 1;
+    pow_batch_launch(a, slots + k * POW_BATCH_MAX, slots + (k ^ 1) * POW_BATCH_MAX, hmin_d, st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    for (int q = 0; q < n; ++q) {
+        PowJob& j = *jobs[q];
+        uint64_t best = ~0ull;  // the block minima, in mapped host memory
+        for (uint32_t b = a.job[q].block0; b < a.job[q].block0 + j.blocks; ++b) best = std::min<uint64_t>(best, hmin[b]);
+        found[q] = best != ~0ull;
+        nonce[q] = best;
+        if (!found[q]) {
+            if (j.base > ~0ull - 2 * j.window) return bad_arg("proof of work found no nonce");
+            j.base += j.window;  // next window
+        }
+    }
+    return 0;
+}
+
 int whir_gpu::commit_device(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_polys, size_t poly_size, size_t cw,
                       size_t depth, uint64_t* d_matrix, uint8_t* d_nodes, uint8_t* out_root, int ext) {
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
