@@ -108,6 +108,10 @@ struct Request {
 // meanwhile queue for the next launch.  No request waits for a batch to fill: a lone search launches
 // at once, and under load the batches form by themselves.  A search is a pure
 // function of (challenge, threshold), so its nonce is the one it gets alone.
+#ifndef SERVE_POW_EXTRA_BITS
+#define SERVE_POW_EXTRA_BITS 1  // search windows of 1-2x the expected work (pow_job_init)
+#endif
+
 class PowBatcher {
 public:
     static constexpr int LANES = 2;  // launches in flight at once
@@ -163,7 +167,7 @@ private:
 
     int search(const uint8_t* ch, uint64_t threshold, uint64_t* nonce) {
         Waiter w;
-        pow_job_init(w.job, ch, threshold);
+        pow_job_init(w.job, ch, threshold, SERVE_POW_EXTRA_BITS);
         std::unique_lock<std::mutex> lk(mu_);
         queue_.push_back(&w);
         while (!w.done) {

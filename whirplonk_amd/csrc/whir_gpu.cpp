@@ -180,19 +180,22 @@ int whir_gpu::lde_device(whir_ctx* ctx, const LdeShape& s, const uint64_t* in, u
     return run_lde(ctx, s, in, out, leaves, fused);
 }
 
-void whir_gpu::pow_job_init(PowJob& j, const uint8_t ch[32], uint64_t threshold) {
+void whir_gpu::pow_job_init(PowJob& j, const uint8_t ch[32], uint64_t threshold, uint32_t extra_bits) {
     std::memcpy(j.ch, ch, 32);
     j.threshold = threshold;
     j.base = 0;
-    // Window of candidates per launch: 2^(floor(d) + 2) for difficulty d =
-    // 64 - log2(threshold) (2 to 4 times the expected work), 2^8 to 2^26
-    // nonces, one nonce per lane up to 2^22 lanes (blocks dispatched after a
-    // lower hit is known return at once).  One wide step finishes sooner than
-    // a narrower grid walking several nonces per lane, and a window holds a
-    // hit with probability at least 1 - e^-2.
+    // Window of candidates per launch: 2^(floor(d) + extra_bits) for
+    // difficulty d = 64 - log2(threshold), 2^8 to 2^26 nonces, one nonce per
+    // lane up to 2^22 lanes (blocks dispatched after a lower hit is known
+    // return at once).  One wide step finishes sooner than a narrower grid
+    // walking several nonces per lane.  With 2 extra bits (2 to 4 times the
+    // expected work) a window holds a hit with probability at least 1 - e^-2:
+    // the latency choice of a single prover.  With 1 (1 to 2 times) a search
+    // needs a second window more often but hashes about 40 % less: the
+    // throughput choice of the serving loop, whose GPU is shared.
     uint32_t lg_expect = 0;
     while (lg_expect < 64 && (threshold >> (63 - lg_expect)) == 0) ++lg_expect;  // floor(d)
-    const uint32_t lg_window = std::min<uint32_t>(26, std::max<uint32_t>(8, lg_expect + 2));
+    const uint32_t lg_window = std::min<uint32_t>(26, std::max<uint32_t>(8, lg_expect + extra_bits));
     const uint32_t lg_lanes = std::min<uint32_t>(lg_window, POW_MAX_LG_LANES);
     j.blocks = 1u << (lg_lanes - 8);
     j.per_thread = 1u << (lg_window - lg_lanes);
