@@ -160,7 +160,8 @@ int whir_dev_rs_encode(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_polys
 /* Ext3 AoS in and out. */
 int whir_dev_rs_encode_ext3(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_polys, size_t poly_size,
                             size_t codeword_length, size_t interleaving_depth, uint64_t* d_out);
-/* In-place forward (inverse != 0: inverse, unscaled) NTT of blocks of size. */
+/* In-place forward (inverse != 0: inverse, unscaled) NTT of blocks of size
+ * (a power of two <= 2^26). */
 int whir_dev_ntt_batch(whir_ctx* ctx, uint64_t* d_values, size_t total, size_t size, int inverse);
 /* d_leaves[i] = BLAKE3 of row i of a rows x row_u64 u64 matrix. */
 int whir_dev_commit_leaves(whir_ctx* ctx, const uint64_t* d_matrix, size_t rows, size_t row_u64, uint8_t* d_leaves);

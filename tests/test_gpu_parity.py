@@ -46,6 +46,36 @@ def test_ntt_batch(orc, log_n, batch):
     assert np.array_equal(inv, orc.intt_batch(got, n))
 
 
+@pytest.mark.parametrize("log_n,batch", [(21, 3), (22, 1), (24, 1)])
+def test_ntt_long(orc, log_n, batch):
+    # longer than 2^20: E-point DFTs with the twist, then the 2^20-point NTTs
+    n = 1 << log_n
+    x = rand(2000 + log_n, n * batch)
+    x[: len(SPECIAL)] = np.array(SPECIAL, dtype=np.uint64)
+    got = W.ntt_batch(x, n)
+    assert np.array_equal(got, orc.ntt_batch(x, n))
+    assert np.array_equal(W.ntt_batch(got, n, inverse=True), orc.intt_batch(got, n))
+
+
+@pytest.mark.parametrize("log_n", [25, 26])
+def test_ntt_long_round_trip(log_n):
+    # at the largest sizes, properties that fix the transform: the unscaled
+    # inverse of the forward transform is n * x (checked on a sample of
+    # entries), and the transform of the unit vector e_1 is the powers of a
+    # primitive n-th root of unity
+    n = 1 << log_n
+    x = rand(3000 + log_n, n)
+    back = W.ntt_batch(W.ntt_batch(x, n), n, inverse=True)
+    idx = np.random.default_rng(log_n).integers(0, n, 20000)
+    assert all(int(back[i]) == int(x[i]) * n % P for i in idx)
+    e1 = np.zeros(n, dtype=np.uint64)
+    e1[1] = 1
+    y = W.ntt_batch(e1, n)
+    w = int(y[1])
+    assert int(y[0]) == 1 and pow(w, n, P) == 1 and pow(w, n // 2, P) == P - 1
+    assert all(int(y[i]) == pow(w, int(i), P) for i in list(idx[:2000]) + [n - 1])
+
+
 def test_ntt_special_values(orc):
     n = 1 << 12
     x = np.array([SPECIAL[i % len(SPECIAL)] for i in range(n)], dtype=np.uint64)

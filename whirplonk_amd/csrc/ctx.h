@@ -69,7 +69,7 @@ struct whir_ctx {
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
     std::recursive_mutex mu;
-    whir_gpu::DevBuf scratch, partials, hin, hout, hnodes, hidx, hcomp;
+    whir_gpu::DevBuf scratch, nttbuf, partials, hin, hout, hnodes, hidx, hcomp;
     // prove loop: PoW words, term descriptors and tables, opened rows, vector, covector
     whir_gpu::DevBuf powbuf, tdesc, ttab;
     int pow_slot = 0;  // early-exit slot set of the next PoW launch

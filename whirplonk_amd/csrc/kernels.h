@@ -35,6 +35,11 @@ struct LdeShape {
     uint32_t lg_m2 = 0;
 };
 
+// Step 1 of an NTT longer than 2^20 (n = m * E, 2 <= E <= 64): see lde.hip.
+// g receives C * n values; tw is the size-n table of the direction.  Returns
+// nonzero for an unsupported shape.
+int ntt_long_step1_launch(const uint64_t* in, uint64_t* g, uint64_t n, uint64_t m, uint64_t C, const Twiddles& tw,
+                          bool inverse, hipStream_t st);
 // Scratch needed by lde_launch (elements of u64); 0 when m <= 1024 (one
 // pass reads the input directly).
 size_t lde_scratch_elems(const LdeShape& s);

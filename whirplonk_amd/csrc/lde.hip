@@ -527,6 +527,36 @@ int cu_count() {
     return cache[dev];
 }
 
+// NTTs longer than 2^20 (n = m * E, m = 2^20, 2 <= E <= 64), step 1: for
+// column c and j0 < m, the E-point DFT over f[j0 + m j1] (j1 < E) in
+// registers, output s twisted by w_n^(s j0), into g[c][s][j0].  Step 2 is the
+// m-point NTT of every g[c][s] written to rows i = s + E k (row-major over E
+// columns):  Y[s + E k] = sum_j0 w_m^(k j0) w_n^(s j0) sum_j1 f[j0 + m j1] w_E^(s j1).
+template <int E, bool INV>
+__global__ __launch_bounds__(256) void ntt_long_step1(const uint64_t* __restrict__ in, uint64_t* __restrict__ g,
+                                                      uint32_t lg_n, uint32_t lg_m, uint64_t C,
+                                                      const uint64_t* __restrict__ tw_lo,
+                                                      const uint64_t* __restrict__ tw_hi) {
+    const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (C << lg_m)) return;
+    const uint64_t m = 1ull << lg_m, c = q >> lg_m, j0 = q & (m - 1);
+    const uint64_t* f = in + (c << lg_n) + j0;
+    uint64_t v[E];
+#pragma unroll
+    for (int j1 = 0; j1 < E; ++j1) v[j1] = f[m * j1];
+    dft::dft_reg<E, INV>(v);
+    uint64_t w = tw_lo[j0 & 2047];  // w_n^j0 from the two-level table
+    if (j0 >> 11) w = gl::mul(w, tw_hi[j0 >> 11]);
+    uint64_t* o = g + (c << lg_n) + j0;
+    o[0] = v[0];
+    uint64_t t = w;
+#pragma unroll
+    for (int s = 1; s < E; ++s) {
+        o[m * s] = gl::mul(v[s], t);
+        t = gl::mul(t, w);
+    }
+}
+
 int ilog2u(uint64_t x) {
     int k = 0;
     while ((1ull << k) < x) ++k;
@@ -817,6 +847,25 @@ int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* s
             hipLaunchKernelGGL(kb, gb, dim3(p.threads_b), lds_b, st, a, s0);
         }
     }
+    return 0;
+}
+
+int ntt_long_step1_launch(const uint64_t* in, uint64_t* g, uint64_t n, uint64_t m, uint64_t C, const Twiddles& tw,
+                          bool inverse, hipStream_t st) {
+    using F = void (*)(const uint64_t*, uint64_t*, uint32_t, uint32_t, uint64_t, const uint64_t*, const uint64_t*);
+    F f = nullptr;
+    switch (n / m) {
+        case 2: f = inverse ? ntt_long_step1<2, true> : ntt_long_step1<2, false>; break;
+        case 4: f = inverse ? ntt_long_step1<4, true> : ntt_long_step1<4, false>; break;
+        case 8: f = inverse ? ntt_long_step1<8, true> : ntt_long_step1<8, false>; break;
+        case 16: f = inverse ? ntt_long_step1<16, true> : ntt_long_step1<16, false>; break;
+        case 32: f = inverse ? ntt_long_step1<32, true> : ntt_long_step1<32, false>; break;
+        case 64: f = inverse ? ntt_long_step1<64, true> : ntt_long_step1<64, false>; break;
+    }
+    if (!f || !is_pow2(m) || n % m || tw.log_n != (uint32_t)ilog2u(n)) return 1;
+    const uint64_t work = C * m;
+    hipLaunchKernelGGL(f, dim3((unsigned)((work + 255) / 256)), dim3(256), 0, st, in, g, (uint32_t)ilog2u(n),
+                       (uint32_t)ilog2u(m), C, tw.lo, tw.hi);
     return 0;
 }
 

@@ -324,7 +324,7 @@ int whir_ctx_destroy(whir_ctx* ctx) {
     if (!ctx) return 0;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    for (DevBuf* b : {&ctx->scratch, &ctx->partials, &ctx->hin, &ctx->hout, &ctx->hnodes, &ctx->hidx,
+    for (DevBuf* b : {&ctx->scratch, &ctx->nttbuf, &ctx->partials, &ctx->hin, &ctx->hout, &ctx->hnodes, &ctx->hidx,
                       &ctx->hcomp, &ctx->powbuf, &ctx->tdesc, &ctx->ttab})
         if (b->p) (void)hipFree(b->p);
     for (auto& kv : ctx->tw) {
@@ -405,7 +405,30 @@ int whir_dev_ntt_batch(whir_ctx* ctx, uint64_t* d_values, size_t total, size_t s
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     if (size == 0 || total % size) return bad_arg("total must be a multiple of size");
     if (total == 0 || size == 1) return 0;
-    if (!is_pow2(size) || size > (1ull << 20)) return bad_arg("NTT size must be a power of two <= 2^20");
+    if (!is_pow2(size) || size > (1ull << 26)) return bad_arg("NTT size must be a power of two <= 2^26");
+    if (size > (1ull << 20)) {
+        // n = m * E: E-point DFTs with the twist into scratch, then the
+        // m-point NTTs written back in rows s + E k
+        const uint64_t m = 1ull << 20, E = size / m, C = total / size;
+        Twiddles tw;
+        int rc = ctx->twiddles(log2u(size), inverse != 0, tw);
+        if (rc) return rc;
+        if ((rc = ctx->ensure(ctx->nttbuf, total * 8))) return rc;
+        uint64_t* g = (uint64_t*)ctx->nttbuf.p;
+        if (ntt_long_step1_launch(d_values, g, size, m, C, tw, inverse != 0, ctx->stream))
+            return bad_arg("NTT shape not supported");
+        HIPCHK(hipGetLastError());
+        for (uint64_t c = 0; c < C; ++c) {
+            LdeShape s2;
+            s2.C = E;
+            s2.m = m;
+            s2.n = m;
+            s2.row_major = true;
+            s2.inverse = inverse != 0;
+            if ((rc = run_lde(ctx, s2, g + c * size, d_values + c * size))) return rc;
+        }
+        return 0;
+    }
     LdeShape s;
     s.C = total / size;
     s.m = size;
