@@ -83,6 +83,53 @@ __device__ __forceinline__ void hash64(const uint32_t* m, uint32_t* out) {
     compress(cv, m, 0, 64, CHUNK_START | CHUNK_END | ROOT, out);
 }
 
+// Lane k of a quad (lanes 4q..4q+3) reads the value of lane (k + R) % 4.
+template <int R>
+__device__ __forceinline__ uint32_t quad_rot(uint32_t x) {
+    constexpr int ctrl = (R & 3) | (((R + 1) & 3) << 2) | (((R + 2) & 3) << 4) | (((R + 3) & 3) << 6);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, ctrl, 0xF, 0xF, false);
+}
+
+// hash64 shared by the four lanes of a quad (all four active): lane k holds
+// column k of the 4x4 state and runs the column G of index k; for the
+// diagonal step rows 1-3 rotate across the quad (DPP quad permutes) and back.
+// Each lane does 2 of the 8 G functions per round, so a node's latency is
+// about a third of hash64's.  m: the 16 message words (the same in the four
+// lanes).  Returns words k and 4 + k of the hash.
+__device__ __forceinline__ void hash64_quad(const uint32_t* m, uint32_t k, uint32_t& w_lo, uint32_t& w_hi) {
+    const bool k1 = k & 1, k2 = k & 2;
+    auto sel = [&](uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3) { return k2 ? (k1 ? x3 : x2) : (k1 ? x1 : x0); };
+    uint32_t a = sel(iv(0), iv(1), iv(2), iv(3));
+    uint32_t b = sel(iv(4), iv(5), iv(6), iv(7));
+    uint32_t c = a;  // v[8 + k] = iv(k)
+    uint32_t d = sel(0u, 0u, 64u, CHUNK_START | CHUNK_END | ROOT);  // counter, block length, flags
+    auto G = [](uint32_t& a, uint32_t& b, uint32_t& c, uint32_t& d, uint32_t x, uint32_t y) {
+        a = a + b + x;
+        d = rotr(d ^ a, 16);
+        c = c + d;
+        b = rotr(b ^ c, 12);
+        a = a + b + y;
+        d = rotr(d ^ a, 8);
+        c = c + d;
+        b = rotr(b ^ c, 7);
+    };
+#pragma unroll
+    for (int r = 0; r < 7; ++r) {
+        G(a, b, c, d, sel(m[sched(r, 0)], m[sched(r, 2)], m[sched(r, 4)], m[sched(r, 6)]),
+          sel(m[sched(r, 1)], m[sched(r, 3)], m[sched(r, 5)], m[sched(r, 7)]));
+        b = quad_rot<1>(b);
+        c = quad_rot<2>(c);
+        d = quad_rot<3>(d);
+        G(a, b, c, d, sel(m[sched(r, 8)], m[sched(r, 10)], m[sched(r, 12)], m[sched(r, 14)]),
+          sel(m[sched(r, 9)], m[sched(r, 11)], m[sched(r, 13)], m[sched(r, 15)]));
+        b = quad_rot<3>(b);
+        c = quad_rot<2>(c);
+        d = quad_rot<1>(d);
+    }
+    w_lo = a ^ c;
+    w_hi = b ^ d;
+}
+
 // Chaining value of one chunk (<= 1024 bytes) whose bytes are produced by
 // the functor load(word_index) -> uint32 (little-endian words; bytes past len
 // must read as zero).  is_root marks the single-chunk root case.

@@ -146,8 +146,31 @@ __global__ __launch_bounds__(1024) void merkle_top_kernel(uint8_t* __restrict__ 
     }
     __syncthreads();
     while (child_width > 1) {
-        // width <= TOP_WIDTH / 2 = blockDim.x: one parent per lane
         const uint64_t width = child_width / 2, parent_off = child_off + child_width;
+        if (width <= blockDim.x / 4) {
+            // narrow levels: one parent per quad of lanes (latency, not throughput)
+            const uint32_t i = threadIdx.x >> 2, k = threadIdx.x & 3;
+            uint32_t lo = 0, hi = 0;
+            if (i < width) {
+                uint32_t m[16];
+#pragma unroll
+                for (int w = 0; w < 16; ++w) m[w] = lvl[16 * i + w];
+                b3::hash64_quad(m, k, lo, hi);
+                uint32_t* dst = reinterpret_cast<uint32_t*>(nodes + 32 * (parent_off + i));
+                dst[k] = lo;
+                dst[4 + k] = hi;
+            }
+            __syncthreads();  // every child read before any parent overwrites the level
+            if (i < width) {
+                lvl[8 * i + k] = lo;
+                lvl[8 * i + 4 + k] = hi;
+            }
+            __syncthreads();
+            child_off = parent_off;
+            child_width = width;
+            continue;
+        }
+        // width <= TOP_WIDTH / 2 = blockDim.x: one parent per lane
         const uint32_t i = threadIdx.x;
         uint32_t h[8];
         if (i < width) {
