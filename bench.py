@@ -52,6 +52,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Merkle-committed codeword elem/s at 2^24; served proofs/s + p99 latency"
 HBM_PEAK_GBS = 8000.0
+VALU_PEAK_WAVE_INSTS = 256 * 2.4e9  # wave64 VALU instructions per second (1 per CU per cycle)
 NV, LOG_INV_RATE, FOLD = 24, 2, 4
 SEED = 0x5748495200000002  # config seed: "WHIR" || 2
 
@@ -74,13 +75,15 @@ def make_inputs():
     return coeffs, b, r
 
 
-def traffic_per_lde():
+def lde_profile():
+    """HBM bytes and VALU wave instructions per LDE, from the rocprofv3
+    counters recorded in profiles/lde_traffic.json (None when absent)."""
     path = os.path.join(ROOT, "profiles", "lde_traffic.json")
     if os.path.exists(path):
         with open(path) as f:
             d = json.load(f)
-        return d.get("hbm_bytes_per_lde")
-    return None
+        return d.get("hbm_bytes_per_lde"), d.get("valu_wave_insts_per_lde")
+    return None, None
 
 
 def gpu_leg(steps: int, warmup: int, coeffs, b, r):
@@ -420,11 +423,20 @@ def commit_leg(args):
     lde_avg_s = lde_ms / 1000.0 / max(lde_n, 1)
     alg_bytes = 8 * (poly + n_cw)
     achieved = alg_bytes / lde_avg_s / 1e9
-    traffic = traffic_per_lde()
+    traffic, valu_insts = lde_profile()
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "rs_encode (lde_pass_a + lde_pass_b over 4 cosets)",
                 "algorithmic_bytes": alg_bytes, "avg_ms": round(lde_avg_s * 1000, 4)}
+    # The LDE is bound by integer VALU issue, not HBM: its instruction count
+    # (SQ_INSTS_VALU of both passes, rocprofv3) over the live time, against one
+    # wave64 VALU instruction per CU per cycle (256 CUs at 2.4 GHz).
+    valu = None
+    if valu_insts:
+        rate = valu_insts / lde_avg_s
+        valu = {"bound": "valu", "achieved": round(rate, -6), "peak": VALU_PEAK_WAVE_INSTS,
+                "unit": "wave-instr/s", "frac": round(rate / VALU_PEAK_WAVE_INSTS, 4),
+                "wave_insts_per_launch": valu_insts}
     stage_ms = {k: round(v / max(c, 1), 4) for k, (v, c) in stages.items()}
 
     cpu = None
@@ -449,6 +461,7 @@ def commit_leg(args):
                    "hash": "blake3", "seed": hex(SEED)},
         "stage_ms": stage_ms,
         "roofline": roofline,
+        "valu_roofline": valu,
         "cpu_baseline": cpu,
         "bit_exact_vs_cpu": root_ok,
         "root": root.hex(),
