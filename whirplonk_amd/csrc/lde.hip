@@ -339,9 +339,16 @@ __global__ __launch_bounds__(THREADS) void lde_pass_b(LdeArgs a, uint64_t s0) {
 // (c, n2) = (tid / 32, tid % 32) loads the step-1 inputs of column c straight
 // into registers, so the input needs no LDS staging pass; LDS carries the
 // step-1 -> step-2 transposition and the row-major staging of the leaves.
+// LDS slot of (row r, column c) of a 16-column, 1024-row tile in the padded
+// DFT buffer (column stride 1057): the slot that step-2 item (c, r % 32)
+// read its input n2 = r / 32 from.  A wave reading 64 consecutive rows of one
+// column hits every bank pair four times (1057 and 33 are 1 mod 16).
+__device__ __forceinline__ int rows_idx(int r, int c) { return c * 1057 + 33 * (r & 31) + (r >> 5); }
+
 template <bool INV>
 __global__ __launch_bounds__(THREADS) void lde_pass_b_rows(LdeArgs a, uint64_t s0, uint32_t ntiles) {
     constexpr int M2 = 1024, R = 32, PAD = M2 + M2 / R + 1;
+    static_assert(PAD == 1057, "rows_idx assumes the padded 1024-point layout");
     static_assert(THREADS == 16 * R, "one thread per (column, n2)");
     extern __shared__ uint64_t lds[];
     uint64_t* twt = lds + 16 * PAD;  // past both the DFT buffer and the row stage
@@ -377,11 +384,14 @@ __global__ __launch_bounds__(THREADS) void lde_pass_b_rows(LdeArgs a, uint64_t s
         // the next tile's loads stay in flight through step 2 and the hashing
         if (tile + gridDim.x < ntiles) fetch(tile + gridDim.x);
         __syncthreads();
+        // Step 2: item (cb, k1) reads its 32 values buf[cb][33 k1 + n2] and
+        // writes row k1 + 32 k2 of column cb back into the same 32 slots
+        // (rows_idx), so no other item's input is touched and no barrier is
+        // needed between the reads and the writes.
         const uint32_t cb = tid % 16, k1 = tid / 16;
         dft::lds_dft_step2<M2, INV>(lds + cb * PAD, k1, v);
-        __syncthreads();
 #pragma unroll
-        for (int k2 = 0; k2 < R; ++k2) lds[stage_idx(k1 + R * k2, cb)] = v[k2];
+        for (int k2 = 0; k2 < R; ++k2) lds[rows_idx(k1 + R * k2, cb)] = v[k2];
         __syncthreads();
 #pragma unroll
         for (int h = 0; h < M2 / THREADS; ++h) {
@@ -393,7 +403,7 @@ __global__ __launch_bounds__(THREADS) void lde_pass_b_rows(LdeArgs a, uint64_t s
             for (int blk = 0; blk < 2; ++blk) {
 #pragma unroll
                 for (int cc = 0; cc < 8; ++cc) {
-                    uint64_t x = lds[stage_idx(r, 8 * blk + cc)];
+                    uint64_t x = lds[rows_idx(r, 8 * blk + cc)];
                     m[2 * cc] = (uint32_t)x;
                     m[2 * cc + 1] = (uint32_t)(x >> 32);
                 }
@@ -408,7 +418,7 @@ __global__ __launch_bounds__(THREADS) void lde_pass_b_rows(LdeArgs a, uint64_t s
         for (int h = 0; h < 16 * M2 / (2 * THREADS); ++h) {
             const uint32_t q = tid + h * THREADS;
             const uint32_t r = q / 8, cp = 2 * (q % 8);
-            const uint64_t x0 = lds[stage_idx(r, cp)], x1 = lds[stage_idx(r, cp + 1)];
+            const uint64_t x0 = lds[rows_idx(r, cp)], x1 = lds[rows_idx(r, cp + 1)];
             *reinterpret_cast<ulonglong2*>(a.out + (i1 + a.L * r) * 16 + cp) = make_ulonglong2(x0, x1);
         }
         __syncthreads();  // the next tile rewrites the twiddles and the stage
