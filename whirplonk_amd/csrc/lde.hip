@@ -351,7 +351,9 @@ __global__ __launch_bounds__(THREADS) void lde_pass_b_rows(LdeArgs a, uint64_t s
     static_assert(PAD == 1057, "rows_idx assumes the padded 1024-point layout");
     static_assert(THREADS == 16 * R, "one thread per (column, n2)");
     extern __shared__ uint64_t lds[];
-    uint64_t* twt = lds + 16 * PAD;  // past both the DFT buffer and the row stage
+    // inter-pass twiddles of this tile and the next (two tables past the DFT
+    // buffer): the next tile's are computed while this one is transformed
+    uint64_t* twt2 = lds + 16 * PAD;
     const uint32_t kmask = (1u << a.lgNK) - 1;
     uint64_t pre[R];
     uint32_t off = (threadIdx.x / R) * M2 + threadIdx.x % R;
@@ -361,18 +363,26 @@ __global__ __launch_bounds__(THREADS) void lde_pass_b_rows(LdeArgs a, uint64_t s
 #pragma unroll
         for (int n1 = 0; n1 < R; ++n1) pre[n1] = zb[off + R * n1];
     };
+    auto tile_i1 = [&](uint32_t t) { return s0 + (t >> a.lgNK) + a.E * (t & kmask); };
+    auto twiddles = [&](uint32_t t, uint64_t* dst, uint32_t tid) {
+        const uint64_t i1 = tile_i1(t);
+        for (uint32_t j2 = tid; j2 < M2; j2 += THREADS) dst[j2] = twid(a, (i1 * j2) & (a.n - 1));
+    };
     uint32_t tile = blockIdx.x;
-    if (tile < ntiles) fetch(tile);
-    for (; tile < ntiles; tile += gridDim.x) {
+    if (tile < ntiles) {
+        fetch(tile);
+        twiddles(tile, twt2, threadIdx.x);
+    }
+    __syncthreads();
+    for (uint32_t it = 0; tile < ntiles; tile += gridDim.x, ++it) {
         // The lane indices are re-derived every tile from an opaque copy of
         // the thread id: hoisted out of the loop, the per-lane twiddle and
         // stage addresses they feed would stay live and spill.
         uint32_t tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         const uint32_t c = tid / R, n2 = tid % R;
-        const uint64_t i1 = s0 + (tile >> a.lgNK) + a.E * (tile & kmask);
-        for (uint32_t j2 = tid; j2 < M2; j2 += THREADS) twt[j2] = twid(a, (i1 * j2) & (a.n - 1));
-        __syncthreads();
+        const uint64_t i1 = tile_i1(tile);
+        const uint64_t* twt = twt2 + (it & 1) * M2;
         uint64_t v[R];
 #pragma unroll
         for (int n1 = 0; n1 < R; ++n1) v[n1] = gl::mul(pre[n1], twt[n2 + R * n1]);
@@ -381,8 +391,12 @@ __global__ __launch_bounds__(THREADS) void lde_pass_b_rows(LdeArgs a, uint64_t s
         for (int k1 = 1; k1 < R; ++k1) v[k1] = gl::mul(v[k1], a.t1024[(n2 * k1) & 1023]);
 #pragma unroll
         for (int k1 = 0; k1 < R; ++k1) lds[c * PAD + dft::pidx<M2>(R * k1 + n2)] = v[k1];
-        // the next tile's loads stay in flight through step 2 and the hashing
-        if (tile + gridDim.x < ntiles) fetch(tile + gridDim.x);
+        // the next tile's loads stay in flight through step 2 and the hashing,
+        // and its twiddles go to the other table (last read two tiles ago)
+        if (tile + gridDim.x < ntiles) {
+            fetch(tile + gridDim.x);
+            twiddles(tile + gridDim.x, twt2 + ((it + 1) & 1) * M2, tid);
+        }
         __syncthreads();
         // Step 2: item (cb, k1) reads its 32 values buf[cb][33 k1 + n2] and
         // writes row k1 + 32 k2 of column cb back into the same 32 slots
@@ -848,7 +862,8 @@ int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* s
             // one resident workgroup per CU (the LDS use allows no more)
             const uint32_t ntiles = (uint32_t)(g * p.M1);
             const uint32_t grid = std::min<uint32_t>(ntiles, (uint32_t)cu_count());
-            hipLaunchKernelGGL(kr, dim3(grid), dim3(THREADS), lds_b, st, a, s0, ntiles);
+            const size_t lds_rows = (16 * padded(1024) + 2 * 1024) * 8;  // DFT buffer + two twiddle tables
+            hipLaunchKernelGGL(kr, dim3(grid), dim3(THREADS), lds_rows, st, a, s0, ntiles);
         } else if (cols) {
             const uint32_t ntiles = (uint32_t)(g * gb.x);
             const uint32_t grid = std::min<uint32_t>(ntiles, (uint32_t)cu_count());
