@@ -153,6 +153,7 @@ private:
         PowJob job;
         bool done = false;
         int rc = 0;
+        std::string error;  // the leader's message, re-raised in this thread
         uint64_t nonce = 0;
     };
     // one launch path: its stream, early-exit slot sets and block minima
@@ -203,6 +204,7 @@ private:
                 Waiter* x = batch[k];
                 if (rc) {
                     x->rc = rc;
+                    x->error = err;
                     x->done = true;
                 } else if (found[k]) {
                     x->nonce = got[k];
@@ -211,10 +213,12 @@ private:
                     queue_.push_front(x);  // its next window goes first
                 }
             }
-            if (rc) last_error() = err;
             cv_.notify_all();
         }
-        if (w.rc) return w.rc;
+        if (w.rc) {
+            last_error() = w.error;  // per-thread error text of this worker
+            return w.rc;
+        }
         *nonce = w.nonce;
         return 0;
     }
