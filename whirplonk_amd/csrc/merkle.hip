@@ -75,6 +75,34 @@ __global__ __launch_bounds__(256) void leaf_hash_kernel(const uint64_t* __restri
     store_hash(leaves + 32 * r, out);
 }
 
+// leaf_hash_kernel for rows of NB whole 64-byte blocks (row_u64 = 8 NB <= 128):
+// each block's 16 message words come in four 16-byte loads.
+template <int NB>
+__global__ __launch_bounds__(256) void leaf_hash_blocks_kernel(const uint64_t* __restrict__ mat, uint64_t rows,
+                                                               uint8_t* __restrict__ leaves) {
+    uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const uint4* row = reinterpret_cast<const uint4*>(mat + r * 8 * NB);
+    uint32_t cv[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) cv[i] = b3::iv(i);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        uint32_t m[16];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint4 v = row[4 * b + q];
+            m[4 * q] = v.x;
+            m[4 * q + 1] = v.y;
+            m[4 * q + 2] = v.z;
+            m[4 * q + 3] = v.w;
+        }
+        uint32_t flags = (b == 0 ? b3::CHUNK_START : 0) | (b + 1 == NB ? (b3::CHUNK_END | b3::ROOT) : 0);
+        b3::compress(cv, m, 0, 64, flags, cv);
+    }
+    store_hash(leaves + 32 * r, cv);
+}
+
 // out[i] = BLAKE3(in[i*size .. (i+1)*size]) for any byte size (hash_many,
 // hash/blake3_engine.hpp:90-199; both of its branches compute standard BLAKE3)
 __global__ __launch_bounds__(256) void hash_many_kernel(const uint8_t* __restrict__ in, uint64_t size,
@@ -220,7 +248,12 @@ unsigned blocks_for(uint64_t n, unsigned t) { return (unsigned)((n + t - 1) / t)
 
 void leaf_hash_launch(const uint64_t* matrix, uint64_t rows, uint32_t row_u64, uint8_t* leaves, hipStream_t st) {
     if (!rows) return;
-    hipLaunchKernelGGL(leaf_hash_kernel, dim3(blocks_for(rows, 256)), dim3(256), 0, st, matrix, rows, row_u64, leaves);
+    const dim3 g(blocks_for(rows, 256));
+    switch (row_u64) {  // the row shapes of the prover's commits: base 16, Ext3 48
+        case 16: hipLaunchKernelGGL(leaf_hash_blocks_kernel<2>, g, dim3(256), 0, st, matrix, rows, leaves); return;
+        case 48: hipLaunchKernelGGL(leaf_hash_blocks_kernel<6>, g, dim3(256), 0, st, matrix, rows, leaves); return;
+    }
+    hipLaunchKernelGGL(leaf_hash_kernel, g, dim3(256), 0, st, matrix, rows, row_u64, leaves);
 }
 
 void hash_many_launch(const uint8_t* in, uint64_t size, uint64_t count, uint8_t* out, hipStream_t st) {
