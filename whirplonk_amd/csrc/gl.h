@@ -20,60 +20,9 @@ namespace gl {
 __host__ __device__ __forceinline__ uint64_t canon(uint64_t x) { return x >= GL_P ? x - GL_P : x; }
 
 #if defined(__HIP_DEVICE_COMPILE__)
-// Device forms use explicit 32-bit carry chains (v_add_co/v_addc,
-// v_sub_co/v_subb) so that carries and borrows come from the adders instead
-// of 64-bit compares.
-__device__ __forceinline__ uint32_t lo32(uint64_t x) { return (uint32_t)x; }
-__device__ __forceinline__ uint32_t hi32(uint64_t x) { return (uint32_t)(x >> 32); }
-__device__ __forceinline__ uint64_t mk64(uint32_t lo, uint32_t hi) { return ((uint64_t)hi << 32) | lo; }
-
-// a, b canonical: s = a + b < 2p.  s - p = s + eps (mod 2^64); the subtraction
-// applies exactly when a + b carried or s + eps carried (s >= p).
-__device__ __forceinline__ uint64_t add(uint64_t a, uint64_t b) {
-    uint32_t c1, c2, c3, c4;
-    uint32_t s0 = __builtin_addc(lo32(a), lo32(b), 0u, &c1);
-    uint32_t s1 = __builtin_addc(hi32(a), hi32(b), c1, &c2);
-    uint32_t t0 = __builtin_addc(s0, 0xFFFFFFFFu, 0u, &c3);
-    uint32_t t1 = __builtin_addc(s1, 0u, c3, &c4);
-    bool over = (c2 | c4) != 0;
-    return mk64(over ? t0 : s0, over ? t1 : s1);
-}
-
-// a, b canonical: d = a - b; on borrow d + p = d - eps (mod 2^64).
-__device__ __forceinline__ uint64_t sub(uint64_t a, uint64_t b) {
-    uint32_t b1, b2, b3, b4;
-    uint32_t d0 = __builtin_subc(lo32(a), lo32(b), 0u, &b1);
-    uint32_t d1 = __builtin_subc(hi32(a), hi32(b), b1, &b2);
-    uint32_t e0 = __builtin_subc(d0, 0xFFFFFFFFu, 0u, &b3);
-    uint32_t e1 = __builtin_subc(d1, 0u, b3, &b4);
-    return mk64(b2 ? e0 : d0, b2 ? e1 : d1);
-}
-
-__device__ __forceinline__ uint64_t neg(uint64_t a) { return sub(0, a); }
-
-// x = hi*2^64 + lo (any 128-bit value) -> x mod p, canonical.
-//   t0 = lo - hi_hi          (on borrow: -2^64 = -eps)
-//   t1 = hi_lo * eps = (hi_lo << 32) - hi_lo    (< p)
-//   r  = t0 + t1             (on carry: +2^64 = +eps, and then r + eps < p)
-// The carry fix-up and the final "r >= p" subtraction are one select: both
-// are r + eps (mod 2^64), taken when t0 + t1 carried or r + eps carried.
-__device__ __forceinline__ uint64_t reduce128(uint64_t lo, uint64_t hi) {
-    uint32_t hl = lo32(hi), hh = hi32(hi);
-    uint32_t b1, b2, b3, b4, b5, b6, c1, c2, c3, c4;
-    uint32_t t00 = __builtin_subc(lo32(lo), hh, 0u, &b1);
-    uint32_t t01 = __builtin_subc(hi32(lo), 0u, b1, &b2);
-    uint32_t m = b2 ? 0xFFFFFFFFu : 0u;
-    t00 = __builtin_subc(t00, m, 0u, &b3);
-    t01 = __builtin_subc(t01, 0u, b3, &b4);
-    uint32_t t10 = __builtin_subc(0u, hl, 0u, &b5);
-    uint32_t t11 = __builtin_subc(hl, 0u, b5, &b6);
-    uint32_t r0 = __builtin_addc(t00, t10, 0u, &c1);
-    uint32_t r1 = __builtin_addc(t01, t11, c1, &c2);
-    uint32_t q0 = __builtin_addc(r0, 0xFFFFFFFFu, 0u, &c3);
-    uint32_t q1 = __builtin_addc(r1, 0u, c3, &c4);
-    bool sel = (c2 | c4) != 0;
-    return mk64(sel ? q0 : r0, sel ? q1 : r1);
-}
+#define GL_FORM __device__ __forceinline__
+#include "gl_forms.h"
+#undef GL_FORM
 #else
 __host__ __forceinline__ uint64_t add(uint64_t a, uint64_t b) {
     // a, b canonical: a + b < 2p; on carry the wrapped sum minus p is exact
