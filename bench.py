@@ -52,7 +52,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Merkle-committed codeword elem/s at 2^24; served proofs/s + p99 latency"
 HBM_PEAK_GBS = 8000.0
-VALU_PEAK_WAVE_INSTS = 256 * 2.4e9  # wave64 VALU instructions per second (1 per CU per cycle)
+# wave64 VALU instructions per second at one per CU per cycle: the issue ceiling
+# of the VOP3 forms (64-bit adds and compares, 32x32->64 multiplies, rotates)
+# that the LDE's field arithmetic and BLAKE3 consist of
+VALU_PEAK_WAVE_INSTS = 256 * 2.4e9
 NV, LOG_INV_RATE, FOLD = 24, 2, 4
 SEED = 0x5748495200000002  # config seed: "WHIR" || 2
 
@@ -430,7 +433,8 @@ def commit_leg(args):
                 "algorithmic_bytes": alg_bytes, "avg_ms": round(lde_avg_s * 1000, 4)}
     # The LDE is bound by integer VALU issue, not HBM: its instruction count
     # (SQ_INSTS_VALU of both passes, rocprofv3) over the live time, against one
-    # wave64 VALU instruction per CU per cycle (256 CUs at 2.4 GHz).
+    # wave64 VALU instruction per CU per cycle (256 CUs at 2.4 GHz), the issue
+    # ceiling of its instruction mix.
     valu = None
     if valu_insts:
         rate = valu_insts / lde_avg_s
