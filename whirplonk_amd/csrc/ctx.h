@@ -40,6 +40,22 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
+// The host waits for a stream by polling a word of pinned, device-mapped host
+// memory that a one-lane kernel (signal_launch) sets after the stream's
+// earlier work: about 6 us from a short kernel's end to the host seeing it,
+// against about 11 us through hipStreamSynchronize.  h / d: host and device
+// addresses of the word; seq: the last number enqueued.
+struct StreamSignal {
+    uint64_t* h = nullptr;
+    uint64_t* d = nullptr;
+    uint64_t seq = 0;
+};
+int signal_init(StreamSignal& s);  // allocates the word once
+void signal_free(StreamSignal& s);
+// Enqueues the signal on st and waits for it: polls, then yields, and after
+// 2 ms falls back to hipStreamSynchronize (which also reports stream errors).
+int signal_wait(StreamSignal& s, hipStream_t st);
+
 // One proof-of-work search and its current window.
 constexpr uint32_t POW_MAX_LG_LANES = 22;
 struct PowJob {
@@ -52,9 +68,9 @@ void pow_job_init(PowJob& j, const uint8_t ch[32], uint64_t threshold, uint32_t 
 // of the mapped block-minimum buffer hmin / hmin_d).  found[k], nonce[k] get
 // job k's answer; a job without a hit moves to its next window.  slots holds
 // two sets of POW_BATCH_MAX device early-exit slots (~0 at start); `set`
-// alternates between them.  Waits for the stream.
+// alternates between them.  Waits for the stream (through sig when given).
 int pow_run(hipStream_t st, unsigned long long* slots, int& set, const uint64_t* hmin, uint64_t* hmin_d, size_t cap,
-            PowJob* const* jobs, int n, bool* found, uint64_t* nonce);
+            PowJob* const* jobs, int n, bool* found, uint64_t* nonce, StreamSignal* sig = nullptr);
 
 }  // namespace whir_gpu
 
@@ -126,7 +142,7 @@ struct whir_ctx {
     // hpart by a launcher called with device_sum = false (lazy 128-bit sums:
     // g <= 2048 canonical limbs stay below 2^75).
     int host_sums(unsigned g, int nv, gl::e3* out) {
-        HIPCHK(hipStreamSynchronize(stream));
+        if (int rc = wait()) return rc;
         for (int v = 0; v < nv; ++v) {
             unsigned __int128 acc[3] = {0, 0, 0};
             for (unsigned b = 0; b < g; ++b) {
@@ -150,6 +166,12 @@ struct whir_ctx {
     // its requests): the hook returns the nonce for (challenge, threshold).
     int (*pow_hook)(void* arg, const uint8_t* ch, uint64_t threshold, uint64_t* nonce) = nullptr;
     void* pow_hook_arg = nullptr;
+    // waits for all work enqueued on `stream` (through the mapped signal word)
+    whir_gpu::StreamSignal sig;
+    int wait() {
+        if (int rc = whir_gpu::signal_init(sig)) return rc;
+        return whir_gpu::signal_wait(sig, stream);
+    }
     uint64_t* hpart = nullptr;   // host address
     uint64_t* hpart_d = nullptr;  // device address of the same memory
 

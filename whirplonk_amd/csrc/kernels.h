@@ -118,6 +118,9 @@ struct PowBatchArgs {
 void pow_batch_launch(const PowBatchArgs& a, unsigned long long* d_best, unsigned long long* d_reset,
                       uint64_t* block_min, hipStream_t st);
 void fill_u64_launch(uint64_t* p, uint64_t n, uint64_t v, hipStream_t st);
+// One lane stores seq into *flag_d (mapped host memory) after all earlier work
+// of the stream, with a system-scope release.
+void signal_launch(uint64_t* flag_d, uint64_t seq, hipStream_t st);
 
 // Weight terms sum_t s_t w_t(i) over i < 2^nv, factored as hi_t[i >> L] * lo_t[i & (2^L-1)].
 // Descriptor of a term, TERM_WORDS u64: kind (0 eq, 1 power), s (Ext3),

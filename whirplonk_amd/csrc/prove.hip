@@ -285,6 +285,15 @@ __global__ __launch_bounds__(256) void rlc_combine_kernel(const uint64_t* __rest
     st3(out, i, acc);
 }
 
+// Completion signal: one lane makes every earlier write of the stream visible
+// system-wide and then stores the sequence number into mapped host memory.
+__global__ void signal_kernel(uint64_t* flag, uint64_t seq) {
+    if (threadIdx.x == 0) {
+        __threadfence_system();
+        __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 __global__ void fill_u64_kernel(uint64_t* p, uint64_t n, uint64_t v) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -297,6 +306,10 @@ unsigned blocks_of(uint64_t work) { return (unsigned)((work + 255) / 256); }
 void pow_batch_launch(const PowBatchArgs& a, unsigned long long* d_best, unsigned long long* d_reset,
                       uint64_t* block_min, hipStream_t st) {
     hipLaunchKernelGGL(pow_batch_kernel, dim3(a.total_blocks), dim3(256), 0, st, a, d_best, d_reset, block_min);
+}
+
+void signal_launch(uint64_t* flag_d, uint64_t seq, hipStream_t st) {
+    hipLaunchKernelGGL(signal_kernel, dim3(1), dim3(64), 0, st, flag_d, seq);
 }
 
 void fill_u64_launch(uint64_t* p, uint64_t n, uint64_t v, hipStream_t st) {

@@ -155,10 +155,7 @@ struct Phase {
 };
 enum { PH_COMMIT = 0, PH_OOD, PH_WEIGHTS, PH_SUMCHECK, PH_POW, PH_ROUND_COMMIT, PH_OPEN, PH_OTHER };
 
-int sync(whir_prover* P) {
-    HIPCHK(hipStreamSynchronize(P->st()));
-    return 0;
-}
+int sync(whir_prover* P) { return P->ctx->wait(); }
 
 // ------------------------------------------------------------------- PoW
 // PowConfig::prove: nothing at zero difficulty; else squeeze a 32-byte
@@ -170,7 +167,7 @@ int pow_device(whir_ctx* ctx, const uint8_t ch[32], uint64_t threshold, uint64_t
         if (scanned) *scanned += *nonce + 1;
         return 0;
     }
-    if ((rc = ctx->ensure_hpart())) return rc;
+    if ((rc = ctx->ensure_hpart()) || (rc = signal_init(ctx->sig))) return rc;
     if (!ctx->powbuf.p) {
         if ((rc = ctx->ensure(ctx->powbuf, 2 * POW_BATCH_MAX * 8))) return rc;
         fill_u64_launch((uint64_t*)ctx->powbuf.p, 2 * POW_BATCH_MAX, ~0ull, ctx->stream);
@@ -183,7 +180,7 @@ int pow_device(whir_ctx* ctx, const uint8_t ch[32], uint64_t threshold, uint64_t
         bool found = false;
         uint64_t n = 0;
         if ((rc = pow_run(ctx->stream, (unsigned long long*)ctx->powbuf.p, ctx->pow_slot, ctx->hpart, ctx->hpart_d,
-                          whir_ctx::HPART_U64, &jp, 1, &found, &n)))
+                          whir_ctx::HPART_U64, &jp, 1, &found, &n, &ctx->sig)))
             return rc;
         if (found) {
             if (scanned) *scanned += n + 1;  // the serial scan's work
@@ -391,7 +388,7 @@ int open_witness(whir_prover* P, const whir_host::IrsConfig& c, const DevWitness
     gather_open_launch(w.matrix, ru, back_d + ioff, k, back_d, w.nodes, back_d + ioff + k, nh,
                        (uint8_t*)(back_d + hoff), ctx->stream);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if ((rc = ctx->wait())) return rc;
     o.rows.assign(back, back + k * ru);
     P->ts.hint_vector(o.rows.data(), k * w.num_cols, w.ext ? 24 : 8);
     for (size_t h = 0; h < nh; ++h) P->ts.hint(back + hoff + 4 * h, 32);

@@ -9,12 +9,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ctx.h"
@@ -202,8 +204,49 @@ void whir_gpu::pow_job_init(PowJob& j, const uint8_t ch[32], uint64_t threshold,
     j.window = 1ull << lg_window;
 }
 
+int whir_gpu::signal_init(StreamSignal& s) {
+    if (s.h) return 0;
+    void* h = nullptr;
+    HIPCHK(hipHostMalloc(&h, 64, hipHostMallocMapped | hipHostMallocCoherent));
+    void* d = nullptr;
+    hipError_t e = hipHostGetDevicePointer(&d, h, 0);
+    if (e != hipSuccess) {
+        (void)hipHostFree(h);
+        return fail("hipHostGetDevicePointer", e);
+    }
+    s.h = (uint64_t*)h;
+    s.d = (uint64_t*)d;
+    s.seq = 0;
+    __atomic_store_n(s.h, 0, __ATOMIC_RELEASE);
+    return 0;
+}
+
+void whir_gpu::signal_free(StreamSignal& s) {
+    if (s.h) (void)hipHostFree(s.h);
+    s = StreamSignal{};
+}
+
+int whir_gpu::signal_wait(StreamSignal& s, hipStream_t st) {
+    const uint64_t want = ++s.seq;
+    signal_launch(s.d, want, st);
+    HIPCHK(hipGetLastError());
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t i = 0;; ++i) {
+        if (__atomic_load_n(s.h, __ATOMIC_ACQUIRE) >= want) return 0;
+        if (i < 2048) {
+            __builtin_ia32_pause();
+            continue;
+        }
+        if ((i & 31) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) break;
+        std::this_thread::yield();
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    if (__atomic_load_n(s.h, __ATOMIC_ACQUIRE) >= want) return 0;
+    return fail("completion signal not seen after the stream synchronized", hipErrorUnknown);
+}
+
 int whir_gpu::pow_run(hipStream_t st, unsigned long long* slots, int& set, const uint64_t* hmin, uint64_t* hmin_d,
-                      size_t cap, PowJob* const* jobs, int n, bool* found, uint64_t* nonce) {
+                      size_t cap, PowJob* const* jobs, int n, bool* found, uint64_t* nonce, StreamSignal* sig) {
     if (n < 1 || n > (int)POW_BATCH_MAX) return bad_arg("bad PoW batch");
     PowBatchArgs a{};
     uint32_t b0 = 0;
@@ -224,7 +267,11 @@ int whir_gpu::pow_run(hipStream_t st, unsigned long long* slots, int& set, const
     set ^= 1;
     pow_batch_launch(a, slots + k * POW_BATCH_MAX, slots + (k ^ 1) * POW_BATCH_MAX, hmin_d, st);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(st));
+    if (sig) {
+        if (int rc = signal_wait(*sig, st)) return rc;
+    } else {
+        HIPCHK(hipStreamSynchronize(st));
+    }
     for (int q = 0; q < n; ++q) {
         PowJob& j = *jobs[q];
         uint64_t best = ~0ull;  // the block minima, in mapped host memory
@@ -258,7 +305,7 @@ int whir_gpu::commit_device(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_
     HIPCHK(hipGetLastError());
     if ((rc = ctx->mark(3))) return rc;
     if (mapped_root) {
-        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if ((rc = ctx->wait())) return rc;
         std::memcpy(out_root, ctx->hpart, 32);
     } else if (out_root) {
         size_t nn = whir_merkle_num_nodes(cw);
@@ -334,6 +381,7 @@ int whir_ctx_destroy(whir_ctx* ctx) {
     for (auto* p : ctx->t1024)
         if (p) (void)hipFree(p);
     if (ctx->hpart) (void)hipHostFree(ctx->hpart);
+    whir_gpu::signal_free(ctx->sig);
     if (ctx->hio) (void)hipHostFree(ctx->hio);
     if (ctx->hdesc) (void)hipHostFree(ctx->hdesc);
     if (ctx->hdesc_ev) (void)hipEventDestroy(ctx->hdesc_ev);
