@@ -127,6 +127,7 @@ public:
             HIPCHK(hipHostMalloc(&h, cap_ * 8, hipHostMallocMapped | hipHostMallocCoherent));
             l.hmin = (uint64_t*)h;
             HIPCHK(hipHostGetDevicePointer((void**)&l.hmin_d, h, 0));
+            if (int rc = signal_init(l.sig)) return rc;
             HIPCHK(hipStreamSynchronize(l.st));
         }
         return 0;
@@ -136,6 +137,7 @@ public:
             if (l.st) (void)hipStreamSynchronize(l.st);
             if (l.slots) (void)hipFree(l.slots);
             if (l.hmin) (void)hipHostFree(l.hmin);
+            signal_free(l.sig);
             if (l.st) (void)hipStreamDestroy(l.st);
         }
     }
@@ -163,6 +165,7 @@ private:
         int set = 0;
         uint64_t* hmin = nullptr;
         uint64_t* hmin_d = nullptr;
+        StreamSignal sig;  // the leader waits for its launch through this word
         bool busy = false;
     };
 
@@ -194,7 +197,7 @@ private:
             bool found[POW_BATCH_MAX];
             uint64_t got[POW_BATCH_MAX];
             int rc = pow_run(lane->st, lane->slots, lane->set, lane->hmin, lane->hmin_d, cap_, jobs.data(),
-                             (int)jobs.size(), found, got);
+                             (int)jobs.size(), found, got, &lane->sig);
             std::string err = rc ? last_error() : std::string();
             lk.lock();
             lane->busy = false;
