@@ -220,6 +220,52 @@ __global__ __launch_bounds__(1024) void merkle_top_kernel(uint8_t* __restrict__ 
     if (root_copy && threadIdx.x == 0) store_hash(root_copy, lvl);
 }
 
+// The three levels below the top when the child level has 512 to TOP_WIDTH
+// nodes: workgroup w reduces children [256 w, 256 w + 256) to 32 parents, one
+// parent per quad of lanes, with each level staged in LDS and written to the
+// flat node array.  Spread over width / 256 CUs, these levels no longer queue
+// on the single workgroup of merkle_top_kernel, which then starts from at
+// most 256 children, all in quad mode.
+__global__ __launch_bounds__(512) void merkle_top3_kernel(uint8_t* __restrict__ nodes, uint64_t child_off,
+                                                          uint64_t child_width) {
+    __shared__ uint32_t lvl[256 * 8];
+    const uint64_t w = blockIdx.x;
+    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) {
+        const uint4* src = reinterpret_cast<const uint4*>(nodes + 32 * (child_off + 256 * w + i));
+        uint4 a = src[0], b = src[1];
+        uint32_t* d = lvl + 8 * i;
+        d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w;
+        d[4] = b.x; d[5] = b.y; d[6] = b.z; d[7] = b.w;
+    }
+    __syncthreads();
+    uint64_t off = child_off, width = child_width;
+    uint32_t seg = 256;  // this workgroup's nodes at the level being reduced
+    for (int lev = 0; lev < 3; ++lev) {
+        const uint64_t parent_off = off + width;
+        const uint32_t np = seg / 2;
+        const uint32_t i = threadIdx.x >> 2, k = threadIdx.x & 3;
+        uint32_t lo = 0, hi = 0;
+        if (i < np) {
+            uint32_t m[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) m[q] = lvl[16 * i + q];
+            b3::hash64_quad(m, k, lo, hi);
+            uint32_t* dst = reinterpret_cast<uint32_t*>(nodes + 32 * (parent_off + w * np + i));
+            dst[k] = lo;
+            dst[4 + k] = hi;
+        }
+        __syncthreads();  // every child read before any parent overwrites the level
+        if (i < np) {
+            lvl[8 * i + k] = lo;
+            lvl[8 * i + 4 + k] = hi;
+        }
+        __syncthreads();
+        off = parent_off;
+        width /= 2;
+        seg = np;
+    }
+}
+
 __global__ __launch_bounds__(256) void gather_hashes_kernel(const uint8_t* __restrict__ nodes,
                                                             const uint64_t* __restrict__ idx, uint64_t count,
                                                             uint8_t* __restrict__ out) {
@@ -296,6 +342,11 @@ void merkle_build_launch(uint8_t* nodes, uint64_t n_leaves, hipStream_t st, uint
                            off + width, parents);
         off += width;
         width = parents;
+    }
+    if (width > 256) {  // 512 <= width <= TOP_WIDTH
+        hipLaunchKernelGGL(merkle_top3_kernel, dim3((unsigned)(width / 256)), dim3(512), 0, st, nodes, off, width);
+        off += width + width / 2 + width / 4;
+        width /= 8;
     }
     if (width > 1)
         hipLaunchKernelGGL(merkle_top_kernel, dim3(1), dim3(TOP_WIDTH / 2), 0, st, nodes, off, width, root_copy);
