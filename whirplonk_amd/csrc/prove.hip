@@ -226,12 +226,12 @@ __global__ __launch_bounds__(256) void fold_compute_kernel(const uint64_t* a, in
             fa1.c0 = gl::add(fa1.c0, x1);
         } else {
             e3 x0 = ld3(a, i), x1 = ld3(a, i + q), x2 = ld3(a, i + 2 * q), x3 = ld3(a, i + 3 * q);
-            fa0 = gl::e3_add(x0, gl::e3_mul(gl::e3_sub(x2, x0), r));
-            fa1 = gl::e3_add(x1, gl::e3_mul(gl::e3_sub(x3, x1), r));
+            fa0 = gl::e3_add(x0, gl::e3_mul(r, gl::e3_sub(x2, x0)));
+            fa1 = gl::e3_add(x1, gl::e3_mul(r, gl::e3_sub(x3, x1)));
         }
         e3 y0 = ld3(b, i), y1 = ld3(b, i + q), y2 = ld3(b, i + 2 * q), y3 = ld3(b, i + 3 * q);
-        e3 fb0 = gl::e3_add(y0, gl::e3_mul(gl::e3_sub(y2, y0), r));
-        e3 fb1 = gl::e3_add(y1, gl::e3_mul(gl::e3_sub(y3, y1), r));
+        e3 fb0 = gl::e3_add(y0, gl::e3_mul(r, gl::e3_sub(y2, y0)));
+        e3 fb1 = gl::e3_add(y1, gl::e3_mul(r, gl::e3_sub(y3, y1)));
         st3(a_out, i, fa0);
         st3(a_out, i + q, fa1);
         st3(b_out, i, fb0);

@@ -79,10 +79,10 @@ __global__ __launch_bounds__(256) void sumcheck_round_kernel(const uint64_t* a, 
             e3 da = gl::e3_sub(a1, a0);
             gl::e3_mac(s0, a0, b0);
             gl::e3_mac(s2, da, db);
-            fa = gl::e3_add(a0, gl::e3_mul(da, r));
+            fa = gl::e3_add(a0, gl::e3_mul(r, da));
         }
         store_e3(a_out, i, fa);
-        store_e3(b_out, i, gl::e3_add(b0, gl::e3_mul(db, r)));
+        store_e3(b_out, i, gl::e3_add(b0, gl::e3_mul(r, db)));
     }
     e3 acc[2] = {gl::e3acc_reduce(s0), gl::e3acc_reduce(s2)};
     block_sum<2>(acc);
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(256) void sumcheck_fold_kernel(const uint64_t* __re
             v.c0 = gl::add(v.c0, lo);
         } else {
             e3 lo = load_e3(in, i), hi = load_e3(in, i + half);
-            v = gl::e3_add(lo, gl::e3_mul(gl::e3_sub(hi, lo), r));
+            v = gl::e3_add(lo, gl::e3_mul(r, gl::e3_sub(hi, lo)));
         }
         store_e3(out, i, v);
     }
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(256) void horner_kernel(const uint64_t* __restrict_
         uint64_t nq = (n - t + T - 1) / T;
         for (uint64_t q = nq; q-- > 0;) {
             uint64_t i = t + T * q;
-            acc[0] = gl::e3_mul(acc[0], Z);
+            acc[0] = gl::e3_mul(Z, acc[0]);  // the invariant factor first: its doublings hoist
             if (f_ext3) {
                 acc[0] = gl::e3_add(acc[0], load_e3(f, i));
             } else {
