@@ -63,7 +63,7 @@ struct PowJob {
     uint64_t threshold = 0, base = 0, window = 0;
     uint32_t blocks = 0, per_thread = 0;
 };
-void pow_job_init(PowJob& j, const uint8_t ch[32], uint64_t threshold, uint32_t extra_bits = 2);
+void pow_job_init(PowJob& j, const uint8_t ch[32], uint64_t threshold, uint32_t extra_bits = 1);
 // One launch over jobs[0..n) (n <= POW_BATCH_MAX, their blocks <= cap entries
 // of the mapped block-minimum buffer hmin / hmin_d).  found[k], nonce[k] get
 // job k's answer; a job without a hit moves to its next window.  slots holds

@@ -190,11 +190,12 @@ void whir_gpu::pow_job_init(PowJob& j, const uint8_t ch[32], uint64_t threshold,
     // difficulty d = 64 - log2(threshold), 2^8 to 2^26 nonces, one nonce per
     // lane up to 2^22 lanes (blocks dispatched after a lower hit is known
     // return at once).  One wide step finishes sooner than a narrower grid
-    // walking several nonces per lane.  With 2 extra bits (2 to 4 times the
-    // expected work) a window holds a hit with probability at least 1 - e^-2:
-    // the latency choice of a single prover.  With 1 (1 to 2 times) a search
-    // needs a second window more often but hashes about 40 % less: the
-    // throughput choice of the serving loop, whose GPU is shared.
+    // walking several nonces per lane.  Every block of a window this size is
+    // resident at once, so a launch hashes its whole window: with 1 extra bit
+    // (1 to 2 times the expected work) a search needs a second window with
+    // probability e^-1 to e^-2 and still finishes sooner on average than with
+    // 2 (2 to 4 times), both for a single prover (a proof at nv = 22 took 3 %
+    // less) and in the serving loop, whose GPU is shared.
     uint32_t lg_expect = 0;
     while (lg_expect < 64 && (threshold >> (63 - lg_expect)) == 0) ++lg_expect;  // floor(d)
     const uint32_t lg_window = std::min<uint32_t>(26, std::max<uint32_t>(8, lg_expect + extra_bits));
