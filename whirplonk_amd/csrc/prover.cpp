@@ -566,10 +566,13 @@ int prove_impl(whir_prover* P, size_t num_forms, const uint64_t* points, const u
     // final vector, PoW, opening, sumcheck (whir_prover.hpp:355-398)
     if (cur != cfg.final_sumcheck.initial_size) return bad_arg("final vector size mismatch");
     {
-        std::vector<uint64_t> fin(3 * cur);
-        HIPCHK(hipMemcpyAsync(fin.data(), vec, fin.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        for (size_t i = 0; i < cur; ++i) P->ts.message(fin.data() + 3 * i, 24);
+        // into the pinned staging buffer (an asynchronous copy), then the
+        // completion word instead of a pageable copy and a stream sync
+        if ((rc = ctx->ensure_hio(24 * cur))) return rc;
+        HIPCHK(hipMemcpyAsync(ctx->hio, vec, 24 * cur, hipMemcpyDeviceToHost, ctx->stream));
+        if ((rc = ctx->wait())) return rc;
+        const uint64_t* fin = (const uint64_t*)ctx->hio;
+        for (size_t i = 0; i < cur; ++i) P->ts.message(fin + 3 * i, 24);
     }
     if ((rc = pow_prove(P, cfg.final_pow))) return rc;
     {
