@@ -171,13 +171,14 @@ def prove_inputs(nv, seed, nforms=3):
     return coeffs, pts.reshape(nforms, nv, 3)
 
 
-def prove_leg(steps: int, warmup: int, cpu: bool):
+def prove_leg(steps: int, warmup: int, cpu: bool, hash_id: int = 1):
     """Full WHIR prove at 2^22 (rate 1/2, folding factor 4, 3 MLE constraints),
-    the CLI's parameters and domain separator; inputs resident in HBM."""
+    the CLI's parameters and domain separator; inputs resident in HBM.
+    hash_id 2 is the CLI's --hash Sha2 (SHA-256 trees, leaves and PoW)."""
     import whirplonk_amd as W
 
     coeffs, pts = prove_inputs(PROVE_NV, PROVE_SEED)
-    params = W.Params.default(starting_log_inv_rate=1)
+    params = W.Params.default(starting_log_inv_rate=1, hash_id=hash_id)
     pid, sid = W.domain_separator_cli(params)
     ctx = W.Context(0)
     d = ctx.upload(coeffs)
@@ -200,7 +201,8 @@ def prove_leg(steps: int, warmup: int, cpu: bool):
     ctx.sync()
     ms = 1000.0 * (time.perf_counter() - t0) / steps
     ctx.close()
-    out = {"workload": "full WHIR prove nv=22 rate 1/2 folding_factor 4, 3 MLE constraints, CLI parameters",
+    out = {"workload": "full WHIR prove nv=22 rate 1/2 folding_factor 4, 3 MLE constraints, CLI parameters"
+                       + (", --hash Sha2" if hash_id == 2 else ""),
            "ms_per_proof": round(ms, 3), "proofs_per_s": round(1000.0 / ms, 2),
            "phase_ms": {k: round(v, 3) for k, v in phases.items()}, "pow_hashes": int(pow_h),
            "narg_bytes": len(narg), "hints_bytes": len(hints),
@@ -211,7 +213,7 @@ def prove_leg(steps: int, warmup: int, cpu: bool):
 
         if oracle.ref() is not None:
             t0 = time.perf_counter()
-            _, n2, h2 = oracle.ref_prove(PROVE_NV, coeffs, pts, rate=1)
+            _, n2, h2 = oracle.ref_prove(PROVE_NV, coeffs, pts, hash_id=hash_id, rate=1)
             out["cpu_reference_ms"] = round(1000.0 * (time.perf_counter() - t0), 1)
             out["cpu_threads"] = int(oracle.ref().ref_num_threads())
             out["bit_exact_vs_cpu"] = (n2 == narg and h2 == hints)
@@ -500,6 +502,8 @@ def main():
         "vs_baseline": None, "dtype": "u64", "data": "synthetic", "config": {"workload": "commit leg skipped"},
         "roofline": None, "cpu_baseline": None}
     out["full_prove"] = prove_leg(args.steps, args.warmup, not args.no_cpu) if "prove" in legs else None
+    out["full_prove_sha2"] = (prove_leg(args.steps, args.warmup, not args.no_cpu, hash_id=2)
+                              if "prove_sha2" in legs else None)
     out["batched"] = batched_leg(args.steps, not args.no_cpu) if "batched" in legs else None
     out["served"] = (served_leg(args.requests, args.offered, args.slots, args.budget_gb, not args.no_cpu)
                      if "served" in legs else None)

@@ -67,6 +67,12 @@ int whir_ctx_destroy(whir_ctx* ctx);
 int whir_ctx_set_stream(whir_ctx* ctx, void* hip_stream);
 void* whir_ctx_stream(whir_ctx* ctx);
 int whir_ctx_synchronize(whir_ctx* ctx);
+/* Hash engine of this context's commits, leaves, trees and PoW searches
+ * (whir_dev_commit_*, whir_dev_commit_leaves, whir_dev_merkle_build,
+ * whir_dev_pow_search): WHIR_HASH_BLAKE3 (the default) or WHIR_HASH_SHA2.  The
+ * reference selects it per tree with merkle_tree::make_config(EngineId, n)
+ * (irs_commit.hpp:62-73); provers take it from whir_params.hash_id. */
+int whir_ctx_set_hash(whir_ctx* ctx, uint32_t hash_id);
 
 /* Device memory helpers for callers without their own allocator. */
 int whir_dev_malloc(whir_ctx* ctx, size_t bytes, void** out);
@@ -114,6 +120,10 @@ int whir_gpu_interleaved_rs_encode_blake3_ext3_matrix_leaves(const uint64_t* coe
  * witness, (2^(L+1) - 1) * 32 bytes, L = ceil(log2 num_leaves). */
 int whir_gpu_blake3_merkle_tree(const uint8_t* leaves, size_t num_leaves, uint8_t* out_root, uint8_t* out_nodes);
 
+/* gpu_sha256_merkle_tree, cuda_integration.hpp:888-925: the same tree with
+ * parent = SHA-256(left || right). */
+int whir_gpu_sha256_merkle_tree(const uint8_t* leaves, size_t num_leaves, uint8_t* out_root, uint8_t* out_nodes);
+
 /* merkle_hint_node_indices, cuda_integration.hpp:854-885: node indices of the
  * open_path hints (merkle_tree.hpp:165-211) in order.  Returns the count;
  * writes them when out != NULL. */
@@ -123,6 +133,10 @@ size_t whir_merkle_hint_node_indices(size_t num_leaves, const uint64_t* indices,
  * sibling hints of open_path for the given (unsorted, possibly repeated) leaf
  * indices.  out_hints must hold whir_merkle_hint_node_indices(...) * 32 bytes. */
 int whir_gpu_blake3_merkle_open_path(const uint8_t* leaves, size_t num_leaves, const uint64_t* indices,
+                                     size_t num_indices, uint8_t* out_root, uint8_t* out_hints);
+/* gpu_sha256_merkle_open_path, cuda_integration.hpp:995-1045: the same over
+ * the SHA-256 tree. */
+int whir_gpu_sha256_merkle_open_path(const uint8_t* leaves, size_t num_leaves, const uint64_t* indices,
                                      size_t num_indices, uint8_t* out_root, uint8_t* out_hints);
 
 /* gpu_ood_evaluate_ext3_from_base, cuda_integration.hpp:1103-1149: for each
@@ -142,6 +156,9 @@ int whir_gpu_encode_to_bytes(const uint64_t* values, uint8_t* out, size_t count)
 /* Blake3::hash_many, hash/blake3_engine.hpp:90-199: out[i] = BLAKE3 of the
  * i-th `size`-byte message (any size, including 0 and > 1024). */
 int whir_gpu_blake3_hash_many(size_t size, const uint8_t* input, size_t count, uint8_t* output);
+/* Sha2::hash_many, hash/sha2_engine.hpp:62-79: out[i] = SHA-256 of the i-th
+ * `size`-byte message (any size, including 0). */
+int whir_gpu_sha256_hash_many(size_t size, const uint8_t* input, size_t count, uint8_t* output);
 
 /* compute_sumcheck_polynomial over Ext3, algebra/sumcheck.hpp:43-98 (no GPU
  * counterpart in the reference): a, b are n Ext3 AoS values (a_is_base: a is
@@ -203,10 +220,12 @@ int whir_dev_sumcheck_round(whir_ctx* ctx, const uint64_t* d_a, int a_is_base, c
 
 /* ------------------------------------------------------------ prove loop */
 
-/* ProtocolParameters (parameters.hpp:25-34).  hash_id selects the Merkle and
- * PoW hash engine; only WHIR_HASH_BLAKE3 (hash/blake3_engine.hpp:50) is
- * served. */
+/* ProtocolParameters (parameters.hpp:25-34).  hash_id selects the Merkle,
+ * row-leaf and PoW hash engine: WHIR_HASH_BLAKE3 (hash/blake3_engine.hpp:50,
+ * the CLI's default) or WHIR_HASH_SHA2 (hash/sha2_engine.hpp:39, the CLI's
+ * --hash Sha2). */
 #define WHIR_HASH_BLAKE3 1u
+#define WHIR_HASH_SHA2 2u
 typedef struct whir_params {
     int unique_decoding;
     uint32_t starting_log_inv_rate;

@@ -15,6 +15,7 @@ as uint8 arrays of 32 bytes per hash.
 from __future__ import annotations
 
 import ctypes
+import hashlib
 import os
 import subprocess
 
@@ -132,6 +133,15 @@ def ref():
                                             _u64p, _u8p, _sz, ctypes.POINTER(_sz), _u8p, _sz, ctypes.POINTER(_sz)]
         R.ref_cli_ds.restype = None
         R.ref_cli_ds.argtypes = [ctypes.POINTER(ctypes.c_uint32), _u8p, _u8p]
+        R.ref_whir_prove_hash.restype = ctypes.c_int
+        R.ref_whir_prove_hash.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, ctypes.c_uint32, _u64p,
+                                          _sz, _u64p, _u64p, _u8p, _sz, ctypes.POINTER(_sz), _u8p, _sz,
+                                          ctypes.POINTER(_sz)]
+        R.ref_cli_ds_hash.restype = None
+        R.ref_cli_ds_hash.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, _u8p, _u8p]
+        R.ref_sha256_many.argtypes = [_sz, _u8p, _sz, _u8p]
+        R.ref_merkle_build_sha2.argtypes = [_u8p, _sz, _u8p]
+        R.ref_commit_leaves_base_sha2.argtypes = [_u64p, _sz, _sz, _u8p]
         _ref = R
     return _ref
 
@@ -233,6 +243,43 @@ def commit_leaves_ext3(matrix, rows, cols) -> np.ndarray:
     out = np.zeros(32 * rows, dtype=np.uint8)
     lib().or_commit_leaves_ext3(_ptr64(m), rows, cols, _ptr8(out))
     return out
+
+
+# ------------------------------------------------------------ SHA-256 engine
+# The second hash engine (hash/sha2_engine.hpp:62-79: SHA-256 of each message's
+# bytes), restated with hashlib's SHA-256 (FIPS 180-4): row leaves over LE u64
+# limbs (matrix_commit.hpp:169-195), the zero-padded tree with parent =
+# SHA-256(left || right) (merkle_tree.hpp:104-147) and the PoW scan over
+# SHA-256(challenge || LE64(n) || 0^24) (proof_of_work.hpp:95-193).
+def sha256(data: bytes) -> bytes:
+    return hashlib.sha256(bytes(data)).digest()
+
+
+def sha256_leaves(matrix, rows: int, row_u64: int) -> np.ndarray:
+    m = u64(matrix).reshape(rows, row_u64)
+    return np.frombuffer(b"".join(sha256(m[r].tobytes()) for r in range(rows)), dtype=np.uint8).copy()
+
+
+def sha256_merkle_build(leaves: np.ndarray, n: int) -> np.ndarray:
+    width = 1
+    while width < n:
+        width <<= 1
+    level = [bytes(np.asarray(leaves, dtype=np.uint8)[32 * i:32 * i + 32]) for i in range(n)]
+    level += [bytes(32)] * (width - n)
+    out = list(level)
+    while len(level) > 1:
+        level = [sha256(level[2 * i] + level[2 * i + 1]) for i in range(len(level) // 2)]
+        out += level
+    return np.frombuffer(b"".join(out), dtype=np.uint8).copy()
+
+
+def sha256_pow_find(challenge: bytes, threshold: int) -> int:
+    n = 0
+    while True:
+        h = sha256(bytes(challenge) + n.to_bytes(8, "little") + bytes(24))
+        if int.from_bytes(h[:8], "little") <= threshold:
+            return n
+        n += 1
 
 
 def merkle_num_nodes(n: int) -> int:
@@ -383,14 +430,14 @@ def ref_config(nv: int, rate: int, ff: int, sec: int, pow_bits: int) -> np.ndarr
     return out
 
 
-def ref_cli_ds(**prm) -> tuple[bytes, bytes]:
+def ref_cli_ds(hash_id: int = 1, **prm) -> tuple[bytes, bytes]:
     pid = np.zeros(64, dtype=np.uint8)
     sid = np.zeros(32, dtype=np.uint8)
-    ref().ref_cli_ds(_prm(**prm), _ptr8(pid), _ptr8(sid))
+    ref().ref_cli_ds_hash(_prm(**prm), hash_id, _ptr8(pid), _ptr8(sid))
     return pid.tobytes(), sid.tobytes()
 
 
-def ref_prove(nv: int, coeffs, points, **prm):
+def ref_prove(nv: int, coeffs, points, hash_id: int = 1, **prm):
     """The reference's own prover (compiled from its sources): the CLI's PCS
     flow with caller-given coefficients (batch x 2^nv) and MLE points
     (num_constraints x nv x 3).  Returns (evaluations, narg, hints)."""
@@ -405,8 +452,9 @@ def ref_prove(nv: int, coeffs, points, **prm):
         narg = np.zeros(cap_n, dtype=np.uint8)
         hints = np.zeros(cap_h, dtype=np.uint8)
         nl, hl = _sz(), _sz()
-        rc = R.ref_whir_prove_params(_prm(**prm), nv, _ptr64(c), nf, _ptr64(pts) if nf else None, _ptr64(ev),
-                                     _ptr8(narg), cap_n, ctypes.byref(nl), _ptr8(hints), cap_h, ctypes.byref(hl))
+        rc = R.ref_whir_prove_hash(_prm(**prm), hash_id, nv, _ptr64(c), nf, _ptr64(pts) if nf else None,
+                                   _ptr64(ev), _ptr8(narg), cap_n, ctypes.byref(nl), _ptr8(hints), cap_h,
+                                   ctypes.byref(hl))
         if rc == 0:
             return ev[:3 * nf * batch].reshape(-1, 3), narg[:nl.value].tobytes(), hints[:hl.value].tobytes()
         cap_n, cap_h = max(cap_n, nl.value), max(cap_h, hl.value)

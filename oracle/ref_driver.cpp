@@ -17,6 +17,9 @@
 //   ref_commit_base        the irs_commit.hpp:344-460 sequence (encode, leaves, tree)
 //                          without the transcript
 //   ref_blake3_many        hash/blake3_engine.hpp:90-199 Blake3::hash_many (both branches)
+//   ref_sha256_many        hash/sha2_engine.hpp:62-79 Sha2::hash_many
+//   ref_*_sha2             the SHA-256 tree and row leaves (ENGINE_ID_SHA2)
+//   ref_whir_prove_hash, ref_cli_ds_hash  the PCS flow and DS with --hash Sha2
 //   ref_whir_config        protocols/whir/whir.hpp:118-224 Config::from_params, flattened
 //   ref_whir_prove(_params) the main.cpp:292-424 PCS flow: from_params, the CLI's domain
 //                          separator (main.cpp:176-215), commit, MLE constraints, prove
@@ -28,6 +31,7 @@
 #include "whir/algebra/sumcheck.hpp"
 #include "whir/algebra/utilities.hpp"
 #include "whir/hash/blake3_engine.hpp"
+#include "whir/hash/sha2_engine.hpp"
 #include "whir/protocols/matrix_commit.hpp"
 #include "whir/protocols/merkle_tree.hpp"
 #include "whir/protocols/whir/whir.hpp"
@@ -73,6 +77,10 @@ void from_e3(const GoldilocksExt3& x, uint64_t* o) {
 const whir::hash::Blake3& blake3() {
     static const whir::hash::Blake3 b;
     return b;
+}
+const whir::hash::Sha2& sha2() {
+    static const whir::hash::Sha2 s;
+    return s;
 }
 }  // namespace
 
@@ -140,6 +148,33 @@ void ref_merkle_build(const uint8_t* leaves, size_t n, uint8_t* nodes) {
     auto w = mt::build_tree(cfg, std::move(lv),
                             [](whir::EngineId) -> const whir::hash::HashEngine& { return blake3(); });
     std::memcpy(nodes, w.nodes.data(), w.nodes.size() * 32);
+}
+
+// Sha2::hash_many (hash/sha2_engine.hpp:62-79)
+void ref_sha256_many(size_t size, const uint8_t* in, size_t count, uint8_t* out) {
+    std::vector<Hash> o(count);
+    sha2().hash_many(size, std::span<const uint8_t>{in, size * count}, std::span<Hash>{o});
+    if (count) std::memcpy(out, o.data(), count * 32);
+}
+
+// the SHA-256 tree (merkle_tree.hpp:104-147 with ENGINE_ID_SHA2)
+void ref_merkle_build_sha2(const uint8_t* leaves, size_t n, uint8_t* nodes) {
+    namespace mt = whir::protocols::merkle_tree;
+    auto cfg = mt::make_config(whir::hash::ENGINE_ID_SHA2, n);
+    std::vector<Hash> lv(n);
+    if (n) std::memcpy(lv.data(), leaves, n * 32);
+    auto w = mt::build_tree(cfg, std::move(lv),
+                            [](whir::EngineId) -> const whir::hash::HashEngine& { return sha2(); });
+    std::memcpy(nodes, w.nodes.data(), w.nodes.size() * 32);
+}
+
+// SHA-256 row leaves of a base-field matrix (matrix_commit.hpp:169-195)
+void ref_commit_leaves_base_sha2(const uint64_t* matrix, size_t rows, size_t cols, uint8_t* leaves) {
+    std::vector<Hash> out(rows);
+    whir::protocols::matrix_commit::commit_leaves<Goldilocks>(
+        sha2(), std::span<const Goldilocks>{reinterpret_cast<const Goldilocks*>(matrix), rows * cols}, cols,
+        std::span<Hash>{out});
+    std::memcpy(leaves, out.data(), rows * 32);
 }
 
 size_t ref_merkle_open(const uint8_t* nodes, size_t n_leaves, const uint64_t* idx, size_t n_idx,
@@ -314,10 +349,10 @@ size_t ref_whir_config(uint32_t num_variables, uint32_t log_inv_rate, uint32_t f
 // folding_factor, security_level, pow_bits, batch_size}; coeffs holds batch_size vectors
 // of 2^num_variables base-field coefficients; out_evals gets num_constraints x batch_size
 // Ext3 values (constraint-major).  The domain separator is the CLI's for these parameters.
-int ref_whir_prove_params(const uint32_t* prm, uint32_t num_variables, const uint64_t* coeffs,
-                          size_t num_constraints, const uint64_t* points, uint64_t* out_evals, uint8_t* narg,
-                          size_t narg_cap, size_t* narg_len, uint8_t* hints, size_t hints_cap,
-                          size_t* hints_len) {
+// hash_id: 1 BLAKE3, 2 SHA-256 (the CLI's --hash Blake3 / Sha2)
+int ref_whir_prove_hash(const uint32_t* prm, uint32_t hash_id, uint32_t num_variables, const uint64_t* coeffs,
+                        size_t num_constraints, const uint64_t* points, uint64_t* out_evals, uint8_t* narg,
+                        size_t narg_cap, size_t* narg_len, uint8_t* hints, size_t hints_cap, size_t* hints_len) {
     namespace tx = whir::transcript;
     namespace alg = whir::algebra;
     whir::ProtocolParameters p;
@@ -328,7 +363,7 @@ int ref_whir_prove_params(const uint32_t* prm, uint32_t num_variables, const uin
     p.security_level = prm[4];
     p.pow_bits = prm[5];
     p.batch_size = prm[6];
-    p.hash_id = whir::hash::ENGINE_ID_BLAKE3;
+    p.hash_id = hash_id == 2 ? whir::hash::ENGINE_ID_SHA2 : whir::hash::ENGINE_ID_BLAKE3;
     const std::size_t size = std::size_t{1} << num_variables;
     const std::size_t nvec = p.batch_size;
     auto config = whirp::Config<Emb>::from_params(size, p);
@@ -372,8 +407,16 @@ int ref_whir_prove_params(const uint32_t* prm, uint32_t num_variables, const uin
     return 0;
 }
 
-// The CLI domain separator for prm (layout as above): protocol_id[64], session_id[32].
-void ref_cli_ds(const uint32_t* prm, uint8_t* protocol_id, uint8_t* session_id) {
+int ref_whir_prove_params(const uint32_t* prm, uint32_t num_variables, const uint64_t* coeffs,
+                          size_t num_constraints, const uint64_t* points, uint64_t* out_evals, uint8_t* narg,
+                          size_t narg_cap, size_t* narg_len, uint8_t* hints, size_t hints_cap,
+                          size_t* hints_len) {
+    return ref_whir_prove_hash(prm, 1, num_variables, coeffs, num_constraints, points, out_evals, narg, narg_cap,
+                               narg_len, hints, hints_cap, hints_len);
+}
+
+// The CLI domain separator for prm (layout as above) and hash_id: protocol_id[64], session_id[32].
+void ref_cli_ds_hash(const uint32_t* prm, uint32_t hash_id, uint8_t* protocol_id, uint8_t* session_id) {
     whir::ProtocolParameters p;
     p.unique_decoding = prm[0] != 0;
     p.starting_log_inv_rate = prm[1];
@@ -382,10 +425,14 @@ void ref_cli_ds(const uint32_t* prm, uint8_t* protocol_id, uint8_t* session_id) 
     p.security_level = prm[4];
     p.pow_bits = prm[5];
     p.batch_size = prm[6];
-    p.hash_id = whir::hash::ENGINE_ID_BLAKE3;
+    p.hash_id = hash_id == 2 ? whir::hash::ENGINE_ID_SHA2 : whir::hash::ENGINE_ID_BLAKE3;
     auto ds = cli_ds(p);
     std::memcpy(protocol_id, ds.protocol_id.data(), 64);
     std::memcpy(session_id, ds.session_id.data(), 32);
+}
+
+void ref_cli_ds(const uint32_t* prm, uint8_t* protocol_id, uint8_t* session_id) {
+    ref_cli_ds_hash(prm, 1, protocol_id, session_id);
 }
 
 int ref_whir_prove(uint32_t num_variables, uint32_t log_inv_rate, uint32_t folding_factor,

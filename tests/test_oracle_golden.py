@@ -4,6 +4,7 @@ Fixtures come from tests/golden/fixtures.json (made by make_fixtures.py from
 the reference's cpp/dump_output goldens).
 """
 import numpy as np
+import pytest
 
 P = 0xFFFFFFFF00000001
 
@@ -69,6 +70,57 @@ def test_merkle_golden(orc, fixtures):
         assert [hints[32 * i:32 * i + 32].tobytes().hex() for i in range(len(hints) // 32)] == c["hints"]
         lh = np.concatenate([leaves[32 * i:32 * i + 32] for i in c["indices"]])
         assert orc.merkle_verify(bytes.fromhex(c["root"]), n, c["indices"], lh, hints)
+
+
+def test_sha256_golden(orc, fixtures):
+    """The SHA-256 engine's digests (dump_hash.cpp, engine "sha2")."""
+    n = 0
+    for c in fixtures["hash"]:
+        if c["engine"] != "sha2":
+            continue
+        data = bytes.fromhex(c["input"])
+        for i in range(c["count"]):
+            assert orc.sha256(data[i * c["size"]:(i + 1) * c["size"]]).hex() == c["digests"][i]
+            n += 1
+    assert n > 0
+
+
+def test_sha256_merkle_golden(orc, fixtures):
+    """The SHA-256 tree (dump_merkle_tree.cpp "sha2-8-2"): root and open_path hints."""
+    cases = [c for c in fixtures["merkle_tree"] if c["hash"] == "sha2"]
+    assert cases
+    for c in cases:
+        n = c["num_leaves"]
+        leaves = np.frombuffer(bytes.fromhex(c["leaves"]), dtype=np.uint8)
+        nodes = orc.sha256_merkle_build(leaves, n)
+        assert nodes[-32:].tobytes().hex() == c["root"]
+        hints = orc.merkle_open(nodes, n, c["indices"])
+        assert [hints[32 * i:32 * i + 32].tobytes().hex() for i in range(len(hints) // 32)] == c["hints"]
+
+
+def test_sha256_oracle_vs_reference(orc):
+    """Row leaves, trees (ragged sizes, zero padding) and hash_many against the
+    reference's own Sha2 engine built from its sources."""
+    R = orc.ref()
+    if R is None:
+        pytest.skip("reference oracle not built")
+    rng = np.random.default_rng(7)
+    for rows, cols in ((1, 1), (3, 7), (5, 8), (4, 9), (2, 16), (6, 15)):
+        m = rng.integers(0, orc.P, size=rows * cols, dtype=np.uint64)
+        ref = np.zeros(32 * rows, dtype=np.uint8)
+        R.ref_commit_leaves_base_sha2(orc._ptr64(m), rows, cols, orc._ptr8(ref))
+        assert (orc.sha256_leaves(m, rows, cols) == ref).all()
+    for n in (1, 2, 3, 7, 17, 64):
+        lv = rng.integers(0, 256, size=32 * n, dtype=np.uint8)
+        ref = np.zeros(32 * orc.merkle_num_nodes(n), dtype=np.uint8)
+        R.ref_merkle_build_sha2(orc._ptr8(lv), n, orc._ptr8(ref))
+        assert (orc.sha256_merkle_build(lv, n) == ref).all()
+    for size in (0, 1, 55, 56, 63, 64, 65, 119, 120, 200):
+        data = rng.integers(0, 256, size=max(3 * size, 1), dtype=np.uint8)
+        ref = np.zeros(96, dtype=np.uint8)
+        R.ref_sha256_many(size, orc._ptr8(data), 3, orc._ptr8(ref))
+        want = b"".join(orc.sha256(data[i * size:(i + 1) * size].tobytes()) for i in range(3))
+        assert ref.tobytes() == want
 
 
 def test_univariate_evaluate_golden(orc, fixtures):

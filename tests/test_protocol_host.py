@@ -258,3 +258,12 @@ def test_product_domain_separator_vs_reference(W, orc):
                              security_level=kw.get("sec", 128), pow_bits=kw.get("pow_bits", 20),
                              batch_size=kw.get("batch", 1))
         assert W.domain_separator_cli(p) == (pid, sid)
+
+
+def test_product_domain_separator_sha2_vs_reference(W, orc):
+    """--hash Sha2 changes the engine id hashed into the protocol id."""
+    for kw in ({}, {"rate": 2, "pow_bits": 16}):
+        pid, sid = orc.ref_cli_ds(hash_id=2, **kw)
+        assert pid != orc.ref_cli_ds(hash_id=1, **kw)[0]
+        p = W.Params.default(starting_log_inv_rate=kw.get("rate", 1), pow_bits=kw.get("pow_bits", 20), hash_id=2)
+        assert W.domain_separator_cli(p) == (pid, sid)

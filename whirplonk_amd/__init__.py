@@ -51,6 +51,7 @@ _SIGS = [
     ("whir_ctx_create", _int, [_int, ctypes.POINTER(_vp)]),
     ("whir_ctx_destroy", _int, [_vp]),
     ("whir_ctx_set_stream", _int, [_vp, _vp]),
+    ("whir_ctx_set_hash", _int, [_vp, ctypes.c_uint32]),
     ("whir_ctx_stream", _vp, [_vp]),
     ("whir_ctx_synchronize", _int, [_vp]),
     ("whir_dev_malloc", _int, [_vp, _sz, ctypes.POINTER(_vp)]),
@@ -65,12 +66,15 @@ _SIGS = [
     ("whir_gpu_interleaved_rs_encode_blake3_ext3_matrix_leaves", _int,
      [_u64p, _u64p, _u64p, _u64p, _u64p, _u64p, _u8p, _sz, _sz, _sz, _sz]),
     ("whir_gpu_blake3_merkle_tree", _int, [_u8p, _sz, _u8p, _u8p]),
+    ("whir_gpu_sha256_merkle_tree", _int, [_u8p, _sz, _u8p, _u8p]),
     ("whir_merkle_hint_node_indices", _sz, [_sz, _u64p, _sz, _u64p]),
     ("whir_gpu_blake3_merkle_open_path", _int, [_u8p, _sz, _u64p, _sz, _u8p, _u8p]),
+    ("whir_gpu_sha256_merkle_open_path", _int, [_u8p, _sz, _u64p, _sz, _u8p, _u8p]),
     ("whir_gpu_ood_evaluate_ext3_from_base", _int, [_u64p, _u64p, _u64p, _sz, _sz, _sz]),
     ("whir_gpu_ood_evaluate_ext3_from_ext3", _int, [_u64p, _u64p, _u64p, _u64p, _u64p, _sz, _sz, _sz]),
     ("whir_gpu_encode_to_bytes", _int, [_u64p, _u8p, _sz]),
     ("whir_gpu_blake3_hash_many", _int, [_sz, _u8p, _sz, _u8p]),
+    ("whir_gpu_sha256_hash_many", _int, [_sz, _u8p, _sz, _u8p]),
     ("whir_gpu_sumcheck_compute", _int, [_u64p, _int, _u64p, _sz, _u64p, _u64p]),
     ("whir_gpu_sumcheck_fold", _int, [_u64p, _int, _sz, _u64p, _u64p]),
     ("whir_dev_rs_encode", _int, [_vp, _vp, _sz, _sz, _sz, _sz, _vp]),
@@ -227,13 +231,18 @@ def rs_encode_leaves_ext3(coeffs_aos, num_polys, poly_size, codeword_length, dep
     return aos, leaves
 
 
-def merkle_tree(leaves: np.ndarray, num_leaves: int, full: bool = True):
+HASH_BLAKE3 = 1
+HASH_SHA2 = 2
+
+
+def merkle_tree(leaves: np.ndarray, num_leaves: int, full: bool = True, hash: int = HASH_BLAKE3):
     lv = np.ascontiguousarray(leaves, dtype=np.uint8)
     if lv.size == 0:
         lv = np.zeros(32, dtype=np.uint8)
     root = np.empty(32, dtype=np.uint8)
     nodes = np.empty(32 * load().whir_merkle_num_nodes(num_leaves), dtype=np.uint8) if full else None
-    _check(load().whir_gpu_blake3_merkle_tree(_p8(lv), num_leaves, _p8(root), _p8(nodes) if full else None))
+    fn = load().whir_gpu_sha256_merkle_tree if hash == HASH_SHA2 else load().whir_gpu_blake3_merkle_tree
+    _check(fn(_p8(lv), num_leaves, _p8(root), _p8(nodes) if full else None))
     return root, nodes
 
 
@@ -245,13 +254,14 @@ def merkle_hint_node_indices(num_leaves: int, indices) -> np.ndarray:
     return out[:n]
 
 
-def merkle_open_path(leaves: np.ndarray, num_leaves: int, indices):
+def merkle_open_path(leaves: np.ndarray, num_leaves: int, indices, hash: int = HASH_BLAKE3):
     lv = np.ascontiguousarray(leaves, dtype=np.uint8)
     ix = _u64(indices) if len(indices) else np.zeros(1, dtype=np.uint64)
     nh = len(merkle_hint_node_indices(num_leaves, indices))
     root = np.empty(32, dtype=np.uint8)
     hints = np.empty(max(32 * nh, 32), dtype=np.uint8)
-    _check(load().whir_gpu_blake3_merkle_open_path(_p8(lv), num_leaves, _p64(ix), len(indices), _p8(root), _p8(hints)))
+    fn = load().whir_gpu_sha256_merkle_open_path if hash == HASH_SHA2 else load().whir_gpu_blake3_merkle_open_path
+    _check(fn(_p8(lv), num_leaves, _p64(ix), len(indices), _p8(root), _p8(hints)))
     return root, hints[: 32 * nh]
 
 
@@ -277,6 +287,13 @@ def blake3_hash_many(size: int, data: bytes, count: int) -> np.ndarray:
     buf = np.frombuffer(bytes(data) + b"\0" * 16, dtype=np.uint8)
     out = np.empty(max(32 * count, 32), dtype=np.uint8)
     _check(load().whir_gpu_blake3_hash_many(size, _p8(buf), count, _p8(out)))
+    return out[: 32 * count]
+
+
+def sha256_hash_many(size: int, data: bytes, count: int) -> np.ndarray:
+    buf = np.frombuffer(bytes(data) + b"\0" * 16, dtype=np.uint8)
+    out = np.empty(max(32 * count, 32), dtype=np.uint8)
+    _check(load().whir_gpu_sha256_hash_many(size, _p8(buf), count, _p8(out)))
     return out[: 32 * count]
 
 
@@ -345,6 +362,10 @@ class Context:
 
     def copy(self, dst: int, src: int, nbytes: int):
         _check(load().whir_dev_copy(self.h, dst, src, nbytes))
+
+    def set_hash(self, hash: int):
+        """Hash engine of this context's trees, row leaves and PoW (HASH_BLAKE3 or HASH_SHA2)."""
+        _check(load().whir_ctx_set_hash(self.h, hash))
 
     def sync(self):
         _check(load().whir_ctx_synchronize(self.h))

@@ -62,6 +62,7 @@ struct PowJob {
     uint8_t ch[32];
     uint64_t threshold = 0, base = 0, window = 0;
     uint32_t blocks = 0, per_thread = 0;
+    uint32_t hash = HASH_BLAKE3;  // engine of the nonce hash; one per launch
 };
 void pow_job_init(PowJob& j, const uint8_t ch[32], uint64_t threshold, uint32_t extra_bits = 1);
 // One launch over jobs[0..n) (n <= POW_BATCH_MAX, their blocks <= cap entries
@@ -89,6 +90,9 @@ struct whir_ctx {
     // prove loop: PoW words, term descriptors and tables, opened rows, vector, covector
     whir_gpu::DevBuf powbuf, tdesc, ttab;
     int pow_slot = 0;  // early-exit slot set of the next PoW launch
+    // hash engine of the commits, trees and PoW searches run on this context
+    // (HASH_BLAKE3 or HASH_SHA2; whir_ctx_set_hash)
+    uint32_t hash = whir_gpu::HASH_BLAKE3;
     // when set, LDE passes use this caller-owned scratch instead of `scratch`
     void* ext_scratch = nullptr;
     size_t ext_scratch_bytes = 0;

@@ -61,6 +61,13 @@ void hash_many_launch(const uint8_t* in, uint64_t size, uint64_t count, uint8_t*
 // Build the flat tree in nodes (leaves already in nodes[0..n), padded to 2^L).
 // With n_leaves >= 2 and root_copy set, the root is also written there.
 void merkle_build_launch(uint8_t* nodes, uint64_t n_leaves, hipStream_t st, uint8_t* root_copy = nullptr);
+// The SHA-256 engine (hash_id 2): the same three operations with SHA-256 of
+// the message bytes in place of BLAKE3 (hash/sha2_engine.hpp:47-81).
+enum : uint32_t { HASH_BLAKE3 = 1, HASH_SHA2 = 2 };
+void sha256_leaves_launch(const uint64_t* matrix, uint64_t rows, uint32_t row_u64, uint8_t* leaves,
+                          hipStream_t st);
+void sha256_hash_many_launch(const uint8_t* in, uint64_t size, uint64_t count, uint8_t* out, hipStream_t st);
+void sha256_merkle_build_launch(uint8_t* nodes, uint64_t n_leaves, hipStream_t st, uint8_t* root_copy = nullptr);
 // out[i] = nodes[idx[i]] (32 bytes each)
 void gather_hashes_launch(const uint8_t* nodes, const uint64_t* idx, uint64_t count, uint8_t* out, hipStream_t st);
 void fill_zero_launch(uint8_t* p, uint64_t bytes, hipStream_t st);
@@ -114,6 +121,7 @@ struct PowJobDesc {
 struct PowBatchArgs {
     PowJobDesc job[POW_BATCH_MAX];
     uint32_t njobs, total_blocks;
+    uint32_t hash;  // HASH_BLAKE3 or HASH_SHA2, for every job of the launch
 };
 void pow_batch_launch(const PowBatchArgs& a, unsigned long long* d_best, unsigned long long* d_reset,
                       uint64_t* block_min, hipStream_t st);

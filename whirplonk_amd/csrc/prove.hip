@@ -21,6 +21,7 @@
 //  * rlc_combine: vector = f_0 + sum_k c_k f_k lifted to Ext3
 //    (lift / mixed_scalar_mul_add, utilities.hpp:152-181).
 #include "blake3_dev.h"
+#include "sha256_dev.h"
 #include "gl.h"
 #include "kernels.h"
 #include "reduce_dev.h"
@@ -49,6 +50,7 @@ using red::store_block_totals;
 // (device) is job k's running minimum hit, which lets lanes and whole blocks
 // stop early; each block also writes the minimum of its own hits (or ~0) to
 // block_min[blockIdx.x], and the host takes the minimum over a job's blocks.
+template <uint32_t HASH>
 __global__ __launch_bounds__(256) void pow_batch_kernel(PowBatchArgs a, unsigned long long* best,
                                                         unsigned long long* reset, uint64_t* block_min) {
     __shared__ unsigned long long smin;
@@ -97,8 +99,11 @@ __global__ __launch_bounds__(256) void pow_batch_kernel(PowBatchArgs a, unsigned
         if (k && nonce > __hip_atomic_load(bj, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
         m[8] = (uint32_t)nonce;
         m[9] = (uint32_t)(nonce >> 32);
-        uint32_t out[8];
-        b3::compress(cv, m, 0, 64, b3::CHUNK_START | b3::CHUNK_END | b3::ROOT, out);
+        uint32_t out[8];  // the digest's bytes, little-endian words
+        if constexpr (HASH == HASH_SHA2)
+            s256::hash64(m, out);
+        else
+            b3::compress(cv, m, 0, 64, b3::CHUNK_START | b3::CHUNK_END | b3::ROOT, out);
         uint64_t h = ((uint64_t)out[1] << 32) | out[0];
         if (h <= J.threshold) {
             atomicMin(bj, (unsigned long long)nonce);
@@ -305,7 +310,12 @@ unsigned blocks_of(uint64_t work) { return (unsigned)((work + 255) / 256); }
 
 void pow_batch_launch(const PowBatchArgs& a, unsigned long long* d_best, unsigned long long* d_reset,
                       uint64_t* block_min, hipStream_t st) {
-    hipLaunchKernelGGL(pow_batch_kernel, dim3(a.total_blocks), dim3(256), 0, st, a, d_best, d_reset, block_min);
+    if (a.hash == HASH_SHA2)
+        hipLaunchKernelGGL(pow_batch_kernel<HASH_SHA2>, dim3(a.total_blocks), dim3(256), 0, st, a, d_best, d_reset,
+                           block_min);
+    else
+        hipLaunchKernelGGL(pow_batch_kernel<HASH_BLAKE3>, dim3(a.total_blocks), dim3(256), 0, st, a, d_best, d_reset,
+                           block_min);
 }
 
 void signal_launch(uint64_t* flag_d, uint64_t seq, hipStream_t st) {

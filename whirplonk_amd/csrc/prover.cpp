@@ -104,6 +104,18 @@ Layout plan_layout(const whir_host::WhirConfig& cfg, size_t size) {
 const uint8_t kBlake3EngineId[32] = {0x03, 0xe0, 0x17, 0x49, 0xeb, 0xcc, 0x04, 0x77, 0x92, 0x42, 0x54,
                                      0xeb, 0x48, 0x20, 0x66, 0xb8, 0x64, 0xa8, 0xdd, 0x4d, 0x77, 0x25,
                                      0x24, 0x64, 0xca, 0x6f, 0x5b, 0x6f, 0x5c, 0xc0, 0x5b, 0x4c};
+// EngineId of the SHA-256 engine (hash/sha2_engine.hpp:39-44), likewise fixed.
+const uint8_t kSha2EngineId[32] = {0x01, 0x8e, 0xaa, 0x24, 0x7c, 0xb8, 0x95, 0x7a, 0xb1, 0xe9, 0xfd,
+                                   0xac, 0x88, 0x54, 0x50, 0x40, 0x3c, 0x5e, 0x7b, 0xda, 0x1a, 0xca,
+                                   0xaa, 0x50, 0x4e, 0x4c, 0xc8, 0xf2, 0xf7, 0x6b, 0xb0, 0x76};
+
+// The prover's hash engine on its context for the duration of a call.
+struct HashScope {
+    whir_ctx* ctx;
+    uint32_t prev;
+    HashScope(whir_ctx* c, uint32_t h) : ctx(c), prev(c->hash) { c->hash = h; }
+    ~HashScope() { ctx->hash = prev; }
+};
 
 }  // namespace
 
@@ -123,6 +135,7 @@ struct whir_prover {
     bool committed = false, proved = false;
     std::vector<e3> eval_point, forms_rlc;
     size_t pow_hashes = 0;
+    uint32_t hash = HASH_BLAKE3;  // engine of the trees and PoW (params.hash_id)
     // host wall time per phase (include the waits on the device)
     double phase_ms[WHIR_PROVER_PHASES] = {};
     int phase = WHIR_PROVER_PHASES - 1;
@@ -175,6 +188,7 @@ int pow_device(whir_ctx* ctx, const uint8_t ch[32], uint64_t threshold, uint64_t
     }
     PowJob job;
     pow_job_init(job, ch, threshold);
+    job.hash = ctx->hash;
     PowJob* jp = &job;
     for (;;) {
         bool found = false;
@@ -604,7 +618,7 @@ whir_host::Params to_params(const whir_params* p) {
 
 int check_params(const whir_params* p, uint64_t size) {
     if (!p) return bad_arg("null parameters");
-    if (p->hash_id != WHIR_HASH_BLAKE3) return bad_arg("only the BLAKE3 engine is supported");
+    if (p->hash_id != WHIR_HASH_BLAKE3 && p->hash_id != WHIR_HASH_SHA2) return bad_arg("unknown hash engine");
     if (!is_pow2(size) || size < 2) return bad_arg("size must be a power of two >= 2");
     if (p->batch_size == 0) return bad_arg("batch_size must be >= 1");
     if (p->folding_factor == 0 || p->initial_folding_factor == 0) return bad_arg("folding factors must be >= 1");
@@ -665,8 +679,8 @@ int whir_domain_separator_cli(const whir_params* p, const char* session, uint8_t
     whir_host::cbor_text(m, "batch_size");
     whir_host::cbor_uint(m, p->batch_size);
     whir_host::cbor_text(m, "hash_id");
-    // EngineId bytes of BLAKE3, as an array of byte-valued integers
-    const uint8_t* id = kBlake3EngineId;
+    // EngineId bytes of the hash engine, as an array of byte-valued integers
+    const uint8_t* id = p->hash_id == WHIR_HASH_SHA2 ? kSha2EngineId : kBlake3EngineId;
     whir_host::cbor_head(m, 4, 32);
     for (int i = 0; i < 32; ++i) whir_host::cbor_uint(m, id[i]);
     whir_host::sha3_512(m.data(), m.size(), protocol_id);
@@ -685,6 +699,7 @@ static int prover_create(whir_ctx* ctx, const whir_params* params, uint64_t size
     whir_prover* P = new whir_prover();
     P->ctx = ctx;
     P->params = to_params(params);
+    P->hash = params->hash_id == WHIR_HASH_SHA2 ? HASH_SHA2 : HASH_BLAKE3;
     P->size = size;
     P->cfg = whir_host::WhirConfig::derive(size, P->params);
     P->L = plan_layout(P->cfg, size);
@@ -738,6 +753,7 @@ int whir_prover_commit(whir_prover* P, const uint64_t* d_vectors, size_t num_vec
     if (P->committed) return bad_arg("already committed");
     if (num_vectors != P->cfg.initial.num_vectors) return bad_arg("num_vectors must equal batch_size");
     std::lock_guard<std::recursive_mutex> g(P->ctx->mu);
+    HashScope hs(P->ctx, P->hash);
     P->d_vectors = d_vectors;
     P->num_vectors = num_vectors;
     int rc = commit_witness(P, P->cfg.initial, d_vectors, false, P->initial, P->L.init_matrix, P->L.init_nodes);
@@ -753,6 +769,7 @@ int whir_prover_prove(whir_prover* P, size_t num_forms, const uint64_t* points, 
     if (P->proved) return bad_arg("already proved");
     if (num_forms && (!points || !evaluations)) return bad_arg("null constraint arrays");
     std::lock_guard<std::recursive_mutex> g(P->ctx->mu);
+    HashScope hs(P->ctx, P->hash);
     int rc = prove_impl(P, num_forms, points, evaluations);
     if (rc) return rc;
     P->proved = true;

@@ -116,8 +116,9 @@ class PowBatcher {
 public:
     static constexpr int LANES = 2;  // launches in flight at once
 
-    int init(int device) {
+    int init(int device, uint32_t hash) {
         HIPCHK(hipSetDevice(device));
+        hash_ = hash;
         cap_ = (size_t)POW_BATCH_MAX << (POW_MAX_LG_LANES - 8);  // one minimum per block
         for (Lane& l : lanes_) {
             HIPCHK(hipStreamCreateWithFlags(&l.st, hipStreamNonBlocking));
@@ -159,6 +160,7 @@ private:
         uint64_t nonce = 0;
     };
     // one launch path: its stream, early-exit slot sets and block minima
+    uint32_t hash_ = HASH_BLAKE3;  // the engine of every search (the server's hash_id)
     struct Lane {
         hipStream_t st = nullptr;
         unsigned long long* slots = nullptr;
@@ -172,6 +174,7 @@ private:
     int search(const uint8_t* ch, uint64_t threshold, uint64_t* nonce) {
         Waiter w;
         pow_job_init(w.job, ch, threshold, SERVE_POW_EXTRA_BITS);
+        w.job.hash = hash_;
         std::unique_lock<std::mutex> lk(mu_);
         queue_.push_back(&w);
         while (!w.done) {
@@ -386,7 +389,7 @@ int whir_host_free(void* p) {
 int whir_server_create(int device, const whir_params* params, uint32_t num_slots, size_t hbm_budget_bytes,
                        whir_server** out) {
     if (!params || !out || num_slots == 0 || hbm_budget_bytes == 0) return bad_arg("bad server arguments");
-    if (params->hash_id != WHIR_HASH_BLAKE3) return bad_arg("only the BLAKE3 engine is supported");
+    if (params->hash_id != WHIR_HASH_BLAKE3 && params->hash_id != WHIR_HASH_SHA2) return bad_arg("unknown hash engine");
     HIPCHK(hipSetDevice(device));
     std::unique_ptr<whir_server> S(new whir_server());
     S->device = device;
@@ -396,7 +399,7 @@ int whir_server_create(int device, const whir_params* params, uint32_t num_slots
     S->budget = (hbm_budget_bytes + 255) & ~size_t{255};
     HIPCHK(hipMalloc((void**)&S->region, S->budget));
     S->arena.init(S->region, S->budget);
-    if ((rc = S->pow.init(device))) {
+    if ((rc = S->pow.init(device, params->hash_id == WHIR_HASH_SHA2 ? HASH_SHA2 : HASH_BLAKE3))) {
         (void)hipFree(S->region);
         return rc;
     }

@@ -262,6 +262,10 @@ int whir_gpu::pow_run(hipStream_t st, unsigned long long* slots, int& set, const
         b0 += j.blocks;
     }
     if (b0 > cap) return bad_arg("PoW batch exceeds the block-minimum buffer");
+    for (int k = 1; k < n; ++k)
+        if (jobs[k]->hash != jobs[0]->hash) return bad_arg("PoW batch mixes hash engines");
+    if (jobs[0]->hash != HASH_BLAKE3 && jobs[0]->hash != HASH_SHA2) return bad_arg("unknown PoW hash engine");
+    a.hash = jobs[0]->hash;
     a.njobs = (uint32_t)n;
     a.total_blocks = b0;
     const int k = set;
@@ -292,17 +296,27 @@ int whir_gpu::commit_device(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     int rc;
     bool fused = false;
+    const bool sha2 = ctx->hash == HASH_SHA2;  // fused leaves are BLAKE3's
     if ((rc = ctx->mark(0))) return rc;
-    if ((rc = rs_encode_any(ctx, d_coeffs, num_polys, poly_size, cw, depth, d_matrix, ext, d_nodes, &fused))) return rc;
+    if ((rc = rs_encode_any(ctx, d_coeffs, num_polys, poly_size, cw, depth, d_matrix, ext, sha2 ? nullptr : d_nodes,
+                            &fused)))
+        return rc;
     if ((rc = ctx->mark(1))) return rc;
     size_t row_u64 = num_polys * depth * (ext ? 3 : 1);
-    if (!fused) leaf_hash_launch(d_matrix, cw, (uint32_t)row_u64, d_nodes, ctx->stream);
+    if (sha2)
+        sha256_leaves_launch(d_matrix, cw, (uint32_t)row_u64, d_nodes, ctx->stream);
+    else if (!fused)
+        leaf_hash_launch(d_matrix, cw, (uint32_t)row_u64, d_nodes, ctx->stream);
     if ((rc = ctx->mark(2))) return rc;
     // from two leaves up the tree's last kernel also writes the root to mapped
     // host memory, so reading it costs no copy
     const bool mapped_root = out_root && cw >= 2;
     if (mapped_root && (rc = ctx->ensure_hpart())) return rc;
-    merkle_build_launch(d_nodes, cw, ctx->stream, mapped_root ? (uint8_t*)ctx->hpart_d : nullptr);
+    uint8_t* root_copy = mapped_root ? (uint8_t*)ctx->hpart_d : nullptr;
+    if (sha2)
+        sha256_merkle_build_launch(d_nodes, cw, ctx->stream, root_copy);
+    else
+        merkle_build_launch(d_nodes, cw, ctx->stream, root_copy);
     HIPCHK(hipGetLastError());
     if ((rc = ctx->mark(3))) return rc;
     if (mapped_root) {
@@ -351,6 +365,14 @@ int whir_gpu_warmup(void) {
         HIPCHK(hipStreamSynchronize(ctx->stream));
         return 0;
     });
+}
+
+int whir_ctx_set_hash(whir_ctx* ctx, uint32_t hash_id) {
+    if (!ctx) return bad_arg("null context");
+    if (hash_id != WHIR_HASH_BLAKE3 && hash_id != WHIR_HASH_SHA2) return bad_arg("unknown hash engine");
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    ctx->hash = hash_id == WHIR_HASH_SHA2 ? HASH_SHA2 : HASH_BLAKE3;
+    return 0;
 }
 
 int whir_ctx_create(int device, whir_ctx** out) {
@@ -489,7 +511,10 @@ int whir_dev_ntt_batch(whir_ctx* ctx, uint64_t* d_values, size_t total, size_t s
 
 int whir_dev_commit_leaves(whir_ctx* ctx, const uint64_t* d_matrix, size_t rows, size_t row_u64, uint8_t* d_leaves) {
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
-    leaf_hash_launch(d_matrix, rows, (uint32_t)row_u64, d_leaves, ctx->stream);
+    if (ctx->hash == HASH_SHA2)
+        sha256_leaves_launch(d_matrix, rows, (uint32_t)row_u64, d_leaves, ctx->stream);
+    else
+        leaf_hash_launch(d_matrix, rows, (uint32_t)row_u64, d_leaves, ctx->stream);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -509,7 +534,10 @@ size_t whir_merkle_num_nodes(size_t n) {
 
 int whir_dev_merkle_build(whir_ctx* ctx, uint8_t* d_nodes, size_t num_leaves) {
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
-    merkle_build_launch(d_nodes, num_leaves, ctx->stream);
+    if (ctx->hash == HASH_SHA2)
+        sha256_merkle_build_launch(d_nodes, num_leaves, ctx->stream);
+    else
+        merkle_build_launch(d_nodes, num_leaves, ctx->stream);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -726,20 +754,35 @@ int whir_gpu_interleaved_rs_encode_blake3_ext3_matrix_leaves(const uint64_t* c0,
     });
 }
 
-int whir_gpu_blake3_merkle_tree(const uint8_t* leaves, size_t n, uint8_t* out_root, uint8_t* out_nodes) {
+static void build_tree(uint32_t hash, uint8_t* d_nodes, size_t n, hipStream_t st) {
+    if (hash == HASH_SHA2)
+        sha256_merkle_build_launch(d_nodes, n, st);
+    else
+        merkle_build_launch(d_nodes, n, st);
+}
+
+static int host_merkle_tree(uint32_t hash, const uint8_t* leaves, size_t n, uint8_t* out_root, uint8_t* out_nodes) {
     return with_default([&](whir_ctx* ctx) {
         size_t nn = whir_merkle_num_nodes(n);
         int rc = ctx->ensure(ctx->hnodes, nn * 32);
         if (rc) return rc;
         uint8_t* d = (uint8_t*)ctx->hnodes.p;
         if (n) HIPCHK(hipMemcpyAsync(d, leaves, n * 32, hipMemcpyHostToDevice, ctx->stream));
-        merkle_build_launch(d, n, ctx->stream);
+        build_tree(hash, d, n, ctx->stream);
         HIPCHK(hipGetLastError());
         if (out_nodes) HIPCHK(hipMemcpyAsync(out_nodes, d, nn * 32, hipMemcpyDeviceToHost, ctx->stream));
         if (out_root) HIPCHK(hipMemcpyAsync(out_root, d + 32 * (nn - 1), 32, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
         return 0;
     });
+}
+
+int whir_gpu_blake3_merkle_tree(const uint8_t* leaves, size_t n, uint8_t* out_root, uint8_t* out_nodes) {
+    return host_merkle_tree(HASH_BLAKE3, leaves, n, out_root, out_nodes);
+}
+
+int whir_gpu_sha256_merkle_tree(const uint8_t* leaves, size_t n, uint8_t* out_root, uint8_t* out_nodes) {
+    return host_merkle_tree(HASH_SHA2, leaves, n, out_root, out_nodes);
 }
 
 size_t whir_merkle_hint_node_indices(size_t num_leaves, const uint64_t* indices, size_t k, uint64_t* out) {
@@ -771,8 +814,8 @@ size_t whir_merkle_hint_node_indices(size_t num_leaves, const uint64_t* indices,
     return nh;
 }
 
-int whir_gpu_blake3_merkle_open_path(const uint8_t* leaves, size_t n, const uint64_t* indices, size_t k,
-                                     uint8_t* out_root, uint8_t* out_hints) {
+static int host_open_path(uint32_t hash, const uint8_t* leaves, size_t n, const uint64_t* indices, size_t k,
+                          uint8_t* out_root, uint8_t* out_hints) {
     for (size_t i = 0; i < k; ++i)
         if (indices[i] >= n) return bad_arg("opening index out of range");
     return with_default([&](whir_ctx* ctx) {
@@ -786,7 +829,7 @@ int whir_gpu_blake3_merkle_open_path(const uint8_t* leaves, size_t n, const uint
         if ((rc = ctx->ensure(ctx->hout, nh * 32))) return rc;
         uint8_t* d = (uint8_t*)ctx->hnodes.p;
         if (n) HIPCHK(hipMemcpyAsync(d, leaves, n * 32, hipMemcpyHostToDevice, ctx->stream));
-        merkle_build_launch(d, n, ctx->stream);
+        build_tree(hash, d, n, ctx->stream);
         if (nh) {
             HIPCHK(hipMemcpyAsync(ctx->hidx.p, idx.data(), nh * 8, hipMemcpyHostToDevice, ctx->stream));
             gather_hashes_launch(d, (uint64_t*)ctx->hidx.p, nh, (uint8_t*)ctx->hout.p, ctx->stream);
@@ -797,6 +840,16 @@ int whir_gpu_blake3_merkle_open_path(const uint8_t* leaves, size_t n, const uint
         HIPCHK(hipStreamSynchronize(ctx->stream));
         return 0;
     });
+}
+
+int whir_gpu_blake3_merkle_open_path(const uint8_t* leaves, size_t n, const uint64_t* indices, size_t k,
+                                     uint8_t* out_root, uint8_t* out_hints) {
+    return host_open_path(HASH_BLAKE3, leaves, n, indices, k, out_root, out_hints);
+}
+
+int whir_gpu_sha256_merkle_open_path(const uint8_t* leaves, size_t n, const uint64_t* indices, size_t k,
+                                     uint8_t* out_root, uint8_t* out_hints) {
+    return host_open_path(HASH_SHA2, leaves, n, indices, k, out_root, out_hints);
 }
 
 static int ood_host(const uint64_t* const* comps, int ext, const uint64_t* points, uint64_t* out, size_t nv,
@@ -853,18 +906,29 @@ int whir_gpu_encode_to_bytes(const uint64_t* values, uint8_t* out, size_t count)
     });
 }
 
-int whir_gpu_blake3_hash_many(size_t size, const uint8_t* input, size_t count, uint8_t* output) {
+static int host_hash_many(uint32_t hash, size_t size, const uint8_t* input, size_t count, uint8_t* output) {
     return with_default([&](whir_ctx* ctx) {
         int rc;
         if ((rc = ctx->ensure(ctx->hin, size * count + 16))) return rc;
         if ((rc = ctx->ensure(ctx->hout, count * 32))) return rc;
         if (size * count) HIPCHK(hipMemcpyAsync(ctx->hin.p, input, size * count, hipMemcpyHostToDevice, ctx->stream));
-        hash_many_launch((uint8_t*)ctx->hin.p, size, count, (uint8_t*)ctx->hout.p, ctx->stream);
+        if (hash == HASH_SHA2)
+            sha256_hash_many_launch((uint8_t*)ctx->hin.p, size, count, (uint8_t*)ctx->hout.p, ctx->stream);
+        else
+            hash_many_launch((uint8_t*)ctx->hin.p, size, count, (uint8_t*)ctx->hout.p, ctx->stream);
         HIPCHK(hipGetLastError());
         if (count) HIPCHK(hipMemcpyAsync(output, ctx->hout.p, count * 32, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
         return 0;
     });
+}
+
+int whir_gpu_blake3_hash_many(size_t size, const uint8_t* input, size_t count, uint8_t* output) {
+    return host_hash_many(HASH_BLAKE3, size, input, count, output);
+}
+
+int whir_gpu_sha256_hash_many(size_t size, const uint8_t* input, size_t count, uint8_t* output) {
+    return host_hash_many(HASH_SHA2, size, input, count, output);
 }
 
 int whir_gpu_sumcheck_compute(const uint64_t* a, int a_is_base, const uint64_t* b, size_t n, uint64_t* c0,
