@@ -139,6 +139,9 @@ def ref():
                                           ctypes.POINTER(_sz)]
         R.ref_cli_ds_hash.restype = None
         R.ref_cli_ds_hash.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, _u8p, _u8p]
+        R.ref_whir_verify_hash.restype = ctypes.c_int
+        R.ref_whir_verify_hash.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, ctypes.c_uint32, _sz,
+                                           _u64p, _u64p, _u8p, _sz, _u8p, _sz]
         R.ref_sha256_many.argtypes = [_sz, _u8p, _sz, _u8p]
         R.ref_merkle_build_sha2.argtypes = [_u8p, _sz, _u8p]
         R.ref_commit_leaves_base_sha2.argtypes = [_u64p, _sz, _sz, _u8p]
@@ -435,6 +438,20 @@ def ref_cli_ds(hash_id: int = 1, **prm) -> tuple[bytes, bytes]:
     sid = np.zeros(32, dtype=np.uint8)
     ref().ref_cli_ds_hash(_prm(**prm), hash_id, _ptr8(pid), _ptr8(sid))
     return pid.tobytes(), sid.tobytes()
+
+
+def ref_verify(nv: int, points, evaluations, narg: bytes, hints: bytes, hash_id: int = 1, **prm) -> bool:
+    """The reference's own verifier (whir_verifier.hpp through the CLI's
+    main.cpp:434-446 flow) on a proof of the claims f(points[e]) =
+    evaluations[e * batch + v]; True when it accepts."""
+    R = ref()
+    pts = u64(points).reshape(-1)
+    nf = pts.size // (3 * nv) if nv else 0
+    ev = u64(evaluations).reshape(-1) if nf else np.zeros(3, dtype=np.uint64)
+    n = np.frombuffer(bytes(narg) + b"\0", dtype=np.uint8).copy()
+    h = np.frombuffer(bytes(hints) + b"\0", dtype=np.uint8).copy()
+    return R.ref_whir_verify_hash(_prm(**prm), hash_id, nv, nf, _ptr64(pts) if nf else None, _ptr64(ev),
+                                  _ptr8(n), len(narg), _ptr8(h), len(hints)) == 1
 
 
 def ref_prove(nv: int, coeffs, points, hash_id: int = 1, **prm):

@@ -20,6 +20,8 @@
 //   ref_sha256_many        hash/sha2_engine.hpp:62-79 Sha2::hash_many
 //   ref_*_sha2             the SHA-256 tree and row leaves (ENGINE_ID_SHA2)
 //   ref_whir_prove_hash, ref_cli_ds_hash  the PCS flow and DS with --hash Sha2
+//   ref_whir_verify_hash   the main.cpp:434-446 verify flow (receive_commitment, verify,
+//                          FinalClaim::verify, check_eof) on a given proof
 //   ref_whir_config        protocols/whir/whir.hpp:118-224 Config::from_params, flattened
 //   ref_whir_prove(_params) the main.cpp:292-424 PCS flow: from_params, the CLI's domain
 //                          separator (main.cpp:176-215), commit, MLE constraints, prove
@@ -405,6 +407,60 @@ int ref_whir_prove_hash(const uint32_t* prm, uint32_t hash_id, uint32_t num_vari
     std::memcpy(narg, proof.narg_string.data(), proof.narg_string.size());
     std::memcpy(hints, proof.hints.data(), proof.hints.size());
     return 0;
+}
+
+// 1 when the reference verifier accepts (narg, hints) for the claims
+// f(points[e]) = evals[e * batch + v], 0 when it rejects.  Layouts as in
+// ref_whir_prove_hash; the domain separator is the CLI's for these parameters.
+int ref_whir_verify_hash(const uint32_t* prm, uint32_t hash_id, uint32_t num_variables, size_t num_constraints,
+                         const uint64_t* points, const uint64_t* evals, const uint8_t* narg, size_t narg_len,
+                         const uint8_t* hints, size_t hints_len) {
+    namespace tx = whir::transcript;
+    namespace alg = whir::algebra;
+    whir::ProtocolParameters p;
+    p.unique_decoding = prm[0] != 0;
+    p.starting_log_inv_rate = prm[1];
+    p.initial_folding_factor = prm[2];
+    p.folding_factor = prm[3];
+    p.security_level = prm[4];
+    p.pow_bits = prm[5];
+    p.batch_size = prm[6];
+    p.hash_id = hash_id == 2 ? whir::hash::ENGINE_ID_SHA2 : whir::hash::ENGINE_ID_BLAKE3;
+    const std::size_t size = std::size_t{1} << num_variables;
+    const std::size_t nvec = p.batch_size;
+    auto config = whirp::Config<Emb>::from_params(size, p);
+    std::vector<std::unique_ptr<alg::LinearForm<GoldilocksExt3>>> forms;
+    std::vector<const alg::LinearForm<GoldilocksExt3>*> refs;
+    std::vector<GoldilocksExt3> evaluations;
+    for (std::size_t e = 0; e < num_constraints; ++e) {
+        std::vector<GoldilocksExt3> pt(num_variables);
+        for (std::size_t k = 0; k < num_variables; ++k) {
+            const uint64_t* q = points + 3 * (e * num_variables + k);
+            pt[k] = GoldilocksExt3(Goldilocks::from_u64(q[0]), Goldilocks::from_u64(q[1]),
+                                   Goldilocks::from_u64(q[2]));
+        }
+        forms.push_back(std::make_unique<alg::MultilinearExtension<GoldilocksExt3>>(std::move(pt)));
+        refs.push_back(forms.back().get());
+        for (std::size_t v = 0; v < nvec; ++v) {
+            const uint64_t* q = evals + 3 * (e * nvec + v);
+            evaluations.push_back(GoldilocksExt3(Goldilocks::from_u64(q[0]), Goldilocks::from_u64(q[1]),
+                                                 Goldilocks::from_u64(q[2])));
+        }
+    }
+    tx::ProverState::Proof proof;
+    proof.narg_string.assign(narg, narg + narg_len);
+    proof.hints.assign(hints, hints + hints_len);
+    auto ds = cli_ds(p);
+    tx::Empty instance;
+    try {
+        auto vs = tx::VerifierState::from_ds(ds, instance, proof);
+        auto comm = config.receive_commitment(vs);
+        std::vector<const whir::protocols::irs_commit::Commitment<GoldilocksExt3>*> cp{&comm};
+        auto claim = config.verify(vs, cp, std::span<const GoldilocksExt3>{evaluations});
+        return claim.verify(refs) && vs.check_eof() ? 1 : 0;
+    } catch (...) {
+        return 0;
+    }
 }
 
 int ref_whir_prove_params(const uint32_t* prm, uint32_t num_variables, const uint64_t* coeffs,

@@ -240,3 +240,48 @@ def test_proof_digests(ctx):
         assert hashlib.sha256(narg).hexdigest() == c["narg_sha256"]
         assert hashlib.sha256(hints).hexdigest() == c["hints_sha256"]
         assert (len(narg), len(hints)) == (c["narg_len"], c["hints_len"])
+
+
+# Proofs the reference verifier must accept, at sizes and parameters beyond the
+# byte-for-byte comparison above (nv 24 included); the claimed values come from
+# the GPU's own MLE evaluation.
+VERIFY_CASES = [
+    (12, dict(nforms=1)),
+    (16, dict(nforms=2, rate=2)),
+    (20, dict(nforms=3)),
+    (22, dict(nforms=3)),
+    (24, dict(nforms=1)),
+    (14, dict(nforms=2, batch=4)),
+    (15, dict(nforms=1, unique=1)),
+    (17, dict(nforms=2, ff=3, iff=5)),
+    (18, dict(nforms=2, hash_id=2)),
+]
+
+
+@pytest.mark.skipif("not __import__('oracle').ref()")
+@pytest.mark.parametrize("case", VERIFY_CASES, ids=_case_id)
+def test_gpu_proof_verifies(ctx, case):
+    nv, kw = case
+    kw = dict(kw)
+    nforms = kw.pop("nforms")
+    hid = kw.pop("hash_id", 1)
+    batch = kw.get("batch", 1)
+    coeffs, pts = _inputs(nv, nforms, batch, 0x7E51F000 + nv)
+    print("verify", nv, nforms, kw, hid, flush=True)
+    n = 1 << nv
+    d = ctx.upload(coeffs)
+    ev = np.array([ctx.mle_evaluate(d + 8 * n * v, True, pts[f]) for f in range(nforms) for v in range(batch)],
+                  dtype=np.uint64).reshape(-1, 3)
+    p = W.Params.default(unique_decoding=kw.get("unique", 0), starting_log_inv_rate=kw.get("rate", 1),
+                         initial_folding_factor=kw.get("iff", 4), folding_factor=kw.get("ff", 4),
+                         batch_size=batch, hash_id=hid)
+    pid, sid = W.domain_separator_cli(p)
+    with W.Prover(ctx, p, n, pid, sid) as pr:
+        pr.commit(d, batch)
+        pr.prove(pts, ev)
+        narg, hints = pr.proof()
+    ctx.free(d)
+    assert oracle.ref_verify(nv, pts, ev, narg, hints, hash_id=hid, **kw)
+    b = bytearray(narg)
+    b[len(b) // 2] ^= 0x10
+    assert not oracle.ref_verify(nv, pts, ev, bytes(b), hints, hash_id=hid, **kw)

@@ -78,6 +78,28 @@ def test_served_vs_reference():
             assert srv.wait(rid)[:2] == (narg, hints)
 
 
+def test_served_proofs_verify():
+    """Every proof of a mixed served batch is accepted by the reference
+    verifier (the claimed values from the GPU's MLE evaluation)."""
+    if oracle.ref() is None:
+        pytest.skip("reference build not available")
+    params = W.Params.default()
+    rnd = random.Random(0x5E7)
+    sizes = [rnd.choice([12, 14, 16, 18, 20, 22]) for _ in range(12)]
+    with W.Server(params, num_slots=4, hbm_budget=4 << 30) as srv, W.Context() as ctx:
+        reqs = []
+        for i, nv in enumerate(sizes):
+            c, p = _request(nv, 2, 0x5E7000 + i)
+            reqs.append((nv, c, p, srv.submit(1 << nv, c, p)))
+        for nv, c, p, rid in reqs:
+            narg, hints, _ = srv.wait(rid)
+            d = ctx.upload(c)
+            ev = np.array([ctx.mle_evaluate(d, True, p[f]) for f in range(len(p))], dtype=np.uint64)
+            ctx.free(d)
+            print("served verify", nv, flush=True)
+            assert oracle.ref_verify(nv, p, ev, narg, hints)
+
+
 def test_request_larger_than_budget():
     params = W.Params.default()
     with W.Server(params, num_slots=1, hbm_budget=1 << 20) as srv:

@@ -124,6 +124,21 @@ def test_sha256_dev_commit(ctx, shape):
     assert root_b == oracle.merkle_build(oracle.commit_leaves_base(want, cw, cols), cw)[-32:].tobytes()
 
 
+def test_sha256_commit_golden_whir(ctx, fixtures):
+    """The reference's golden_whir proofs (dump_whir.cpp: SHA-256 trees, depth
+    1, codeword 2x the vector size): the device commit of their input vectors
+    gives their witness matrix, and its root opens their narg string."""
+    for c in fixtures["whir"]:
+        nv, vs = c["num_vectors"], c["vector_size"]
+        print("sha256 golden_whir commit", nv, vs, flush=True)
+        dx = ctx.upload(np.array(sum(c["vectors"], []), dtype=np.uint64))
+        dm = ctx.alloc(8 * 2 * vs * nv)
+        dn = ctx.alloc(32 * W.merkle_num_nodes(2 * vs))
+        root = ctx.commit(dx, nv, vs, 2 * vs, 1, dm, dn)
+        assert list(ctx.download(dm, 2 * vs * nv)) == c["witness_matrix"]
+        assert root.hex() == c["proof_narg"][:64]
+
+
 @pytest.mark.parametrize("bits", [1, 4, 8, 12])
 def test_sha256_pow(ctx, bits):
     rnd = random.Random(100 + bits)

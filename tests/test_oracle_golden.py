@@ -98,6 +98,16 @@ def test_sha256_merkle_golden(orc, fixtures):
         assert [hints[32 * i:32 * i + 32].tobytes().hex() for i in range(len(hints) // 32)] == c["hints"]
 
 
+def test_sha256_commit_golden_whir(orc, fixtures):
+    """golden_whir proofs (dump_whir.cpp) commit with SHA-256 trees: the
+    narg string opens with the root of the witness matrix's row leaves."""
+    for c in fixtures["whir"]:
+        nv, vs = c["num_vectors"], c["vector_size"]
+        m = np.array(c["witness_matrix"], dtype=np.uint64)
+        nodes = orc.sha256_merkle_build(orc.sha256_leaves(m, 2 * vs, nv), 2 * vs)
+        assert nodes[-32:].tobytes().hex() == c["proof_narg"][:64]
+
+
 def test_sha256_oracle_vs_reference(orc):
     """Row leaves, trees (ragged sizes, zero padding) and hash_many against the
     reference's own Sha2 engine built from its sources."""

@@ -48,6 +48,50 @@ __global__ __launch_bounds__(256) void sha256_level_kernel(uint8_t* __restrict__
     if (root_copy && count == 1) store32(root_copy, h);
 }
 
+// The last levels of a tree in one workgroup: the level of 2 * parents
+// children at child_off (parents <= 256, a power of two) up to the root, each
+// level's hashes kept in LDS for the next and stored to the node array.  A
+// level this small is bound by the latency of one node hash (one wave per
+// SIMD), so merging the levels saves each one's launch and global reload.
+__global__ __launch_bounds__(256) void sha256_top_kernel(uint8_t* __restrict__ nodes, uint64_t child_off,
+                                                         uint32_t parents, uint8_t* root_copy) {
+    __shared__ uint32_t lvl[2][256 * 8];
+    const uint32_t t = threadIdx.x;
+    uint64_t off = child_off;
+    uint32_t width = 2 * parents;
+    int cur = 0;
+    for (bool first = true; width > 1; first = false) {
+        const uint32_t np = width / 2;
+        if (t < np) {
+            uint32_t m[16];
+            if (first) {
+                const uint4* src = reinterpret_cast<const uint4*>(nodes + 32 * (off + 2 * t));
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const uint4 v = src[q];
+                    m[4 * q] = v.x;
+                    m[4 * q + 1] = v.y;
+                    m[4 * q + 2] = v.z;
+                    m[4 * q + 3] = v.w;
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) m[k] = lvl[cur][16 * t + k];
+            }
+            uint32_t h[8];
+            s256::hash64(m, h);
+            store32(nodes + 32 * (off + width + t), h);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) lvl[cur ^ 1][8 * t + k] = h[k];
+            if (root_copy && np == 1) store32(root_copy, h);
+        }
+        __syncthreads();
+        cur ^= 1;
+        off += width;
+        width = np;
+    }
+}
+
 // big-endian word k of the 64-byte block at p (bytes past `avail` are zero,
 // and the byte at `avail` is the 0x80 end marker when `mark`)
 __device__ __forceinline__ uint32_t block_word(const uint8_t* p, int k, uint64_t avail, bool mark) {
@@ -112,13 +156,15 @@ void sha256_merkle_build_launch(uint8_t* nodes, uint64_t n_leaves, hipStream_t s
         return;
     }
     uint64_t off = 0;
-    while (width > 1) {
+    while (width > 512) {  // one level per launch while a level fills more than one CU
         const uint64_t parents = width / 2;
         hipLaunchKernelGGL(sha256_level_kernel, dim3(blocks_256(parents)), dim3(256), 0, st, nodes, off, parents,
                            root_copy);
         off += width;
         width = parents;
     }
+    if (width > 1)
+        hipLaunchKernelGGL(sha256_top_kernel, dim3(1), dim3(256), 0, st, nodes, off, (uint32_t)(width / 2), root_copy);
 }
 
 void sha256_hash_many_launch(const uint8_t* in, uint64_t size, uint64_t count, uint8_t* out, hipStream_t st) {

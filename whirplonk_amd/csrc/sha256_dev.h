@@ -8,7 +8,8 @@
 // work).  SHA-256 reads big-endian 32-bit words, so a little-endian word of
 // the byte string enters the schedule byte-swapped, and the digest's words
 // are byte-swapped back into bytes.  Integer VALU only: add, xor, rotate
-// (v_alignbit_b32), bitfield select (v_bfi_b32), byte permute (v_perm_b32).
+// (v_alignbit_b32), three-input logic (v_bitop3_b32), bitfield select (v_bfi_b32),
+// byte permute (v_perm_b32).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -71,6 +72,14 @@ static_assert(T.h0[0] == 0x6A09E667u && T.k[0] == 0x428A2F98u && T.k[63] == 0xC6
 // ------------------------------------------------------------ compression
 __device__ __forceinline__ uint32_t rotr(uint32_t x, int n) { return __builtin_rotateright32(x, n); }
 __device__ __forceinline__ uint32_t bswap(uint32_t x) { return __builtin_bswap32(x); }
+// a ^ b ^ c in one v_bitop3_b32 (truth table 0x96, symmetric in its inputs);
+// the compiler otherwise spends two v_xor_b32 on each sigma function.  The
+// builtin is opaque to constant folding, so a value known at compile time
+// (the padding block's message schedule) takes the plain form and folds away.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    if (__builtin_constant_p(a ^ b ^ c)) return a ^ b ^ c;
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
 
 __device__ __forceinline__ void init(uint32_t s[8]) {
 #pragma unroll
@@ -87,14 +96,14 @@ __device__ __forceinline__ void compress(uint32_t s[8], const uint32_t* win) {
     for (int i = 0; i < 64; ++i) {
         if (i >= 16) {
             const uint32_t w15 = w[(i - 15) & 15], w2 = w[(i - 2) & 15];
-            const uint32_t s0 = rotr(w15, 7) ^ rotr(w15, 18) ^ (w15 >> 3);
-            const uint32_t s1 = rotr(w2, 17) ^ rotr(w2, 19) ^ (w2 >> 10);
+            const uint32_t s0 = xor3(rotr(w15, 7), rotr(w15, 18), w15 >> 3);
+            const uint32_t s1 = xor3(rotr(w2, 17), rotr(w2, 19), w2 >> 10);
             w[i & 15] = w[i & 15] + s0 + w[(i - 7) & 15] + s1;
         }
-        const uint32_t S1 = rotr(e, 6) ^ rotr(e, 11) ^ rotr(e, 25);
+        const uint32_t S1 = xor3(rotr(e, 6), rotr(e, 11), rotr(e, 25));
         const uint32_t ch = (e & f) ^ (~e & g);
         const uint32_t t1 = h + S1 + ch + T.k[i] + w[i & 15];
-        const uint32_t S0 = rotr(a, 2) ^ rotr(a, 13) ^ rotr(a, 22);
+        const uint32_t S0 = xor3(rotr(a, 2), rotr(a, 13), rotr(a, 22));
         const uint32_t mj = (a & b) | (c & (a | b));
         const uint32_t t2 = S0 + mj;
         h = g;
