@@ -23,6 +23,8 @@ Beside the headline line's workload it measures, in the same JSON line:
   * full_prove: the whole WHIR prove (commit, every round, PoW, openings) at
     num_variables 22, rate 1/2, folding factor 4, 3 MLE constraints, against
     the reference's own prover on the host (bit-exact proof bytes);
+  * full_prove_sha2: the same prove with SHA-256 trees, leaves and PoW (the
+    CLI's --hash Sha2);
   * batched: 64 polynomials of 2^14..2^22 coefficients (rate 1/4) committed by
     one graph launch, against one-at-a-time commits and the CPU roots;
   * served: commit+prove requests (sizes 2^16..2^24 with weights
@@ -31,7 +33,7 @@ Beside the headline line's workload it measures, in the same JSON line:
     measured plain-path rate) with p50/p99 latency, and the plain
     one-at-a-time GPU path on the same requests.
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--legs commit,prove,batched,served]
+    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--legs commit,prove,prove_sha2,batched,served]
 """
 from __future__ import annotations
 
@@ -481,7 +483,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--legs", default="commit,prove,batched,served")
+    ap.add_argument("--legs", default="commit,prove,prove_sha2,batched,served")
     ap.add_argument("--requests", type=int, default=2000, help="served requests")
     ap.add_argument("--offered", type=float, default=0.0,
                     help="served offered rate (requests/s, Poisson); 0: 0.8x the measured plain-path rate")

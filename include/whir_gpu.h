@@ -222,7 +222,7 @@ int whir_dev_sumcheck_round(whir_ctx* ctx, const uint64_t* d_a, int a_is_base, c
 
 /* ProtocolParameters (parameters.hpp:25-34).  hash_id selects the Merkle,
  * row-leaf and PoW hash engine: WHIR_HASH_BLAKE3 (hash/blake3_engine.hpp:50,
- * the CLI's default) or WHIR_HASH_SHA2 (hash/sha2_engine.hpp:39, the CLI's
+ * the CLI's default) or WHIR_HASH_SHA2 (hash/sha2_engine.hpp:40, the CLI's
  * --hash Sha2). */
 #define WHIR_HASH_BLAKE3 1u
 #define WHIR_HASH_SHA2 2u

@@ -104,7 +104,7 @@ Layout plan_layout(const whir_host::WhirConfig& cfg, size_t size) {
 const uint8_t kBlake3EngineId[32] = {0x03, 0xe0, 0x17, 0x49, 0xeb, 0xcc, 0x04, 0x77, 0x92, 0x42, 0x54,
                                      0xeb, 0x48, 0x20, 0x66, 0xb8, 0x64, 0xa8, 0xdd, 0x4d, 0x77, 0x25,
                                      0x24, 0x64, 0xca, 0x6f, 0x5b, 0x6f, 0x5c, 0xc0, 0x5b, 0x4c};
-// EngineId of the SHA-256 engine (hash/sha2_engine.hpp:39-44), likewise fixed.
+// EngineId of the SHA-256 engine (hash/sha2_engine.hpp:40-44), likewise fixed.
 const uint8_t kSha2EngineId[32] = {0x01, 0x8e, 0xaa, 0x24, 0x7c, 0xb8, 0x95, 0x7a, 0xb1, 0xe9, 0xfd,
                                    0xac, 0x88, 0x54, 0x50, 0x40, 0x3c, 0x5e, 0x7b, 0xda, 0x1a, 0xca,
                                    0xaa, 0x50, 0x4e, 0x4c, 0xc8, 0xf2, 0xf7, 0x6b, 0xb0, 0x76};
