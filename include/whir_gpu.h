@@ -364,8 +364,9 @@ int whir_tuning_import(const uint64_t* records, size_t words);
 
 /* Many independent base-field polynomials of (possibly different) power-of-two
  * sizes, each committed as irs_commit::Config::commit does (RS at rate
- * 2^-log_inv_rate with interleaving depth 2^folding_factor, BLAKE3 row
- * leaves, Merkle tree; irs_commit.hpp:344-460).  create plans one device
+ * 2^-log_inv_rate with interleaving depth 2^folding_factor, row leaves and
+ * Merkle tree with the context's hash engine at create, BLAKE3 by default;
+ * irs_commit.hpp:344-460).  create plans one device
  * region for all of them, autotunes each shape (autotune != 0), and captures
  * every encode, leaf and tree kernel, spread over num_branches streams, plus a
  * final root gather, into one HIP graph.  Fill the coefficients

@@ -78,6 +78,32 @@ def _force(rec_key, lg):
     W.tuning_import(np.array(list(rec_key) + [lg], dtype=np.uint64))
 
 
+def test_batch_sha256(ctx):
+    """A batch made on a SHA-256 context commits with SHA-256 leaves and trees,
+    eagerly and on graph replays, also after the context goes back to BLAKE3."""
+    sizes = [1 << 10, 1 << 14, 1 << 12]
+    polys = _polys(sizes, 0x5A2B)
+    print("batch sha256", sizes, flush=True)
+    ctx.set_hash(W.HASH_SHA2)
+    try:
+        b = W.Batch(ctx, sizes, log_inv_rate=2, folding_factor=4, branches=2)
+    finally:
+        ctx.set_hash(W.HASH_BLAKE3)
+    with b:
+        for k, c in enumerate(polys):
+            b.upload(k, c)
+        want = []
+        for k, c in enumerate(polys):
+            cw = (sizes[k] // 16) << 2
+            m = oracle.rs_encode(c, 1, sizes[k], cw, 16)
+            want.append(oracle.sha256_merkle_build(oracle.sha256_leaves(m, cw, 16), cw)[-32:].tobytes())
+        b.run(graph=False)
+        assert b.roots() == want
+        for rep in range(2):
+            b.run(graph=True)
+            assert b.roots() == want, rep
+
+
 @pytest.mark.parametrize("m,n,ext3", [(1 << 14, 1 << 16, False), (1 << 12, 1 << 17, True), (1 << 16, 1 << 18, False)])
 def test_every_split_same_codeword(ctx, m, n, ext3):
     depth = 16

@@ -94,6 +94,7 @@ struct whir_batch {
     whir_ctx* ctx = nullptr;
     std::vector<uint64_t> sizes;
     uint32_t depth = 16, rate = 2;
+    uint32_t hash = HASH_BLAKE3;  // the context's engine when the batch was made
     std::vector<size_t> coeff_off, matrix_off, nodes_off, rows;
     size_t roots_off = 0, idx_off = 0, total = 0;
     std::vector<size_t> scratch_off, scratch_bytes;
@@ -120,6 +121,8 @@ struct whir_batch {
     // enqueue every commit: branch b runs its polynomials in order on `on[b]`
     int enqueue(const std::vector<hipStream_t>& on, hipStream_t main) {
         hipStream_t keep = ctx->stream;
+        const uint32_t keep_hash = ctx->hash;
+        ctx->hash = hash;
         int rc = 0;
         for (size_t b = 0; b < branch_polys.size() && !rc; ++b) {
             ctx->stream = on[b];
@@ -132,6 +135,7 @@ struct whir_batch {
             }
         }
         ctx->stream = keep;
+        ctx->hash = keep_hash;
         ctx->ext_scratch = nullptr;
         ctx->ext_scratch_bytes = 0;
         if (rc) return rc;
@@ -205,6 +209,7 @@ int whir_batch_create(whir_ctx* ctx, size_t count, const uint64_t* sizes, uint32
     const uint32_t depth = 1u << folding_factor;
     std::unique_ptr<whir_batch> B(new whir_batch());
     B->ctx = ctx;
+    B->hash = ctx->hash;
     B->sizes.assign(sizes, sizes + count);
     B->depth = depth;
     B->rate = log_inv_rate;
