@@ -493,8 +493,8 @@ def main():
                     help="also run one 512 MiB device copy kernel (reference for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build runs the commit-and-fold path on a single MI355X; "
-              "multi-GPU sharding is design-only (see the sharding section of the design notes).", file=sys.stderr)
+        print("bench.py measures one GPU only: the commit-and-prove path runs on a single MI355X; "
+              "sharding across GPUs is not implemented.", file=sys.stderr)
         sys.exit(2)
 
     legs = set(args.legs.split(","))

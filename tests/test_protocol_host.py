@@ -209,7 +209,7 @@ def test_product_transcript_vs_oracle(W):
 
 
 # derived shapes of the CLI defaults (security 128, PoW 20, list decoding)
-SURVEY_SHAPES = {
+CLI_DEFAULT_SHAPES = {
     # nv, log_inv_rate: in-domain samples of the initial commit and each round
     (22, 1): [252, 56, 32, 22, 17],
     (16, 2): [117, 45, 28, 20],
@@ -223,13 +223,13 @@ def _irs(flat, k):
     return int(flat[off + 4]), int(flat[off + 2])
 
 
-@pytest.mark.parametrize("shape", sorted(SURVEY_SHAPES))
+@pytest.mark.parametrize("shape", sorted(CLI_DEFAULT_SHAPES))
 def test_product_config_shapes(W, shape):
     nv, rate = shape
     flat = W.config_flat(1 << nv, W.Params.default(starting_log_inv_rate=rate))
     rounds = int(flat[0])
-    assert rounds == len(SURVEY_SHAPES[shape]) - 1
-    assert [_irs(flat, k)[0] for k in range(rounds + 1)] == SURVEY_SHAPES[shape]
+    assert rounds == len(CLI_DEFAULT_SHAPES[shape]) - 1
+    assert [_irs(flat, k)[0] for k in range(rounds + 1)] == CLI_DEFAULT_SHAPES[shape]
 
 
 @pytest.mark.skipif("not __import__('oracle').ref()")
