@@ -33,7 +33,7 @@ Beside the headline line's workload it measures, in the same JSON line:
     measured plain-path rate) with p50/p99 latency, and the plain
     one-at-a-time GPU path on the same requests.
 
-    python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--no-cpu] [--legs commit,prove,prove_sha2,batched,served]
+    python bench.py [--gpus 1] [--steps 30] [--warmup 3] [--no-cpu] [--legs commit,prove,prove_sha2,batched,served]
 """
 from __future__ import annotations
 
@@ -499,8 +499,8 @@ def commit_leg(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--legs", default="commit,prove,prove_sha2,batched,served")
     ap.add_argument("--requests", type=int, default=2000, help="served requests")
