@@ -86,7 +86,7 @@ struct whir_ctx {
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
     std::recursive_mutex mu;
-    whir_gpu::DevBuf scratch, nttbuf, partials, hin, hout, hnodes, hidx, hcomp;
+    whir_gpu::DevBuf scratch, nttbuf, partials, hin, hout, hnodes, hidx, hcomp, tbuf;
     // prove loop: PoW words, term descriptors and tables, opened rows, vector, covector
     whir_gpu::DevBuf powbuf, tdesc, ttab;
     int pow_slot = 0;  // early-exit slot set of the next PoW launch
@@ -172,7 +172,16 @@ struct whir_ctx {
     void* pow_hook_arg = nullptr;
     // waits for all work enqueued on `stream` (through the mapped signal word)
     whir_gpu::StreamSignal sig;
+    // Failure injection for tests of the callers' error paths: when nonzero,
+    // the fail_at_wait-th wait() from now returns an error without waiting,
+    // so the work enqueued since the previous wait is still in flight when
+    // the caller unwinds.  Host-side only; no kernel is affected.
+    uint32_t fail_at_wait = 0;
     int wait() {
+        if (fail_at_wait && --fail_at_wait == 0) {
+            whir_gpu::last_error() = "injected failure at a device wait";
+            return -1;
+        }
         if (int rc = whir_gpu::signal_init(sig)) return rc;
         return whir_gpu::signal_wait(sig, stream);
     }

@@ -94,6 +94,7 @@ struct Request {
     bool have_evals = false;
     size_t footprint = 0, ws_bytes = 0, coeff_bytes = 0, arena_off = 0;
     size_t skipped = 0;
+    uint32_t fail_at_wait = 0;  // injected failure (whir_server_inject_failure), 0 = none
     bool done = false;
     int status = 0;
     std::string error;
@@ -258,6 +259,7 @@ struct whir_server {
     std::condition_variable cv_work, cv_done;
     std::deque<Request*> queue;
     std::map<uint64_t, std::unique_ptr<Request>> reqs;
+    std::map<uint64_t, uint32_t> inject;  // request id -> device wait that fails
     PowBatcher pow;
     uint64_t next_id = 1;
     size_t completed = 0, running = 0;
@@ -291,6 +293,19 @@ struct whir_server {
 };
 
 void whir_server::run(Slot& s, Request& r) {
+    // Every exit waits for the slot's stream first: a request that fails part
+    // way may still have copies and kernels queued there that read or write
+    // its arena block and read the slot's pinned staging buffer (or the
+    // caller's borrowed buffer).  Only then may the worker release the block,
+    // the caller free a borrowed buffer, and the next request reuse the slot.
+    struct Drain {
+        whir_ctx* ctx;
+        ~Drain() {
+            (void)hipStreamSynchronize(ctx->stream);
+            ctx->fail_at_wait = 0;
+        }
+    } drain{s.ctx};
+    s.ctx->fail_at_wait = r.fail_at_wait;
     auto fail_with = [&r](int rc) {
         r.status = rc;
         r.error = last_error();
@@ -352,6 +367,11 @@ void whir_server::worker(Slot* s) {
             std::unique_lock<std::mutex> lk(mu);
             cv_work.wait(lk, [&] { return stop || (r = pick()) != nullptr; });
             if (!r) return;  // stopping with nothing admissible
+            auto inj = inject.find(r->id);
+            if (inj != inject.end()) {
+                r->fail_at_wait = inj->second;
+                inject.erase(inj);
+            }
             r->t_admit = Clock::now();
             in_use += r->footprint;
             peak = std::max(peak, in_use);
@@ -501,6 +521,16 @@ int whir_server_wait(whir_server* S, uint64_t id, uint8_t* narg, size_t narg_cap
     if (narg) std::memcpy(narg, r->narg.data(), r->narg.size());
     if (hints) std::memcpy(hints, r->hints.data(), r->hints.size());
     if (narg || hints) S->reqs.erase(it);
+    return 0;
+}
+
+int whir_server_inject_failure(whir_server* S, uint64_t id, uint32_t at_wait) {
+    if (!S) return bad_arg("null server");
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (at_wait)
+        S->inject[id] = at_wait;
+    else
+        S->inject.erase(id);
     return 0;
 }
 
