@@ -92,6 +92,17 @@ int whir_gpu_ntt_batch(uint64_t* values, size_t total, size_t size);
 /* NttEngine::intt_batch (cooley_tukey.hpp:767-776): inverse without 1/N. */
 int whir_gpu_intt_batch(uint64_t* values, size_t total, size_t size);
 
+/* gpu_ntt_batch_transpose, cuda_integration.hpp:388-430 (the GPU branch of
+ * NttEngine::try_gpu_ntt_batch_transpose, cooley_tukey.hpp:262-307, which
+ * ark_ntt calls on the zero-padded interleaved matrix, mod_ntt.hpp:98):
+ * forward NTT of every contiguous block of ntt_size in values[0..total),
+ * then the rows x cols row-major result transposed in place of values
+ * (values[c * rows + r] = NTT'd[r * cols + c]).  total = rows * cols, a
+ * multiple of ntt_size (a power of two <= 2^26).  The reference's `roots`
+ * argument, which its implementation ignores, is dropped.  Returns 1 below
+ * the NTT threshold or with dispatch disabled, as the reference's try_ does. */
+int whir_gpu_ntt_batch_transpose(uint64_t* values, size_t total, size_t ntt_size, size_t rows, size_t cols);
+
 /* gpu_interleaved_rs_encode, cuda_integration.hpp:432-485 (and the CPU
  * interleaved_rs_encode, mod_ntt.hpp:133-141): coeffs num_polys x poly_size,
  * out row-major codeword_length x (num_polys * interleaving_depth). */
@@ -180,6 +191,10 @@ int whir_dev_rs_encode_ext3(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_
 /* In-place forward (inverse != 0: inverse, unscaled) NTT of blocks of size
  * (a power of two <= 2^26). */
 int whir_dev_ntt_batch(whir_ctx* ctx, uint64_t* d_values, size_t total, size_t size, int inverse);
+/* whir_gpu_ntt_batch_transpose on device buffers, out of place: d_values is
+ * not modified and must not overlap d_out (rows * cols values each). */
+int whir_dev_ntt_batch_transpose(whir_ctx* ctx, const uint64_t* d_values, uint64_t* d_out, size_t total,
+                                 size_t ntt_size, size_t rows, size_t cols);
 /* d_leaves[i] = BLAKE3 of row i of a rows x row_u64 u64 matrix. */
 int whir_dev_commit_leaves(whir_ctx* ctx, const uint64_t* d_matrix, size_t rows, size_t row_u64, uint8_t* d_leaves);
 int whir_dev_blake3_hash_many(whir_ctx* ctx, size_t size, const uint8_t* d_input, size_t count, uint8_t* d_output);
@@ -423,6 +438,15 @@ int whir_server_wait(whir_server* server, uint64_t id, uint8_t* narg, size_t nar
 /* out[9]: completed, running, queued, arena bytes in use, peak bytes in use,
  * budget, slots, batched proof-of-work launches, searches they ran. */
 int whir_server_stats(whir_server* server, uint64_t* out);
+/* Test hook for the serving loop's error path (no counterpart in the
+ * reference, whose GPU layer has no serving loop): request `id` (one that
+ * whir_server_submit has returned or will return) fails at its at_wait-th
+ * device wait (1 = the first) with status -1 and the message "injected
+ * failure at a device wait", leaving the copies and kernels it enqueued since
+ * its previous wait in flight.  The loop then drains the slot's stream before
+ * it releases the request's HBM block.  Takes effect only for a request not
+ * yet admitted; at_wait = 0 clears it.  Host-side only: no kernel faults. */
+int whir_server_inject_failure(whir_server* server, uint64_t id, uint32_t at_wait);
 
 /* --------------------------------------------------------- stage timing */
 

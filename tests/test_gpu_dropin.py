@@ -1,0 +1,109 @@
+"""The host-pointer drop-ins that the reference's NTT engine calls, at its
+own call sites: gpu_ntt_batch_transpose (cuda_integration.hpp:388-430), which
+ark_ntt offers the zero-padded interleaved matrix (mod_ntt.hpp:77-98), against
+the reference build's interleaved_rs_encode / ntt_batch and the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+import whirplonk_amd as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _padded(coeffs, num_polys, poly_size, cw, depth):
+    """ark_ntt's step 1 (mod_ntt.hpp:77-93): block b of poly k at
+    k*cw*depth + b*cw, its m coefficients first, zeros after."""
+    m = poly_size // depth
+    out = np.zeros(num_polys * depth * cw, dtype=np.uint64)
+    for k in range(num_polys):
+        for b in range(depth):
+            dst = (k * depth + b) * cw
+            out[dst:dst + m] = coeffs[k * poly_size + b * m:k * poly_size + (b + 1) * m]
+    return out
+
+
+def _rs_ref(coeffs, num_polys, poly_size, cw, depth):
+    R = oracle.ref()
+    if R is None:
+        return oracle.rs_encode(coeffs, num_polys, poly_size, cw, depth)
+    out = np.zeros(cw * num_polys * depth, dtype=np.uint64)
+    R.ref_rs_encode(oracle._ptr64(coeffs), num_polys, poly_size, cw, depth, oracle._ptr64(out))
+    return out
+
+
+@pytest.mark.parametrize("num_polys,poly_size,cw,depth", [
+    (1, 1 << 16, 1 << 14, 16), (3, 1 << 14, 1 << 12, 16), (2, 1 << 12, 1 << 13, 4), (1, 1 << 21, 1 << 21, 16)])
+def test_ntt_batch_transpose_ark_ntt_call_site(num_polys, poly_size, cw, depth):
+    """values = ark_ntt's padded matrix, ntt_size = cols = codeword_length,
+    rows = num_polys * depth: the result is the interleaved RS codeword."""
+    coeffs, _ = W.lcg_uniform(poly_size + cw + num_polys, num_polys * poly_size)
+    vals = _padded(coeffs, num_polys, poly_size, cw, depth)
+    print("ntt transpose call site", num_polys, poly_size, cw, depth, flush=True)
+    got = W.ntt_batch_transpose(vals, cw, num_polys * depth, cw)
+    assert np.array_equal(got, _rs_ref(coeffs, num_polys, poly_size, cw, depth))
+
+
+@pytest.mark.parametrize("total,ntt_size,rows,cols", [
+    (1 << 16, 1 << 10, 1 << 5, 1 << 11), (3 << 15, 1 << 10, 3, 1 << 15), (48 << 12, 1 << 12, 48, 1 << 12),
+    (1 << 22, 1 << 21, 2, 1 << 21), (1 << 17, 1 << 17, 1 << 7, 1 << 10)])
+def test_ntt_batch_transpose_shapes(total, ntt_size, rows, cols):
+    """Any rows x cols split of the NTT'd blocks, as the reference's
+    transpose_kernel (cuda_ntt.cu:149-162) allows: the blocks' forward NTT
+    (the reference build's ntt_batch, or the oracle), then the transpose."""
+    vals, _ = W.lcg_uniform(total + rows, total)
+    R = oracle.ref()
+    if R is not None:
+        want = vals.copy()
+        R.ref_ntt_batch(oracle._ptr64(want), total, ntt_size)
+    else:
+        want = oracle.ntt_batch(vals, ntt_size)
+    want = want.reshape(rows, cols).T.reshape(-1)
+    print("ntt transpose", total, ntt_size, rows, cols, flush=True)
+    assert np.array_equal(W.ntt_batch_transpose(vals, ntt_size, rows, cols), want)
+    # device form: out of place, input untouched
+    with W.Context() as ctx:
+        d_in = ctx.upload(vals)
+        d_out = ctx.alloc(8 * total)
+        ctx.ntt_batch_transpose(d_in, d_out, total, ntt_size, rows, cols)
+        assert np.array_equal(ctx.download(d_out, total), want)
+        assert np.array_equal(ctx.download(d_in, total), vals)
+
+
+def test_ntt_batch_transpose_gates():
+    lib = W.load()
+    vals, _ = W.lcg_uniform(9, 1 << 12)
+    v = vals.copy()
+    # below the threshold (65536 elements): not handled, values untouched
+    assert lib.whir_gpu_ntt_batch_transpose(W._p64(v), v.size, 1 << 10, 4, 1 << 10) == 1
+    assert np.array_equal(v, vals)
+    t = lib.whir_gpu_ntt_threshold()
+    lib.whir_set_gpu_ntt_threshold(1 << 12)
+    try:
+        want = oracle.ntt_batch(vals, 1 << 10).reshape(4, 1 << 10).T.reshape(-1)
+        assert np.array_equal(W.ntt_batch_transpose(vals, 1 << 10, 4, 1 << 10), want)
+        # inconsistent shapes are refused
+        assert lib.whir_gpu_ntt_batch_transpose(W._p64(v), v.size, 1 << 10, 3, 1 << 10) > 0
+        assert lib.whir_gpu_ntt_batch_transpose(W._p64(v), v.size, 3 << 8, 4, 1 << 10) > 0
+    finally:
+        lib.whir_set_gpu_ntt_threshold(t)
+
+
+def test_integration_call_sites_compiled_against_reference():
+    """oracle/_ref/integration_check: every call-site edit of the integration
+    notes, compiled against the reference's own headers and types (std::span,
+    std::vector<Goldilocks>, Hash) and linked to libwhir_gpu.so, gives the
+    reference CPU function's result (ntt_batch, intt_batch, ark_ntt's
+    transpose, interleaved_rs_encode, the encode-and-leaves step in base and
+    Ext3 form, BLAKE3 and SHA-256 trees and open paths, OOD evaluation,
+    encode_into, hash_many, compute_sumcheck_polynomial and fold)."""
+    import os
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(oracle.__file__), "_ref", "integration_check")
+    if not os.path.exists(exe):
+        pytest.skip("integration driver not built (needs the reference headers)")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("ok  ") >= 18 and "FAIL" not in r.stdout

@@ -106,3 +106,80 @@ def test_request_larger_than_budget():
         c, p = _request(16, 1, 1)
         with pytest.raises(W.WhirGpuError):
             srv.submit(1 << 16, c, p)
+
+
+SERVE_SIZES, SERVE_WEIGHTS, SERVE_SEED = (16, 18, 20, 22, 24), (0.3, 0.3, 0.2, 0.15, 0.05), 0x5748495200000005
+
+
+def test_served_workload_mix():
+    """The served workload's mix at rate 1/4: 40 requests with num_variables
+    drawn from {16, 18, 20, 22, 24} (weights .3/.3/.2/.15/.05, seeded, at
+    least one 2^24 request), 16 slots, 3 MLE constraints each with the
+    evaluations computed on the device.  Every served proof equals the plain
+    one-at-a-time path's; the first request of each size equals the
+    reference prover's proof (whir_prover.hpp:42-400); every proof is
+    accepted by the reference verifier."""
+    rnd = random.Random(SERVE_SEED)
+    sizes = rnd.choices(SERVE_SIZES, weights=SERVE_WEIGHTS, k=40)
+    if 24 not in sizes:
+        sizes[-1] = 24
+    params = W.Params.default(starting_log_inv_rate=2)
+    inputs = [_request(nv, 3, SERVE_SEED + 97 * i) for i, nv in enumerate(sizes)]
+    print("served mix", sizes, flush=True)
+    ctx = W.Context()
+    alone = [_alone(ctx, params, nv, c, p) for nv, (c, p) in zip(sizes, inputs)]
+    ctx.close()
+    with W.Server(params, num_slots=16, hbm_budget=24 << 30) as srv:
+        ids = [srv.submit(1 << nv, c, p, borrow=(i % 2 == 0)) for i, (nv, (c, p)) in enumerate(zip(sizes, inputs))]
+        got = [srv.wait(i)[:2] for i in ids]
+        st = srv.stats()
+    assert st["completed"] == len(sizes) and st["in_use"] == 0
+    for i in range(len(sizes)):
+        assert got[i] == alone[i][0], (i, sizes[i])
+    if oracle.ref() is None:
+        pytest.skip("reference build not available")
+    for nv in sorted(set(sizes)):
+        i = sizes.index(nv)
+        print("served vs reference prover", nv, flush=True)
+        _, narg, hints = oracle.ref_prove(nv, inputs[i][0], inputs[i][1], rate=2)
+        assert got[i] == (narg, hints), nv
+    for i, nv in enumerate(sizes):
+        assert oracle.ref_verify(nv, inputs[i][1], alone[i][1], *got[i], rate=2), (i, nv)
+
+
+def test_failed_requests_are_isolated():
+    """Requests made to fail part-way (at their 1st and 5th device waits,
+    with copies and kernels still queued; one reads a borrowed buffer that
+    is a converted temporary) report the error; their HBM blocks go back to
+    the arena and are reused by later requests, and every other proof, before
+    and after, is byte-identical to the same request proved alone."""
+    params = W.Params.default(starting_log_inv_rate=2)
+    rnd = random.Random(0xFA11)
+    sizes = [rnd.choice([10, 12, 14]) for _ in range(14)]
+    inputs = [_request(nv, 2, 0xFA1100 + i) for i, nv in enumerate(sizes)]
+    ctx = W.Context()
+    alone = [_alone(ctx, params, nv, c, p) for nv, (c, p) in zip(sizes, inputs)]
+    ctx.close()
+    with W.Server(params, num_slots=1, hbm_budget=1 << 20) as probe:
+        big = probe.request_bytes(1 << 14)
+    with W.Server(params, num_slots=3, hbm_budget=2 * big + (1 << 20)) as srv:
+        fail = {3: 1, 6: 5, 11: 2}  # request id (1-based submit order) -> failing wait
+        for rid, at in fail.items():
+            srv.inject_failure(rid, at)
+        ids = []
+        for i, (nv, (c, p)) in enumerate(zip(sizes, inputs)):
+            src = [int(x) for x in c] if i + 1 == 6 else c  # id 6 borrows a list: a converted copy
+            ids.append(srv.submit(1 << nv, src, p, borrow=(i % 2 == 1 or i + 1 == 6)))
+        assert ids == list(range(1, len(sizes) + 1))
+        for i, rid in enumerate(ids):
+            if rid in fail:
+                with pytest.raises(W.WhirGpuError, match="injected failure"):
+                    srv.wait(rid)
+            else:
+                assert srv.wait(rid)[:2] == alone[i][0], i
+        st = srv.stats()
+        assert st["in_use"] == 0 and st["running"] == 0 and st["completed"] == len(sizes)
+        # the released blocks serve new requests; proofs unchanged
+        again = [srv.submit(1 << nv, c, p) for nv, (c, p) in zip(sizes[:6], inputs[:6])]
+        for i, rid in enumerate(again):
+            assert srv.wait(rid)[:2] == alone[i][0], ("again", i)

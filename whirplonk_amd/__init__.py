@@ -61,6 +61,7 @@ _SIGS = [
     ("whir_dev_copy", _int, [_vp, _vp, _vp, _sz]),
     ("whir_gpu_ntt_batch", _int, [_u64p, _sz, _sz]),
     ("whir_gpu_intt_batch", _int, [_u64p, _sz, _sz]),
+    ("whir_gpu_ntt_batch_transpose", _int, [_u64p, _sz, _sz, _sz, _sz]),
     ("whir_gpu_interleaved_rs_encode", _int, [_u64p, _u64p, _sz, _sz, _sz, _sz]),
     ("whir_gpu_interleaved_rs_encode_blake3_matrix_leaves", _int, [_u64p, _u64p, _u8p, _sz, _sz, _sz, _sz]),
     ("whir_gpu_interleaved_rs_encode_blake3_ext3_matrix_leaves", _int,
@@ -80,6 +81,7 @@ _SIGS = [
     ("whir_dev_rs_encode", _int, [_vp, _vp, _sz, _sz, _sz, _sz, _vp]),
     ("whir_dev_rs_encode_ext3", _int, [_vp, _vp, _sz, _sz, _sz, _sz, _vp]),
     ("whir_dev_ntt_batch", _int, [_vp, _vp, _sz, _sz, _int]),
+    ("whir_dev_ntt_batch_transpose", _int, [_vp, _vp, _vp, _sz, _sz, _sz, _sz]),
     ("whir_dev_commit_leaves", _int, [_vp, _vp, _sz, _sz, _vp]),
     ("whir_dev_blake3_hash_many", _int, [_vp, _sz, _vp, _sz, _vp]),
     ("whir_merkle_num_nodes", _sz, [_sz]),
@@ -137,6 +139,7 @@ _SIGS = [
     ("whir_server_destroy", _int, [_vp]),
     ("whir_server_request_bytes", _sz, [_vp, ctypes.c_uint64]),
     ("whir_server_submit", _int, [_vp, ctypes.c_uint64, _u64p, _int, _sz, _u64p, _u64p, _u64p]),
+    ("whir_server_inject_failure", _int, [_vp, ctypes.c_uint64, ctypes.c_uint32]),
     ("whir_host_malloc", _int, [_sz, ctypes.POINTER(_vp)]),
     ("whir_host_free", _int, [_vp]),
     ("whir_server_wait", _int, [_vp, ctypes.c_uint64, _u8p, _sz, ctypes.POINTER(_sz), _u8p, _sz, ctypes.POINTER(_sz),
@@ -196,6 +199,14 @@ def ntt_batch(values, size: int, inverse: bool = False) -> np.ndarray:
     v = _u64(values).copy()
     fn = load().whir_gpu_intt_batch if inverse else load().whir_gpu_ntt_batch
     _check(fn(_p64(v), v.size, size))
+    return v
+
+
+def ntt_batch_transpose(values, ntt_size: int, rows: int, cols: int) -> np.ndarray:
+    """gpu_ntt_batch_transpose: NTT of every ntt_size block, then the rows x
+    cols result transposed (cols x rows, row-major)."""
+    v = _u64(values).copy()
+    _check(load().whir_gpu_ntt_batch_transpose(_p64(v), v.size, ntt_size, rows, cols))
     return v
 
 
@@ -376,6 +387,9 @@ class Context:
 
     def ntt_batch(self, d_values, total, size, inverse=False):
         _check(load().whir_dev_ntt_batch(self.h, d_values, total, size, int(inverse)))
+
+    def ntt_batch_transpose(self, d_values, d_out, total, ntt_size, rows, cols):
+        _check(load().whir_dev_ntt_batch_transpose(self.h, d_values, d_out, total, ntt_size, rows, cols))
 
     def commit(self, d_coeffs, num_polys, poly_size, cw, depth, d_matrix, d_nodes, ext3=False) -> bytes:
         root = np.empty(32, dtype=np.uint8)
@@ -714,11 +728,15 @@ class Server:
         h = _vp()
         _check(load().whir_server_create(device, ctypes.byref(params), num_slots, hbm_budget, ctypes.byref(h)))
         self.h = h
+        # request id -> the array a borrowed request reads in place; held here
+        # until its wait() returns, since it may be a converted temporary
+        self._borrowed: dict[int, np.ndarray] = {}
 
     def close(self):
         if self.h:
-            _check(load().whir_server_destroy(self.h))
+            _check(load().whir_server_destroy(self.h))  # finishes every request first
             self.h = None
+            self._borrowed.clear()
 
     def __enter__(self):
         return self
@@ -730,7 +748,10 @@ class Server:
         return load().whir_server_request_bytes(self.h, size)
 
     def submit(self, size: int, coeffs, points=None, evaluations=None, borrow: bool = False) -> int:
-        """borrow=True reads coeffs in place (keep the array alive until wait)."""
+        """borrow=True lets the server read the coefficients in place, without
+        a copy at submit.  The array it reads (coeffs itself when it is already
+        a C-contiguous uint64 array, else the converted copy) is kept alive by
+        this object until the request's wait() returns."""
         c = _u64(coeffs)
         pts = _u64(points).reshape(-1) if points is not None and np.size(points) else None
         nf = pts.size // (3 * (size.bit_length() - 1)) if pts is not None else 0
@@ -738,13 +759,26 @@ class Server:
         out = np.zeros(1, dtype=np.uint64)
         _check(load().whir_server_submit(self.h, size, _p64(c), int(borrow), nf, _p64(pts) if nf else None,
                                          _p64(ev) if ev is not None else None, _p64(out)))
-        return int(out[0])
+        rid = int(out[0])
+        if borrow:
+            self._borrowed[rid] = c
+        return rid
+
+    def inject_failure(self, rid: int, at_wait: int) -> None:
+        """Test hook: request rid (submitted or not yet) fails at its
+        at_wait-th device wait, with work still in flight (0 clears)."""
+        _check(load().whir_server_inject_failure(self.h, rid, at_wait))
 
     def wait(self, rid: int):
         """(narg, hints, (queue_ms, service_ms, latency_ms))"""
         na, nh = _sz(), _sz()
         ms = (ctypes.c_double * 3)()
-        _check(load().whir_server_wait(self.h, rid, None, 0, ctypes.byref(na), None, 0, ctypes.byref(nh), ms))
+        try:
+            _check(load().whir_server_wait(self.h, rid, None, 0, ctypes.byref(na), None, 0, ctypes.byref(nh), ms))
+        except WhirGpuError:
+            self._borrowed.pop(rid, None)  # the request failed and was released
+            raise
+        self._borrowed.pop(rid, None)  # done: the server no longer reads the input
         a = np.zeros(max(na.value, 1), dtype=np.uint8)
         b = np.zeros(max(nh.value, 1), dtype=np.uint8)
         _check(load().whir_server_wait(self.h, rid, _p8(a), a.size, ctypes.byref(na), _p8(b), b.size,

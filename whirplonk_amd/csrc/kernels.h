@@ -40,6 +40,8 @@ struct LdeShape {
 // nonzero for an unsupported shape.
 int ntt_long_step1_launch(const uint64_t* in, uint64_t* g, uint64_t n, uint64_t m, uint64_t C, const Twiddles& tw,
                           bool inverse, hipStream_t st);
+// dst[c * rows + r] = src[r * cols + c] (u64; src and dst must not overlap)
+void transpose_launch(const uint64_t* src, uint64_t* dst, uint64_t rows, uint64_t cols, hipStream_t st);
 // Scratch needed by lde_launch (elements of u64); 0 when m <= 1024 (one
 // pass reads the input directly).
 size_t lde_scratch_elems(const LdeShape& s);

@@ -875,6 +875,41 @@ int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* s
     return 0;
 }
 
+namespace {
+// dst[c * rows + r] = src[r * cols + c] for a rows x cols u64 matrix: 32 x 32
+// tiles through LDS (one padding word per tile row keeps the column reads
+// conflict-free), 8-byte reads along source rows and writes along destination
+// rows.  Workgroup b walks the tiles b, b + gridDim.x, ...; ragged edges are
+// bounds-checked.
+constexpr int TT = 32;
+__global__ __launch_bounds__(256) void transpose_kernel(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst,
+                                                        uint64_t rows, uint64_t cols, uint64_t tiles_c,
+                                                        uint64_t ntiles) {
+    __shared__ uint64_t tile[TT][TT + 1];
+    const uint32_t tx = threadIdx.x % TT, ty = threadIdx.x / TT;  // 32 x 8 threads
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint64_t r0 = (t / tiles_c) * TT, c0 = (t % tiles_c) * TT;
+        for (uint32_t k = ty; k < TT; k += 8) {
+            const uint64_t r = r0 + k, c = c0 + tx;
+            if (r < rows && c < cols) tile[k][tx] = src[r * cols + c];
+        }
+        __syncthreads();
+        for (uint32_t k = ty; k < TT; k += 8) {
+            const uint64_t c = c0 + k, r = r0 + tx;  // destination row c, column r
+            if (r < rows && c < cols) dst[c * rows + r] = tile[tx][k];
+        }
+        __syncthreads();  // the next tile rewrites the LDS tile
+    }
+}
+}  // namespace
+
+void transpose_launch(const uint64_t* src, uint64_t* dst, uint64_t rows, uint64_t cols, hipStream_t st) {
+    if (!rows || !cols) return;
+    const uint64_t tiles_c = (cols + TT - 1) / TT, ntiles = ((rows + TT - 1) / TT) * tiles_c;
+    const unsigned grid = (unsigned)std::min<uint64_t>(ntiles, 16384);
+    hipLaunchKernelGGL(transpose_kernel, dim3(grid), dim3(256), 0, st, src, dst, rows, cols, tiles_c, ntiles);
+}
+
 int ntt_long_step1_launch(const uint64_t* in, uint64_t* g, uint64_t n, uint64_t m, uint64_t C, const Twiddles& tw,
                           bool inverse, hipStream_t st) {
     using F = void (*)(const uint64_t*, uint64_t*, uint32_t, uint32_t, uint64_t, const uint64_t*, const uint64_t*);

@@ -395,7 +395,7 @@ int whir_ctx_destroy(whir_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (DevBuf* b : {&ctx->scratch, &ctx->nttbuf, &ctx->partials, &ctx->hin, &ctx->hout, &ctx->hnodes, &ctx->hidx,
-                      &ctx->hcomp, &ctx->powbuf, &ctx->tdesc, &ctx->ttab})
+                      &ctx->hcomp, &ctx->powbuf, &ctx->tdesc, &ctx->ttab, &ctx->tbuf})
         if (b->p) (void)hipFree(b->p);
     for (auto& kv : ctx->tw) {
         (void)hipFree(kv.second.lo);
@@ -507,6 +507,47 @@ int whir_dev_ntt_batch(whir_ctx* ctx, uint64_t* d_values, size_t total, size_t s
     s.row_major = false;
     s.inverse = inverse != 0;
     return run_lde(ctx, s, d_values, d_values);
+}
+
+namespace {
+// NTT of every ntt_size block of `in` (total = rows * cols values), then the
+// rows x cols result transposed into out.  When cols == ntt_size the LDE
+// writes each block's transform straight into column `block` of the
+// row-major output (one launch, no transpose pass); otherwise the blocks are
+// transformed in place in `work` (in itself when clobber_in, else a copy in
+// ctx->tbuf) and transposed out of it.
+int ntt_transpose(whir_ctx* ctx, uint64_t* in, bool clobber_in, uint64_t* out, size_t total, size_t ntt_size,
+                  size_t rows, size_t cols) {
+    if (ntt_size == 0 || rows == 0 || cols == 0) return bad_arg("empty NTT transpose shape");
+    if (rows * cols != total || total % ntt_size) return bad_arg("total must be rows * cols and a multiple of ntt_size");
+    if (!is_pow2(ntt_size) || ntt_size > (1ull << 26)) return bad_arg("NTT size must be a power of two <= 2^26");
+    if (cols == ntt_size && ntt_size > 1 && ntt_size <= (1ull << 20) && total < (1ull << 31)) {
+        LdeShape s;
+        s.C = rows;
+        s.m = ntt_size;
+        s.n = ntt_size;
+        s.row_major = true;
+        return run_lde(ctx, s, in, out);
+    }
+    uint64_t* work = in;
+    if (!clobber_in) {
+        int rc = ctx->ensure(ctx->tbuf, total * 8);
+        if (rc) return rc;
+        work = (uint64_t*)ctx->tbuf.p;
+        HIPCHK(hipMemcpyAsync(work, in, total * 8, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    if (int rc = whir_dev_ntt_batch(ctx, work, total, ntt_size, 0)) return rc;
+    transpose_launch(work, out, rows, cols, ctx->stream);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+}  // namespace
+
+int whir_dev_ntt_batch_transpose(whir_ctx* ctx, const uint64_t* d_values, uint64_t* d_out, size_t total,
+                                 size_t ntt_size, size_t rows, size_t cols) {
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (total == 0 && rows * cols == 0) return 0;
+    return ntt_transpose(ctx, const_cast<uint64_t*>(d_values), false, d_out, total, ntt_size, rows, cols);
 }
 
 int whir_dev_commit_leaves(whir_ctx* ctx, const uint64_t* d_matrix, size_t rows, size_t row_u64, uint8_t* d_leaves) {
@@ -679,6 +720,28 @@ static int ntt_host(uint64_t* values, size_t total, size_t size, int inverse) {
 }
 
 int whir_gpu_ntt_batch(uint64_t* values, size_t total, size_t size) { return ntt_host(values, total, size, 0); }
+
+int whir_gpu_ntt_batch_transpose(uint64_t* values, size_t total, size_t ntt_size, size_t rows, size_t cols) {
+    {
+        std::lock_guard<std::mutex> g(g_flags_mu);
+        if (total < g_threshold) {
+            whir_gpu::last_error() = "below the GPU NTT threshold";
+            return 1;
+        }
+    }
+    return with_default([&](whir_ctx* ctx) {
+        int rc;
+        if ((rc = ctx->ensure(ctx->hin, total * 8))) return rc;
+        if ((rc = ctx->ensure(ctx->hout, total * 8))) return rc;
+        uint64_t* din = (uint64_t*)ctx->hin.p;
+        uint64_t* dout = (uint64_t*)ctx->hout.p;
+        HIPCHK(hipMemcpyAsync(din, values, total * 8, hipMemcpyHostToDevice, ctx->stream));
+        if ((rc = ntt_transpose(ctx, din, true, dout, total, ntt_size, rows, cols))) return rc;
+        HIPCHK(hipMemcpyAsync(values, dout, total * 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        return 0;
+    });
+}
 int whir_gpu_intt_batch(uint64_t* values, size_t total, size_t size) { return ntt_host(values, total, size, 1); }
 
 static int rs_host(const uint64_t* coeffs, uint64_t* out_matrix, uint8_t* out_hashes, size_t num_polys,
