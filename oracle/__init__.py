@@ -38,6 +38,7 @@ def build(ref: bool = False) -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
     if ref and os.path.isdir("/root/reference/cpp/include"):
         subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+        subprocess.run(["make", "-s", "-C", HERE, "integration"], check=True)
 
 
 def _ptr64(a: np.ndarray):
@@ -124,6 +125,10 @@ def ref():
         R.ref_sumcheck_compute_ext3.argtypes = [_u64p, _u64p, _sz, _u64p, _u64p]
         R.ref_sumcheck_fold_ext3.argtypes = [_u64p, _sz, _u64p]
         R.ref_horner_base_ext3.argtypes = [_u64p, _sz, _u64p, _u64p]
+        R.ref_horner_ext3_ext3.argtypes = [_u64p, _sz, _u64p, _u64p]
+        R.ref_pow_find.restype = _u64
+        R.ref_pow_find.argtypes = [ctypes.c_uint32, _u8p, _u64]
+        R.ref_e3_pow.argtypes = [_u64p, _u64, _u64p]
         R.ref_commit_base.argtypes = [_u64p, _sz, _sz, _sz, _u64p, _u8p]
         R.ref_blake3_many.argtypes = [_sz, _u8p, _sz, _u8p]
         R.ref_whir_config.restype = _sz

@@ -14,6 +14,9 @@
 //   ref_merkle_open        protocols/merkle_tree.hpp:165-211 open_path
 //   ref_sumcheck_*         algebra/sumcheck.hpp:44-116
 //   ref_horner_base_ext3   algebra/utilities.hpp:195-246 mixed_univariate_evaluate
+//   ref_horner_ext3_ext3   the same over Ext3 coefficients (Identity embedding)
+//   ref_pow_find           protocols/proof_of_work.hpp:95-193 find_nonce, BLAKE3 or SHA-256
+//   ref_e3_pow             GoldilocksExt3::pow (algebra/goldilocks_ext3.hpp)
 //   ref_commit_base        the irs_commit.hpp:344-460 sequence (encode, leaves, tree)
 //                          without the transcript
 //   ref_blake3_many        hash/blake3_engine.hpp:90-199 Blake3::hash_many (both branches)
@@ -36,6 +39,7 @@
 #include "whir/hash/sha2_engine.hpp"
 #include "whir/protocols/matrix_commit.hpp"
 #include "whir/protocols/merkle_tree.hpp"
+#include "whir/protocols/proof_of_work.hpp"
 #include "whir/protocols/whir/whir.hpp"
 #include "whir/transcript/transcript.hpp"
 
@@ -213,6 +217,27 @@ void ref_horner_base_ext3(const uint64_t* coeffs, size_t n, const uint64_t z[3],
     auto r = whir::algebra::mixed_univariate_evaluate<whir::algebra::Basefield<GoldilocksExt3>>(
         emb, std::span<const Goldilocks>{reinterpret_cast<const Goldilocks*>(coeffs), n}, p);
     from_e3(r, out);
+}
+
+void ref_horner_ext3_ext3(const uint64_t* coeffs, size_t n, const uint64_t z[3], uint64_t out[3]) {
+    whir::algebra::Identity<GoldilocksExt3> emb;
+    GoldilocksExt3 p(Goldilocks::from_u64(z[0]), Goldilocks::from_u64(z[1]), Goldilocks::from_u64(z[2]));
+    auto c = to_e3(coeffs, n);
+    auto r = whir::algebra::mixed_univariate_evaluate<whir::algebra::Identity<GoldilocksExt3>>(
+        emb, std::span<const GoldilocksExt3>{c}, p);
+    from_e3(r, out);
+}
+
+uint64_t ref_pow_find(uint32_t hash_id, const uint8_t* challenge, uint64_t threshold) {
+    std::array<std::uint8_t, 32> ch;
+    std::memcpy(ch.data(), challenge, 32);
+    if (hash_id == 2) return whir::protocols::pow::find_nonce(sha2(), ch, threshold);
+    return whir::protocols::pow::find_nonce(blake3(), ch, threshold);
+}
+
+void ref_e3_pow(const uint64_t a[3], uint64_t e, uint64_t out[3]) {
+    GoldilocksExt3 x(Goldilocks::from_u64(a[0]), Goldilocks::from_u64(a[1]), Goldilocks::from_u64(a[2]));
+    from_e3(x.pow(e), out);
 }
 
 void ref_blake3_many(size_t size, const uint8_t* input, size_t count, uint8_t* out) {

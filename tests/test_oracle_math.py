@@ -216,3 +216,81 @@ class TestAgainstReferenceBuild:
         out = np.zeros(3, dtype=np.uint64)
         R.ref_horner_base_ext3(orc._ptr64(b[:n]), n, orc._ptr64(z), orc._ptr64(out))
         assert np.array_equal(out, orc.horner_base_ext3(b[:n], z))
+
+
+def _e3pow_big(a, e):
+    # square-and-multiply over big-int polynomial products mod X^3 - 2
+    r, x = [1, 0, 0], list(a)
+    while e:
+        if e & 1:
+            r = _e3mul_big(r, x)
+        x = _e3mul_big(x, x)
+        e >>= 1
+    return r
+
+
+def test_ext3_pow_bigint(orc):
+    """or_e3_pow against big-int square-and-multiply (exponents 0, 1, 2, the
+    multiplicative group order minus one, so a^-1, and random 64-bit ones),
+    and against GoldilocksExt3::pow of the reference build when present."""
+    rnd = random.Random(4)
+    order = P**3 - 1
+    R = orc.ref()
+    for _ in range(60):
+        a = [rnd.choice(SPECIAL + [rnd.randrange(P)]) for _ in range(3)]
+        for e in (0, 1, 2, 3, P - 1, P, 2**64 - 1, rnd.randrange(2**64)):
+            assert list(orc.e3_pow(a, e)) == _e3pow_big(a, e), (a, e)
+            if R is not None:
+                got = np.zeros(3, dtype=np.uint64)
+                R.ref_e3_pow(orc._ptr64(orc.u64(a)), e, orc._ptr64(got))
+                assert list(orc.e3_pow(a, e)) == [int(x) for x in got], (a, e)
+        if any(a):
+            # a^(p^3 - 2) = a^-1 (the exponent exceeds 64 bits: two steps)
+            inv = _e3pow_big(a, order - 1)
+            assert _e3mul_big(a, inv) == [1, 0, 0]
+            assert list(orc.e3_inv(a)) == inv
+
+
+def test_horner_ext3_vs_reference(orc):
+    """or_horner_ext3_ext3 against the reference's mixed_univariate_evaluate
+    over Ext3 coefficients (Identity embedding, utilities.hpp:195-246), both
+    its serial and its chunked OpenMP branch (>= 4096 coefficients), and
+    against a big-int sum of c_i z^i on small inputs."""
+    R = orc.ref()
+    rnd = random.Random(5)
+    for n in (1, 2, 3, 7, 64, 1000):
+        c = [rnd.randrange(P) for _ in range(3 * n)]
+        z = [rnd.randrange(P) for _ in range(3)]
+        want, zi = [0, 0, 0], [1, 0, 0]
+        for i in range(n):
+            t = _e3mul_big(c[3 * i:3 * i + 3], zi)
+            want = [(want[k] + t[k]) % P for k in range(3)]
+            zi = _e3mul_big(zi, z)
+        assert list(orc.horner_ext3_ext3(c, z)) == want, n
+    if R is None:
+        pytest.skip("reference build not available")
+    for n in (1, 5, 4095, 4096, 4097, 20000):
+        c = orc.u64([rnd.randrange(P) for _ in range(3 * n)])
+        z = orc.u64([rnd.randrange(P) for _ in range(3)])
+        got = np.zeros(3, dtype=np.uint64)
+        R.ref_horner_ext3_ext3(orc._ptr64(c), n, orc._ptr64(z), orc._ptr64(got))
+        assert np.array_equal(orc.horner_ext3_ext3(c, z), got), n
+
+
+def test_pow_find_vs_reference(orc):
+    """The serial PoW scans (BLAKE3 in C, SHA-256 with hashlib) against the
+    reference's find_nonce (proof_of_work.hpp:95-193) with either engine, at
+    thresholds from zero difficulty to about 12 bits."""
+    R = orc.ref()
+    if R is None:
+        pytest.skip("reference build not available")
+    rnd = random.Random(6)
+    for trial in range(6):
+        ch = bytes(rnd.randrange(256) for _ in range(32))
+        for bits in (0, 3, 8, 12):
+            thr = 2**64 - 1 if bits == 0 else int(2 ** (64 - bits))
+            chb = np.frombuffer(ch, dtype=np.uint8).copy()
+            want_b3 = int(R.ref_pow_find(1, orc._ptr8(chb), thr))
+            want_sha = int(R.ref_pow_find(2, orc._ptr8(chb), thr))
+            assert orc.pow_find(ch, thr) == want_b3, (trial, bits)
+            assert orc.sha256_pow_find(ch, thr) == want_sha, (trial, bits)
