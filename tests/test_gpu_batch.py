@@ -135,3 +135,42 @@ def test_every_split_same_codeword(ctx, m, n, ext3):
     assert np.array_equal(ctx.download(d_out, n * depth * (3 if ext3 else 1)), want)
     ctx.free(d_in)
     ctx.free(d_out)
+
+
+BATCH_SEED = 0x5748495200000003  # the batched-commit workload's seed (bench.py uses the same)
+
+
+def test_batch_workload_64_polys(ctx):
+    """The batched-commit workload itself: 64 polynomials with num_variables
+    uniform on [14, 22] from the seed (about 2^25.6 coefficients), rate 1/4,
+    folding factor 4, every shape autotuned.  All 64 roots equal the
+    reference build's commit of each polynomial (irs_commit.hpp:344-460), on
+    the eager run and on three graph replays; the largest polynomial's
+    codeword equals the reference's interleaved_rs_encode, and a 2^16 one's
+    the oracle's."""
+    rnd = random.Random(BATCH_SEED)
+    sizes = [1 << rnd.randint(14, 22) for _ in range(64)]
+    polys = _polys(sizes, BATCH_SEED)
+    print("batch workload", sum(sizes), "coefficients", flush=True)
+    want = [_commit_ref(c) for c in polys]
+    with W.Batch(ctx, sizes, log_inv_rate=2, folding_factor=4, branches=4, autotune=True) as b:
+        for k, c in enumerate(polys):
+            b.upload(k, c)
+        b.run(graph=False)
+        assert b.roots() == want
+        for rep in range(3):
+            b.run(graph=True)
+            assert b.roots() == want, rep
+        big = max(range(64), key=lambda k: sizes[k])
+        n, cw = sizes[big], (sizes[big] // 16) << 2
+        R = oracle.ref()
+        if R is not None:
+            ref_m = np.zeros(cw * 16, dtype=np.uint64)
+            R.ref_rs_encode(oracle._ptr64(polys[big]), 1, n, cw, 16, oracle._ptr64(ref_m))
+        else:
+            ref_m = oracle.rs_encode(polys[big], 1, n, cw, 16)
+        assert np.array_equal(b.matrix(big), ref_m)
+        mid = next((k for k in range(64) if sizes[k] == 1 << 16), None)
+        if mid is not None:
+            cw = (1 << 16) // 16 << 2
+            assert np.array_equal(b.matrix(mid), oracle.rs_encode(polys[mid], 1, 1 << 16, cw, 16))
