@@ -233,6 +233,25 @@ int whir_dev_sumcheck_round(whir_ctx* ctx, const uint64_t* d_a, int a_is_base, c
                             const uint64_t* r, uint64_t* d_a_out, uint64_t* d_b_out, uint64_t* out_c0,
                             uint64_t* out_c2);
 
+/* whir_dev_sumcheck_round that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device or pinned memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The per-workgroup (c0, c2) partials go to a file-scope device
+ * array of WHIR_NOALLOC_SCRATCH_BYTES that the library holds on every GPU it
+ * runs on until the process ends (its device address is looked up when a
+ * context is created on that GPU), and c0, c2 are left on the device in
+ * d_c0c2[6] (c0 then c2, Ext3 AoS) instead of being copied to the host.  The
+ * call is stream-ordered on ctx's stream and does not synchronize; the folds
+ * are as in whir_dev_sumcheck_round.  n is at most WHIR_NOALLOC_SUMCHECK_MAX_N
+ * (larger n is refused with status 3).  Every call on one GPU shares that
+ * GPU's scratch, which keeps the last call's partials (each call overwrites
+ * all it reads): calls that overlap on one GPU, from different streams or
+ * threads, race. */
+#define WHIR_NOALLOC_SCRATCH_BYTES 98304u
+#define WHIR_NOALLOC_SUMCHECK_MAX_N (1ull << 32)
+int whir_dev_sumcheck_round_noalloc(whir_ctx* ctx, const uint64_t* d_a, int a_is_base, const uint64_t* d_b, size_t n,
+                                    const uint64_t* r, uint64_t* d_a_out, uint64_t* d_b_out, uint64_t* d_c0c2);
+
 /* ------------------------------------------------------------ prove loop */
 
 /* ProtocolParameters (parameters.hpp:25-34).  hash_id selects the Merkle,

@@ -93,6 +93,7 @@ _SIGS = [
     ("whir_dev_sumcheck_fold", _int, [_vp, _vp, _int, _sz, _u64p, _vp]),
     ("whir_dev_horner", _int, [_vp, _vp, _int, _sz, _u64p, _u64p]),
     ("whir_dev_sumcheck_round", _int, [_vp, _vp, _int, _vp, _sz, _u64p, _vp, _vp, _u64p, _u64p]),
+    ("whir_dev_sumcheck_round_noalloc", _int, [_vp, _vp, _int, _vp, _sz, _u64p, _vp, _vp, _vp]),
     ("whir_params_default", None, [_vp]),
     ("whir_config_flat", _sz, [ctypes.c_uint64, _vp, _u64p, _sz]),
     ("whir_domain_separator_cli", _int, [_vp, ctypes.c_char_p, _u8p, _u8p]),

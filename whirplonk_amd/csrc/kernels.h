@@ -132,6 +132,10 @@ void fill_u64_launch(uint64_t* p, uint64_t n, uint64_t v, hipStream_t st);
 // of the stream, with a system-scope release.
 void signal_launch(uint64_t* flag_d, uint64_t seq, hipStream_t st);
 
+// Looks up, with `device` current, the device address of the file-scope
+// scratch array of whir_dev_sumcheck_round_noalloc (once per device).
+int noalloc_scratch_init(int device);
+
 // Weight terms sum_t s_t w_t(i) over i < 2^nv, factored as hi_t[i >> L] * lo_t[i & (2^L-1)].
 // Descriptor of a term, TERM_WORDS u64: kind (0 eq, 1 power), s (Ext3),
 // z (Ext3, power terms), z^(2^L) (Ext3, power terms), offset of the eq point

@@ -347,7 +347,7 @@ unsigned mle_eval_launch(const uint64_t* f, bool f_base, uint64_t n, uint32_t L,
                          const uint64_t* lo, uint64_t* partials, hipStream_t st, bool device_sum) {
     unsigned g = std::min<unsigned>(blocks_of(n), SUMCHECK_MAX_BLOCKS);
     hipLaunchKernelGGL(mle_eval_kernel, dim3(g), dim3(256), 0, st, f, f_base ? 1 : 0, n, L, hi, lo, partials);
-    if (device_sum) hipLaunchKernelGGL(red::sum_blocks_kernel<1>, dim3(1), dim3(256), 0, st, partials, g);
+    if (device_sum) hipLaunchKernelGGL(red::sum_blocks_kernel<1>, dim3(1), dim3(256), 0, st, partials, g, partials);
     return g;
 }
 
@@ -357,7 +357,7 @@ unsigned fold_compute_launch(const uint64_t* a, bool a_base, const uint64_t* b, 
     e3 rr = {r[0], r[1], r[2]};
     hipLaunchKernelGGL(fold_compute_kernel, dim3(g), dim3(256), 0, st, a, a_base ? 1 : 0, b, n / 4, rr, a_out, b_out,
                        partials);
-    if (device_sum) hipLaunchKernelGGL(red::sum_blocks_kernel<2>, dim3(1), dim3(256), 0, st, partials, g);
+    if (device_sum) hipLaunchKernelGGL(red::sum_blocks_kernel<2>, dim3(1), dim3(256), 0, st, partials, g, partials);
     return g;
 }
 

@@ -61,12 +61,13 @@ __device__ __forceinline__ void store_block_totals(const e3* acc, uint64_t* part
     }
 }
 
-// One workgroup: sums the NV-tuples of `count` blocks into partials[0..NV).
+// One workgroup: sums the NV-tuples of `count` blocks into out[0..3*NV)
+// (out may be partials itself: every read is done before thread 0 writes).
 // (A grid-wide "last workgroup reduces" variant needs a device-scope fence
 // per workgroup, which costs more than this second launch across the XCDs'
 // separate L2 caches.)
 template <int NV>
-__global__ __launch_bounds__(256) void sum_blocks_kernel(uint64_t* partials, uint32_t count) {
+__global__ __launch_bounds__(256) void sum_blocks_kernel(const uint64_t* partials, uint32_t count, uint64_t* out) {
     e3 tot[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) tot[v] = {0, 0, 0};
@@ -81,9 +82,9 @@ __global__ __launch_bounds__(256) void sum_blocks_kernel(uint64_t* partials, uin
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            partials[3 * v] = tot[v].c0;
-            partials[3 * v + 1] = tot[v].c1;
-            partials[3 * v + 2] = tot[v].c2;
+            out[3 * v] = tot[v].c0;
+            out[3 * v + 1] = tot[v].c1;
+            out[3 * v + 2] = tot[v].c2;
         }
     }
 }

@@ -8,6 +8,11 @@
 // Field sums are exact, so any reduction order gives the reference's bits.
 // Reductions: wave64 shuffles, then LDS across the workgroup's waves, then a
 // one-workgroup pass over the per-workgroup partials.
+#include "../../include/whir_gpu.h"
+
+#include <mutex>
+
+#include "ctx.h"
 #include "gl.h"
 #include "kernels.h"
 #include "reduce_dev.h"
@@ -159,7 +164,7 @@ unsigned sumcheck_compute_launch(const uint64_t* a, bool a_base, const uint64_t*
                                  hipStream_t st, bool device_sum) {
     unsigned g = grid_for(n / 2, SUMCHECK_MAX_BLOCKS);
     hipLaunchKernelGGL(sumcheck_compute_kernel, dim3(g), dim3(256), 0, st, a, a_base ? 1 : 0, b, n / 2, partials);
-    if (device_sum) hipLaunchKernelGGL(red::sum_blocks_kernel<2>, dim3(1), dim3(256), 0, st, partials, g);
+    if (device_sum) hipLaunchKernelGGL(red::sum_blocks_kernel<2>, dim3(1), dim3(256), 0, st, partials, g, partials);
     return g;
 }
 
@@ -170,7 +175,7 @@ unsigned sumcheck_round_launch(const uint64_t* a, bool a_base, const uint64_t* b
     e3 rr = {r[0], r[1], r[2]};
     hipLaunchKernelGGL(sumcheck_round_kernel, dim3(g), dim3(256), 0, st, a, a_base ? 1 : 0, b, n / 2, rr, a_out, b_out,
                        partials);
-    if (device_sum) hipLaunchKernelGGL(red::sum_blocks_kernel<2>, dim3(1), dim3(256), 0, st, partials, g);
+    if (device_sum) hipLaunchKernelGGL(red::sum_blocks_kernel<2>, dim3(1), dim3(256), 0, st, partials, g, partials);
     return g;
 }
 
@@ -201,3 +206,59 @@ unsigned horner_launch(const uint64_t* f, bool f_ext3, uint64_t n, const HornerP
 }
 
 }  // namespace whir_gpu
+
+// ------------------------------------------- the allocation-free sumcheck round
+// whir_dev_sumcheck_round_noalloc: the per-workgroup (c0, c2) partials live
+// in one file-scope device array per GPU instead of a buffer the library
+// allocates.  Host code never uses the array's name as an address (it is a
+// host-side stand-in there): its device address is looked up once per device
+// with hipGetSymbolAddress at context creation and kept in g_scratch_ptr.
+namespace whir_gpu {
+namespace {
+constexpr size_t NOALLOC_SCRATCH_U64 = 6 * (size_t)SUMCHECK_MAX_BLOCKS;
+static_assert(NOALLOC_SCRATCH_U64 * 8 == WHIR_NOALLOC_SCRATCH_BYTES, "the header's stated scratch size");
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) uint64_t scratch[NOALLOC_SCRATCH_U64];
+std::mutex g_scratch_mu;
+uint64_t* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+int noalloc_scratch_init(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return bad_arg("device index out of range");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch_ptr[device]) return 0;
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));  // `device` is current (whir_ctx_create)
+    if (e != hipSuccess || !p) return fail("hipGetSymbolAddress(scratch)", e != hipSuccess ? e : hipErrorUnknown);
+    g_scratch_ptr[device] = (uint64_t*)p;
+    return 0;
+}
+
+}  // namespace whir_gpu
+
+extern "C" int whir_dev_sumcheck_round_noalloc(whir_ctx* ctx, const uint64_t* d_a, int a_is_base, const uint64_t* d_b,
+                                               size_t n, const uint64_t* r, uint64_t* d_a_out, uint64_t* d_b_out,
+                                               uint64_t* d_c0c2) {
+    using namespace whir_gpu;
+    if (!ctx || !d_a || !d_b || !r || !d_a_out || !d_b_out || !d_c0c2) return bad_arg("null argument");
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (n % 2) return bad_arg("sumcheck length must be even");
+    if (n > WHIR_NOALLOC_SUMCHECK_MAX_N) return bad_arg("sumcheck length above 2^32 (the no-allocation maximum)");
+    if (a_is_base && (const void*)d_a == (const void*)d_a_out) return bad_arg("base-field fold cannot run in place");
+    uint64_t* sc = nullptr;
+    {
+        std::lock_guard<std::mutex> s(g_scratch_mu);
+        if (ctx->device >= 0 && ctx->device < NOALLOC_MAX_DEVICES) sc = g_scratch_ptr[ctx->device];
+    }
+    if (!sc) return bad_arg("no-allocation scratch not set up for this context's device");
+    const unsigned blocks = grid_for(n / 2, SUMCHECK_MAX_BLOCKS);
+    if ((size_t)6 * blocks > NOALLOC_SCRATCH_U64) return bad_arg("sumcheck partials exceed the no-allocation scratch");
+    // every workgroup overwrites its own partials, so nothing needs clearing
+    const e3 rr = {r[0], r[1], r[2]};
+    hipLaunchKernelGGL(sumcheck_round_kernel, dim3(blocks), dim3(256), 0, ctx->stream, d_a, a_is_base ? 1 : 0, d_b,
+                       (uint64_t)(n / 2), rr, d_a_out, d_b_out, sc);
+    hipLaunchKernelGGL(red::sum_blocks_kernel<2>, dim3(1), dim3(256), 0, ctx->stream, (const uint64_t*)sc, blocks,
+                       d_c0c2);
+    HIPCHK(hipGetLastError());
+    return 0;
+}

@@ -378,6 +378,7 @@ int whir_ctx_set_hash(whir_ctx* ctx, uint32_t hash_id) {
 int whir_ctx_create(int device, whir_ctx** out) {
     if (!out) return bad_arg("null output");
     HIPCHK(hipSetDevice(device));
+    if (int rc = whir_gpu::noalloc_scratch_init(device)) return rc;
     whir_ctx* c = new whir_ctx();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
