@@ -14,10 +14,15 @@ sumcheck round —
 Inputs are resident in HBM before timing starts.  metric value = committed
 codeword elements per second = 2^26 * steps / elapsed.
 
-Also reports the roofline of the dominant stage (the RS encode / LDE, bound by
-HBM: algorithmic bytes 8*(2^24 + 2^26) per commit against 8 TB/s) from HIP
-events on the launch stream, and the reference's own CPU path (oracle/_ref,
-OpenMP) timed on the host over the same workload.
+The step runs on one stream; HIP events on that stream time each stage of the
+timed steps themselves.  Reported from them: the roofline of the dominant
+stage (the RS encode / LDE, bound by HBM: algorithmic bytes 8*(2^24 + 2^26)
+per commit against 8 TB/s), the Merkle tree's 32-bit integer-op rate (792 ops
+per BLAKE3 compression) and the sumcheck round's HBM fraction (56 n bytes).
+Also: the reference's own CPU path (oracle/_ref, OpenMP, OMP_PROC_BIND=close)
+timed on the host over the same workload, with the host CPU model and core
+counts, and the commit from pinned host coefficients to the root on the host
+(upload included) beside the device-resident headline.
 
 Beside the headline line's workload it measures, in the same JSON line:
   * full_prove: the whole WHIR prove (commit, every round, PoW, openings) at
@@ -30,15 +35,15 @@ Beside the headline line's workload it measures, in the same JSON line:
   * served: commit+prove requests (sizes 2^16..2^24 with weights
     .3/.3/.2/.15/.05, rate 1/4, 3 MLE constraints) through the serving loop:
     capacity, Poisson arrivals at a fixed offered rate (default 0.8x the
-    measured plain-path rate) with p50/p99 latency, and the plain
-    one-at-a-time GPU path on the same requests.
+    measured plain-path rate) and at 0.8x the measured serving capacity, each
+    with p50/p99 latency, and the plain one-at-a-time GPU path on the same
+    requests.
 
     python bench.py [--gpus 1] [--steps 30] [--warmup 3] [--no-cpu] [--legs commit,prove,prove_sha2,batched,served]
 """
 from __future__ import annotations
 
 import argparse
-import concurrent.futures
 import hashlib
 import json
 import os
@@ -52,6 +57,16 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the reference CPU baseline's OpenMP threads stay on neighbouring cores;
+# libomp then binds the calling (main) thread to one core, and threads
+# created later inherit that mask, so the original affinity is restored after
+# every CPU-baseline call (the serving loop's workers need all the cores)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+AFFINITY = os.sched_getaffinity(0)
+
+
+def restore_affinity():
+    os.sched_setaffinity(0, AFFINITY)
 
 METRIC = "Merkle-committed codeword elem/s at 2^24; served proofs/s + p99 latency"
 HBM_PEAK_GBS = 8000.0
@@ -59,6 +74,8 @@ HBM_PEAK_GBS = 8000.0
 # of the VOP3 forms (64-bit adds and compares, 32x32->64 multiplies, rotates)
 # that the LDE's field arithmetic and BLAKE3 consist of
 VALU_PEAK_WAVE_INSTS = 256 * 2.4e9
+# 32-bit integer ops at one per lane per cycle: 256 CUs x 64 lanes x 2.4 GHz
+INT_OP_PEAK = 256 * 64 * 2.4e9
 NV, LOG_INV_RATE, FOLD = 24, 2, 4
 SEED = 0x5748495200000002  # config seed: "WHIR" || 2
 
@@ -108,16 +125,7 @@ def gpu_leg(steps: int, warmup: int, coeffs, b, r):
     # the commit, so it runs on a second context's stream, from a worker
     # thread, while the commit runs: the HBM-bound round fills what the
     # VALU-bound LDE leaves of the memory system.
-    ctx2 = W.Context(0)
-    pool = concurrent.futures.ThreadPoolExecutor(1)
-
     def step():
-        fut = pool.submit(ctx2.sumcheck_round, d_f, True, d_b, poly, r, d_fa, d_fb)
-        root = ctx.commit(d_f, 1, poly, cw, depth, d_m, d_n)
-        c0, c2 = fut.result()
-        return root, c0, c2
-
-    def step_one_stream():
         root = ctx.commit(d_f, 1, poly, cw, depth, d_m, d_n)
         c0, c2 = ctx.sumcheck_round(d_f, True, d_b, poly, r, d_fa, d_fb)
         return root, c0, c2
@@ -125,28 +133,81 @@ def gpu_leg(steps: int, warmup: int, coeffs, b, r):
     for _ in range(warmup):
         step()
     ctx.sync()
-    ctx2.sync()
-    # timed region: exactly `steps` steps, synchronized on both sides
+    # timed region: exactly `steps` steps on one stream, synchronized on both
+    # sides; HIP events on that stream time each stage of these same steps
+    ctx.enable_timing(True)
+    ctx.reset_timing()
     t0 = time.perf_counter()
     for _ in range(steps):
         root, c0, c2 = step()
     ctx.sync()
-    ctx2.sync()
     elapsed = time.perf_counter() - t0
-    pool.shutdown()
-
-    # a second pass, on one stream, with HIP-event stage timing
-    ctx.enable_timing(True)
-    ctx.reset_timing()
-    for _ in range(steps):
-        r1 = step_one_stream()
-    ctx.sync()
     stages = ctx.stage_times()
     ctx.enable_timing(False)
-    assert r1[0] == root and np.array_equal(r1[1], c0) and np.array_equal(r1[2], c2)
-    ctx2.close()
     ctx.close()
     return elapsed, root, c0, c2, stages
+
+
+def host_resident_leg(coeffs, steps: int = 10):
+    """The commit as BASELINE.md section 3 times it: coefficients in pinned host
+    memory to the root in host memory (the 128 MiB upload included), and the
+    host-pointer drop-in whir_gpu_interleaved_rs_encode_blake3_matrix_leaves
+    (coefficients in, codeword and leaves back to pageable host memory)."""
+    import whirplonk_amd as W
+
+    poly, depth, cw = shapes()
+    pin = W.Pinned(poly)
+    pin.array[:] = coeffs
+    ctx = W.Context(0)
+    d_f = ctx.alloc(8 * poly)
+    d_m = ctx.alloc(8 * cw * depth)
+    d_n = ctx.alloc(32 * W.merkle_num_nodes(cw))
+
+    def one():
+        W._check(W.load().whir_memcpy_h2d(ctx.h, d_f, pin.ptr, 8 * poly))
+        return ctx.commit(d_f, 1, poly, cw, depth, d_m, d_n)
+
+    root = one()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        root = one()
+    ms = 1000.0 * (time.perf_counter() - t0) / steps
+    ctx.close()
+    del pin
+    W.rs_encode_leaves(coeffs, 1, poly, cw, depth)  # warm the default context
+    t0 = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        m, leaves = W.rs_encode_leaves(coeffs, 1, poly, cw, depth)
+    drop_ms = 1000.0 * (time.perf_counter() - t0) / reps
+    return {"workload": "nv=24 commit from pinned host coefficients to the root on the host (upload included)",
+            "ms_per_commit": round(ms, 3), "elem_per_s": round(cw * depth / ms * 1000.0, 1), "root": root.hex(),
+            "dropin_rs_encode_leaves_ms": round(drop_ms, 2),
+            "dropin_note": "whir_gpu_interleaved_rs_encode_blake3_matrix_leaves: 128 MiB in, 512 MiB codeword "
+                           "and 128 MiB leaves out to pageable host memory"}
+
+
+def host_cpu_info():
+    """Model name, logical CPUs in this process's affinity set and the
+    physical cores among them (from /proc/cpuinfo)."""
+    model, cores, logical = None, set(), set(AFFINITY)
+    try:
+        cpu, phys = None, None
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "processor":
+                    cpu, phys = int(v), None
+                elif k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id" and cpu in logical:
+                    cores.add((phys, v))
+    except OSError:
+        pass
+    return {"cpu_model": model, "logical_cpus": len(logical), "physical_cores": len(cores) or None}
 
 
 def cpu_leg(coeffs, b, r, reps: int = 3):
@@ -174,6 +235,7 @@ def cpu_leg(coeffs, b, r, reps: int = 3):
 
     run()  # warm-up
     times = [run() for _ in range(reps)]
+    restore_affinity()
     return {"seconds": statistics.median(times), "threads": int(R.ref_num_threads()),
             "root": root.tobytes(), "c0": c0.copy(), "c2": c2.copy(), "reps": reps}
 
@@ -235,6 +297,7 @@ def prove_leg(steps: int, warmup: int, cpu: bool, hash_id: int = 1):
             t0 = time.perf_counter()
             _, n2, h2 = oracle.ref_prove(PROVE_NV, coeffs, pts, hash_id=hash_id, rate=1)
             out["cpu_reference_ms"] = round(1000.0 * (time.perf_counter() - t0), 1)
+            restore_affinity()
             out["cpu_threads"] = int(oracle.ref().ref_num_threads())
             out["bit_exact_vs_cpu"] = (n2 == narg and h2 == hints)
     return out
@@ -304,6 +367,7 @@ def batched_leg(steps: int, cpu: bool):
                 root = np.zeros(32, dtype=np.uint8)
                 R.ref_commit_base(oracle._ptr64(polys[k]), sz, (sz // 16) * 4, 16, None, oracle._ptr8(root))
                 ok &= root.tobytes() == roots[k]
+            restore_affinity()
             out["bit_exact_vs_cpu"] = bool(ok)
     return out
 
@@ -375,41 +439,53 @@ def served_leg(n_req: int, offered: float, slots: int, budget_gb: float, cpu: bo
         if offered <= 0:  # 0.8x the measured plain-path capacity
             offered = round(0.8 * n_req / plain_s, 1)
         same_cap = all(res[i][:2] == ref[i] for i in range(n_req))
-        # Poisson arrivals at the fixed offered rate
-        arr = np.cumsum(np.random.default_rng(SERVE_SEED).exponential(1.0 / offered, n_req))
-        ids = [None] * n_req
 
-        def submitter():
-            t_start = time.perf_counter()
-            for i, (nv, (pin, pts)) in enumerate(reqs):
-                dt = t_start + arr[i] - time.perf_counter()
-                if dt > 0:
-                    time.sleep(dt)
-                ids[i] = srv.submit(1 << nv, pin.array, pts, borrow=True)
+        def poisson(rate, seed):
+            """Poisson arrivals at a fixed offered rate: (proofs/s, sorted
+            end-to-end latencies in ms, every proof equal to the plain one)."""
+            arr = np.cumsum(np.random.default_rng(seed).exponential(1.0 / rate, n_req))
+            ids = [None] * n_req
 
-        th = threading.Thread(target=submitter)
-        t0 = time.perf_counter()
-        th.start()
-        lat, same_p = [], True
-        for i in range(n_req):
-            while ids[i] is None:
-                time.sleep(0.0002)
-            narg, hints, ms = srv.wait(ids[i])
-            lat.append(ms[2])
-            same_p &= (narg, hints) == ref[i]
-        th.join()
-        run_s = time.perf_counter() - t0
+            def submitter():
+                t_start = time.perf_counter()
+                for i, (nv, (pin, pts)) in enumerate(reqs):
+                    dt = t_start + arr[i] - time.perf_counter()
+                    if dt > 0:
+                        time.sleep(dt)
+                    ids[i] = srv.submit(1 << nv, pin.array, pts, borrow=True)
+
+            th = threading.Thread(target=submitter)
+            t0 = time.perf_counter()
+            th.start()
+            lat, same = [], True
+            for i in range(n_req):
+                while ids[i] is None:
+                    time.sleep(0.0002)
+                narg, hints, ms = srv.wait(ids[i])
+                lat.append(ms[2])
+                same &= (narg, hints) == ref[i]
+            th.join()
+            return n_req / (time.perf_counter() - t0), sorted(lat), same
+
+        def pct(lat, q):
+            return round(lat[min(len(lat) - 1, int(q * len(lat)))], 3)
+
+        light_pps, lat, same_p = poisson(offered, SERVE_SEED)
+        heavy_rate = round(0.8 * n_req / cap_s, 1)  # 0.8x the measured serving capacity
+        heavy_pps, lat_h, same_h = poisson(heavy_rate, SERVE_SEED + 1)
         st = srv.stats()
-    lat.sort()
     out.update({
         "capacity_proofs_per_s": round(n_req / cap_s, 2),
         "speedup_vs_plain": round(plain_s / cap_s, 3),
         "offered_rate": offered,
-        "proofs_per_s": round(n_req / run_s, 2),
-        "p50_ms": round(lat[len(lat) // 2], 3), "p99_ms": round(lat[min(len(lat) - 1, int(0.99 * len(lat)))], 3),
+        "proofs_per_s": round(light_pps, 2),
+        "p50_ms": pct(lat, 0.5), "p99_ms": pct(lat, 0.99),
+        "heavy_offered_rate": heavy_rate,
+        "heavy_proofs_per_s": round(heavy_pps, 2),
+        "heavy_p50_ms": pct(lat_h, 0.5), "heavy_p99_ms": pct(lat_h, 0.99),
         "peak_hbm_bytes": st["peak"],
         "pow_searches_per_launch": round(pow_searches / max(pow_launches, 1), 2),
-        "bit_exact_vs_plain": bool(same_cap and same_p),
+        "bit_exact_vs_plain": bool(same_cap and same_p and same_h),
         "bit_exact_vs_cpu": None,
     })
     if cpu:
@@ -422,6 +498,7 @@ def served_leg(n_req: int, offered: float, slots: int, budget_gb: float, cpu: bo
                 i = next(j for j, (v, (pp, _)) in enumerate(reqs) if v == nv and pp is pin)
                 _, n2, h2 = oracle.ref_prove(nv, np.array(pin.array), pts, rate=2)
                 ok &= (n2, h2) == ref[i]
+            restore_affinity()
             out["bit_exact_vs_cpu"] = bool(ok)
             out["cpu_sample"] = "input set 0 of each size (5 proofs) vs the reference prover; every request vs plain"
     return out
@@ -451,8 +528,9 @@ def commit_leg(args):
     traffic, valu_insts = lde_profile()
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "rs_encode (lde_pass_a + lde_pass_b over 4 cosets)",
-                "algorithmic_bytes": alg_bytes, "avg_ms": round(lde_avg_s * 1000, 4)}
+                "kernel": "rs_encode (lde_pass_a_regs + lde_pass_b_rows2 over 4 cosets, row leaves fused)",
+                "algorithmic_bytes": alg_bytes, "avg_ms": round(lde_avg_s * 1000, 4),
+                "timing": "HIP events on the launch stream around each commit's encode, in the timed steps"}
     # The LDE is bound by integer VALU issue, not HBM: its instruction count
     # (SQ_INSTS_VALU of both passes, rocprofv3) over the live time, against one
     # wave64 VALU instruction per CU per cycle (256 CUs at 2.4 GHz), the issue
@@ -464,6 +542,19 @@ def commit_leg(args):
                 "unit": "wave-instr/s", "frac": round(rate / VALU_PEAK_WAVE_INSTS, 4),
                 "wave_insts_per_launch": valu_insts}
     stage_ms = {k: round(v / max(c, 1), 4) for k, (v, c) in stages.items()}
+    # The Merkle tree (2^22 leaves: 2^22 - 1 BLAKE3 compressions of 792
+    # 32-bit ops each) against one 32-bit op per lane per cycle, and the
+    # sumcheck round (56 n algorithmic bytes) against HBM.
+    tree_s = stages["merkle"][0] / 1000.0 / max(stages["merkle"][1], 1)
+    tree_ops = 792 * (cw - 1)
+    merkle_roofline = {"bound": "valu", "achieved": round(tree_ops / tree_s, -9), "peak": INT_OP_PEAK,
+                       "unit": "int32-op/s", "frac": round(tree_ops / tree_s / INT_OP_PEAK, 4),
+                       "compressions": cw - 1, "avg_ms": round(tree_s * 1000, 4)}
+    sc_s = stages["sumcheck_round"][0] / 1000.0 / max(stages["sumcheck_round"][1], 1)
+    sc_bytes = 56 * poly
+    sumcheck_roofline = {"bound": "hbm", "achieved": round(sc_bytes / sc_s / 1e9, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(sc_bytes / sc_s / 1e9 / HBM_PEAK_GBS, 4),
+                         "algorithmic_bytes": sc_bytes, "avg_ms": round(sc_s * 1000, 4)}
 
     cpu = None
     root_ok = None
@@ -474,7 +565,8 @@ def commit_leg(args):
                    "kind": "reference",
                    "sample": f"full workload (nv=24 commit + first sumcheck round), reference C++ CPU path "
                              f"(oracle/_ref, OpenMP {c['threads']} threads), 1 warm-up + median of {c['reps']}",
-                   "seconds_per_step": round(c["seconds"], 4)}
+                   "seconds_per_step": round(c["seconds"], 4),
+                   "omp_proc_bind": os.environ.get("OMP_PROC_BIND"), **host_cpu_info()}
             root_ok = (c["root"] == root and np.array_equal(c["c0"], c0) and np.array_equal(c["c2"], c2))
 
     out = {
@@ -489,6 +581,9 @@ def commit_leg(args):
         "stage_ms": stage_ms,
         "roofline": roofline,
         "valu_roofline": valu,
+        "merkle_roofline": merkle_roofline,
+        "sumcheck_roofline": sumcheck_roofline,
+        "host_resident": host_resident_leg(coeffs) if not args.no_host else None,
         "cpu_baseline": cpu,
         "bit_exact_vs_cpu": root_ok,
         "root": root.hex(),
@@ -502,6 +597,7 @@ def main():
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-resident commit timing")
     ap.add_argument("--legs", default="commit,prove,prove_sha2,batched,served")
     ap.add_argument("--requests", type=int, default=2000, help="served requests")
     ap.add_argument("--offered", type=float, default=0.0,
@@ -517,17 +613,27 @@ def main():
         sys.exit(2)
 
     legs = set(args.legs.split(","))
+    t_start = time.perf_counter()
+
+    def done(name):
+        print(f"bench: {name} leg done at {time.perf_counter() - t_start:.1f} s", file=sys.stderr, flush=True)
+
     out = commit_leg(args) if "commit" in legs else {
         "metric": METRIC, "value": None, "unit": "elem/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u64", "data": "synthetic", "config": {"workload": "commit leg skipped"},
         "roofline": None, "cpu_baseline": None}
+    done("commit")
     out["full_prove"] = prove_leg(args.steps, args.warmup, not args.no_cpu) if "prove" in legs else None
+    done("prove")
     out["full_prove_sha2"] = (prove_leg(args.steps, args.warmup, not args.no_cpu, hash_id=2)
                               if "prove_sha2" in legs else None)
+    done("prove_sha2")
     out["batched"] = batched_leg(args.steps, not args.no_cpu) if "batched" in legs else None
+    done("batched")
     out["served"] = (served_leg(args.requests, args.offered, args.slots, args.budget_gb, not args.no_cpu)
                      if "served" in legs else None)
+    done("served")
     print(json.dumps(out))
 
 

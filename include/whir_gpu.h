@@ -385,14 +385,19 @@ int whir_dev_mle_evaluate(whir_ctx* ctx, const uint64_t* d_f, int f_is_base, siz
 
 /* LDE autotuner: times every valid radix split m = M1 * M2 (M2 = the second
  * pass's DFT size, which is also the number of rows, and so of row leaves,
- * each second-pass workgroup produces) for one commit shape (message length,
- * codeword rows, interleaved columns, base or Ext3) and records the fastest
- * for the process; later encodes of that shape use it.  Every split gives
- * the same codeword. */
+ * each second-pass workgroup produces) under both forms of the fused-leaves
+ * second pass (one workgroup per CU holding the whole 16 x 1024 tile, or two
+ * per CU each staging half of it: 1024 or 512 leaves per workgroup round)
+ * for one commit shape (message length, codeword rows, interleaved columns,
+ * base or Ext3), and records the fastest pair for the process; later encodes
+ * of that shape use it.  Every choice gives the same codeword.  out_lg_m2
+ * receives log2 M2. */
 int whir_autotune_lde(whir_ctx* ctx, size_t message_length, size_t codeword_length, size_t columns, int ext3,
                       double* out_best_ms, uint32_t* out_lg_m2);
-/* The tuning table as 9-word records (shape key, log2 M2), for saving and
- * loading across processes.  export returns the word count. */
+/* The tuning table as 9-word records (8 shape-key words, then log2 M2 in bits
+ * 0-7 and the second-pass form in bits 8-15: 0 the library default, 1 one
+ * workgroup per CU, 2 two), for saving and loading across processes.  export
+ * returns the word count. */
 size_t whir_tuning_export(uint64_t* out, size_t cap);
 int whir_tuning_import(const uint64_t* records, size_t words);
 

@@ -128,6 +128,11 @@ def test_every_split_same_codeword(ctx, m, n, ext3):
         assert np.array_equal(got, want), lg
         tried += 1
     assert tried >= 2
+    # both forms of the fused-leaves second pass (bits 8-15 of the record)
+    for form in (1, 2):
+        _force(key, default | (form << 8))
+        ctx.rs_encode(d_in, 1, m * depth, n, depth, d_out, ext3=ext3)
+        assert np.array_equal(ctx.download(d_out, n * depth * (3 if ext3 else 1)), want), form
     _force(key, default)
     ms, lg = W.autotune_lde(ctx, m, n, depth, ext3)
     assert ms > 0 and (m >> lg) <= 1024

@@ -107,3 +107,49 @@ def test_integration_call_sites_compiled_against_reference():
     print(r.stdout, flush=True)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.count("ok  ") >= 18 and "FAIL" not in r.stdout
+
+
+def test_whir_profile_csv_rows():
+    """WHIR_PROFILE=1 makes the drop-ins and the prover print the reference's
+    CSV rows (profiling.hpp:12-142): one `mode,phase,size,stage,time_ms`
+    header, then rows whose stages are the reference's kept vocabulary, with
+    the commit/prove phases of its PhaseGuard."""
+    import os
+    import subprocess
+    import sys
+
+    code = (
+        "import numpy as np, whirplonk_amd as W\n"
+        "c, st = W.lcg_uniform(3, 1 << 14)\n"
+        "W.rs_encode_leaves(c, 1, 1 << 14, 1 << 12, 16)\n"
+        "pts, _ = W.lcg_uniform(st, 3 * 14)\n"
+        "pts = pts.reshape(1, 14, 3)\n"
+        "p = W.Params.default(starting_log_inv_rate=2)\n"
+        "pid, sid = W.domain_separator_cli(p)\n"
+        "with W.Context() as ctx:\n"
+        "    d = ctx.upload(c)\n"
+        "    ev = np.array([ctx.mle_evaluate(d, True, pts[0])])\n"
+        "    with W.Prover(ctx, p, 1 << 14, pid, sid) as pr:\n"
+        "        pr.commit(d, 1)\n"
+        "        pr.prove(pts, ev)\n"
+    )
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, WHIR_PROFILE="1", PYTHONPATH=root)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120, env=env, cwd=root)
+    assert r.returncode == 0, r.stderr
+    lines = [l for l in r.stderr.splitlines() if l.count(",") == 4]
+    assert lines[0] == "mode,phase,size,stage,time_ms" and lines.count(lines[0]) == 1
+    rows = [l.split(",") for l in lines[1:]]
+    kept = {"witness_encoding", "witness_gpu_alloc", "witness_h2d", "witness_rs_encode", "witness_leaf_hash",
+            "witness_d2h", "merkle_build_total", "ood_evaluation", "commit_total", "prove_total", "total_prover"}
+    assert rows and all(st in kept and float(ms) >= 0 and int(sz) > 0 for _, _, sz, st, ms in rows)
+    got = {(m, ph, st) for m, ph, _, st, _ in rows}
+    for want in [("gpu", "none", "witness_h2d"), ("gpu", "none", "witness_rs_encode"), ("gpu", "none", "witness_d2h"),
+                 ("prover", "commit", "witness_encoding"), ("prover", "commit", "merkle_build_total"),
+                 ("prover", "commit", "ood_evaluation"), ("prover", "commit", "commit_total"),
+                 ("prover", "prove", "prove_total"), ("prover", "none", "total_prover")]:
+        assert want in got, want
+    # without the variable nothing is printed
+    env.pop("WHIR_PROFILE")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120, env=env, cwd=root)
+    assert r.returncode == 0 and "mode,phase,size,stage,time_ms" not in r.stderr

@@ -34,8 +34,9 @@ LdeShape commit_shape(uint64_t size, uint32_t depth, uint32_t rate) {
     return s;
 }
 
-// Times every valid split of a shape (one warm-up, then the median of
-// `reps` runs with HIP events on the context's stream) and records the best.
+// Times every valid split of a shape under both pass-B forms (one warm-up,
+// then the median of `reps` runs with HIP events on the context's stream) and
+// records the best pair, packed as log2 M2 | form << 8.
 int tune_shape(whir_ctx* ctx, LdeShape s, int reps, double* best_ms, uint32_t* best_lg) {
     uint32_t cand[11];
     int nc = lde_splits(s, cand, 11);
@@ -51,10 +52,11 @@ int tune_shape(whir_ctx* ctx, LdeShape s, int reps, double* best_ms, uint32_t* b
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     double best = 1e30;
-    uint32_t arg = cand[0];
+    uint32_t arg = cand[0], arg_form = 2;
     int rc = 0;
-    for (int c = 0; c < nc && !rc; ++c) {
-        s.lg_m2 = cand[c];
+    for (int c = 0; c < 2 * nc && !rc; ++c) {
+        s.lg_m2 = cand[c / 2];
+        s.rows_form = 2 - (uint32_t)(c % 2);
         std::vector<float> t;
         for (int r = 0; r <= reps && !rc; ++r) {
             (void)hipEventRecord(e0, ctx->stream);
@@ -71,7 +73,8 @@ int tune_shape(whir_ctx* ctx, LdeShape s, int reps, double* best_ms, uint32_t* b
         double med = t[t.size() / 2];
         if (med < best) {
             best = med;
-            arg = cand[c];
+            arg = cand[c / 2];
+            arg_form = s.rows_form;
         }
     }
     (void)hipStreamSynchronize(ctx->stream);
@@ -82,7 +85,8 @@ int tune_shape(whir_ctx* ctx, LdeShape s, int reps, double* best_ms, uint32_t* b
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     s.lg_m2 = 0;
-    set_tuned_split(s, arg);
+    s.rows_form = 0;
+    set_tuned_split(s, arg | (arg_form << 8));
     if (best_ms) *best_ms = best;
     if (best_lg) *best_lg = arg;
     return 0;
@@ -194,10 +198,12 @@ int whir_tuning_import(const uint64_t* rec, size_t words) {
         s.inverse = rec[k + 7] != 0;
         uint32_t cand[11];
         int nc = lde_splits(s, cand, 11);
+        const uint64_t lg = rec[k + 8] & 0xFF, form = rec[k + 8] >> 8;
         bool ok = false;
-        for (int c = 0; c < nc; ++c) ok |= cand[c] == rec[k + 8];
+        for (int c = 0; c < nc; ++c) ok |= cand[c] == lg;
         if (!ok) return bad_arg("tuning record names an invalid split");
-        set_tuned_split(s, (uint32_t)rec[k + 8]);
+        if (form > 2) return bad_arg("tuning record names an invalid pass-B form");
+        set_tuned_split(s, (uint32_t)(lg | (form << 8)));
     }
     return 0;
 }

@@ -18,6 +18,14 @@ namespace whir_gpu {
 
 std::string& last_error();  // per thread
 
+// Serialises the contexts' grow-only buffer (re)allocations across threads:
+// serving-loop workers each own a context, and a worker that grows a buffer
+// (hipFree + hipMalloc, or the pinned-host forms) does so while other workers
+// keep launching.  Growth is rare after warm-up, so one process-wide lock
+// costs nothing measurable and keeps those runtime calls from ever running
+// concurrently with each other.
+std::mutex& alloc_mutex();
+
 inline int fail(const char* what, hipError_t e) {
     last_error() = std::string(what) + ": " + hipGetErrorString(e);
     return -1;
@@ -123,6 +131,7 @@ struct whir_ctx {
 
     int ensure(whir_gpu::DevBuf& b, size_t bytes) {
         if (b.bytes >= bytes && b.p) return 0;
+        std::lock_guard<std::mutex> lk(whir_gpu::alloc_mutex());
         if (b.p) {
             HIPCHK(hipStreamSynchronize(stream));
             HIPCHK(hipFree(b.p));
@@ -199,6 +208,7 @@ struct whir_ctx {
         if (!bytes) return 0;
         if (hdesc_ev) HIPCHK(hipEventSynchronize(hdesc_ev));
         if (hdesc_bytes < bytes) {
+            std::lock_guard<std::mutex> lk(whir_gpu::alloc_mutex());
             if (hdesc) HIPCHK(hipHostFree(hdesc));
             hdesc = nullptr;
             hdesc_bytes = 0;
@@ -220,6 +230,7 @@ struct whir_ctx {
     size_t hio_bytes = 0;
     int ensure_hio(size_t bytes) {
         if (hio && hio_bytes >= bytes) return 0;
+        std::lock_guard<std::mutex> lk(whir_gpu::alloc_mutex());
         if (hio) {
             HIPCHK(hipStreamSynchronize(stream));
             HIPCHK(hipHostFree(hio));
@@ -239,6 +250,7 @@ struct whir_ctx {
     }
     int ensure_hpart() {
         if (hpart) return 0;
+        std::lock_guard<std::mutex> lk(whir_gpu::alloc_mutex());
         void* h = nullptr;
         HIPCHK(hipHostMalloc(&h, HPART_U64 * 8, hipHostMallocMapped | hipHostMallocCoherent));
         void* d = nullptr;
@@ -260,6 +272,7 @@ struct whir_ctx {
             return 0;
         }
         int inv = inverse ? 1 : 0;
+        std::lock_guard<std::mutex> lk(whir_gpu::alloc_mutex());
         if (!t1024[inv]) {
             uint64_t w = gl::root_of_unity(10);
             if (inverse) w = gl::inv(w);

@@ -6,6 +6,9 @@
 
 namespace whir_gpu {
 
+// hash engines of the leaves, trees and proof-of-work searches
+enum : uint32_t { HASH_BLAKE3 = 1, HASH_SHA2 = 2 };
+
 // Twiddle tables of one transform size n = 2^log_n, forward or inverse:
 // lo[x] = w^x (x < 2048), hi[y] = w^(2048 y) (y < max(1, n/2048)) so that
 // w^e = lo[e & 2047] * hi[e >> 11]; t1024[t] = w_1024^t for the sub-DFTs.
@@ -33,6 +36,15 @@ struct LdeShape {
     // log2 of the second-pass DFT size M2 (m = M1 * M2); 0 = the default
     // split M2 = min(m, 1024).  Any valid split gives the same output.
     uint32_t lg_m2 = 0;
+    // form of the fused-leaves pass B of 16-column rows (M2 = 1024): 1 = one
+    // workgroup per CU with the next tile's loads in registers, 2 = two
+    // workgroups per CU with half-tile LDS rounds, 0 = the library default.
+    // Both give the same output.
+    uint32_t rows_form = 0;
+    // engine of the row leaves hashed inside the last pass (lde_launch's
+    // `leaves`): BLAKE3 fuses into every 16-column row form, SHA-256 into the
+    // persistent forward rows forms (M2 = 1024 with a first pass)
+    uint32_t leaf_hash = HASH_BLAKE3;
 };
 
 // Step 1 of an NTT longer than 2^20 (n = m * E, 2 <= E <= 64): see lde.hip.
@@ -65,7 +77,6 @@ void hash_many_launch(const uint8_t* in, uint64_t size, uint64_t count, uint8_t*
 void merkle_build_launch(uint8_t* nodes, uint64_t n_leaves, hipStream_t st, uint8_t* root_copy = nullptr);
 // The SHA-256 engine (hash_id 2): the same three operations with SHA-256 of
 // the message bytes in place of BLAKE3 (hash/sha2_engine.hpp:47-81).
-enum : uint32_t { HASH_BLAKE3 = 1, HASH_SHA2 = 2 };
 void sha256_leaves_launch(const uint64_t* matrix, uint64_t rows, uint32_t row_u64, uint8_t* leaves,
                           hipStream_t st);
 void sha256_hash_many_launch(const uint8_t* in, uint64_t size, uint64_t count, uint8_t* out, hipStream_t st);

@@ -31,10 +31,12 @@
 // keeps the next tile's loads in flight in registers while it transforms the
 // current one, so the CU's VALU is not idle while a tile loads.
 #include <algorithm>
+#include <cstdlib>
 
 #include "blake3_dev.h"
 #include "dft.h"
 #include "kernels.h"
+#include "sha256_dev.h"
 
 #include <map>
 #include <mutex>
@@ -74,6 +76,49 @@ struct LdeArgs {
 };
 
 constexpr int FUSE_COLS = 16;  // row width (u64) of the fused leaf hashing
+
+// Leaf hash of one 16-limb (128-byte) row whose limb c is ld(c), with the
+// context's engine: BLAKE3 (one chunk of two blocks, CHUNK_START then
+// CHUNK_END | ROOT, matrix_commit.hpp:169-195) or SHA-256 (two data blocks
+// and the constant padding block of a 128-byte message, sha2_engine.hpp:47-81).
+// out: the 32 digest bytes as 8 little-endian words.
+template <uint32_t HASH, typename Load>
+__device__ __forceinline__ void row_leaf(Load ld, uint32_t out[8]) {
+    if constexpr (HASH == HASH_SHA2) {
+        uint32_t st[8], w[16];
+        s256::init(st);
+#pragma unroll
+        for (int blk = 0; blk < 2; ++blk) {
+#pragma unroll
+            for (int cc = 0; cc < 8; ++cc) {
+                const uint64_t x = ld(8 * blk + cc);
+                w[2 * cc] = s256::bswap((uint32_t)x);
+                w[2 * cc + 1] = s256::bswap((uint32_t)(x >> 32));
+            }
+            s256::compress(st, w);
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) w[i] = 0;
+        w[0] = 0x80000000u;
+        w[15] = 128 * 8;  // message length in bits
+        s256::compress(st, w);
+        s256::digest(st, out);
+    } else {
+        uint32_t m[16];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) out[i] = b3::iv(i);
+#pragma unroll
+        for (int blk = 0; blk < 2; ++blk) {
+#pragma unroll
+            for (int cc = 0; cc < 8; ++cc) {
+                const uint64_t x = ld(8 * blk + cc);
+                m[2 * cc] = (uint32_t)x;
+                m[2 * cc + 1] = (uint32_t)(x >> 32);
+            }
+            b3::compress(out, m, 0, 64, blk == 0 ? b3::CHUNK_START : (b3::CHUNK_END | b3::ROOT), out);
+        }
+    }
+}
 
 // swizzled position of (row, col) in the 16-wide LDS row stage: rows are
 // read one per lane, so columns rotate with the row to spread the banks
@@ -345,7 +390,7 @@ __global__ __launch_bounds__(THREADS) void lde_pass_b(LdeArgs a, uint64_t s0) {
 // column hits every bank pair four times (1057 and 33 are 1 mod 16).
 __device__ __forceinline__ int rows_idx(int r, int c) { return c * 1057 + 33 * (r & 31) + (r >> 5); }
 
-template <bool INV>
+template <bool INV, uint32_t HASH = HASH_BLAKE3>
 __global__ __launch_bounds__(THREADS) void lde_pass_b_rows(LdeArgs a, uint64_t s0, uint32_t ntiles) {
     constexpr int M2 = 1024, R = 32, PAD = M2 + M2 / R + 1;
     static_assert(PAD == 1057, "rows_idx assumes the padded 1024-point layout");
@@ -410,19 +455,8 @@ __global__ __launch_bounds__(THREADS) void lde_pass_b_rows(LdeArgs a, uint64_t s
 #pragma unroll
         for (int h = 0; h < M2 / THREADS; ++h) {
             const uint32_t r = tid + h * THREADS;
-            uint32_t cv[8], m[16];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) cv[i] = b3::iv(i);
-#pragma unroll
-            for (int blk = 0; blk < 2; ++blk) {
-#pragma unroll
-                for (int cc = 0; cc < 8; ++cc) {
-                    uint64_t x = lds[rows_idx(r, 8 * blk + cc)];
-                    m[2 * cc] = (uint32_t)x;
-                    m[2 * cc + 1] = (uint32_t)(x >> 32);
-                }
-                b3::compress(cv, m, 0, 64, blk == 0 ? b3::CHUNK_START : (b3::CHUNK_END | b3::ROOT), cv);
-            }
+            uint32_t cv[8];
+            row_leaf<HASH>([&](int c) { return lds[rows_idx(r, c)]; }, cv);
             uint4* dst = reinterpret_cast<uint4*>(a.leaves + 32 * (i1 + a.L * r));
             dst[0] = make_uint4(cv[0], cv[1], cv[2], cv[3]);
             dst[1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
@@ -436,6 +470,91 @@ __global__ __launch_bounds__(THREADS) void lde_pass_b_rows(LdeArgs a, uint64_t s
             *reinterpret_cast<ulonglong2*>(a.out + (i1 + a.L * r) * 16 + cp) = make_ulonglong2(x0, x1);
         }
         __syncthreads();  // the next tile rewrites the twiddles and the stage
+    }
+}
+
+// Pass B of a 16-column base codeword with M2 = 1024 and the row leaves fused,
+// at four waves per SIMD: two 512-lane workgroups share a CU, each with a
+// 66 KB LDS buffer (half a tile) and at most 128 VGPRs, instead of one
+// workgroup holding the whole 16 x 1024 tile in LDS.  The step-1 -> step-2
+// transposition runs in two rounds by the top bit of n2, and the rows are
+// staged for the leaf hashes and the stores in two halves of 512 rows.  While
+// one workgroup waits at a barrier or for its loads, the other computes (no
+// register prefetch of the next tile).
+//   step 1: lane (c, n2), c = (tid >> 4) & 15, n2 = (tid & 15) | ((tid >> 8) << 4)
+//   step 2: lane (cb, k1) = (tid % 16, tid / 16)
+// Exchange slot of (column c, k1, n2 mod 16): c * 519 + 16 k1 + n2 % 16 (at
+// most 3-way bank aliasing for both the writes and the reads).
+__device__ __forceinline__ int xch_idx(int c, int k1, int n2l) { return c * 519 + 16 * k1 + n2l; }
+constexpr int XCH_WORDS = 16 * 519;
+static_assert(XCH_WORDS >= 512 * FUSE_COLS, "the row stage reuses the exchange buffer");
+
+template <bool INV, uint32_t HASH = HASH_BLAKE3>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4, 4))) void lde_pass_b_rows2(
+    LdeArgs a, uint64_t s0, uint32_t ntiles) {
+    constexpr int M2 = 1024, R = 32;
+    static_assert(THREADS == 16 * R, "one thread per (column, n2)");
+    extern __shared__ uint64_t lds[];
+    uint64_t* buf = lds;               // exchange, then row stage
+    uint64_t* twt = lds + XCH_WORDS;   // inter-pass twiddles of the tile
+    const uint32_t kmask = (1u << a.lgNK) - 1;
+    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        uint32_t tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));  // lane indices re-derived per tile (see lde_pass_b_rows)
+        const uint64_t i1 = s0 + (tile >> a.lgNK) + a.E * (tile & kmask);
+        const uint32_t c = (tid >> 4) & 15, hw = tid >> 8, n2 = (tid & 15) | (hw << 4);
+        const uint64_t* zb = a.z + (uint64_t)(tile >> a.lgNK) * a.zstride + (uint64_t)(tile & kmask) * (16 * M2);
+        uint64_t v[R];
+#pragma unroll
+        for (int n1 = 0; n1 < R; ++n1) v[n1] = zb[c * M2 + n2 + R * n1];
+        // the previous tile's last reads of twt and buf ended at its final barrier
+        for (uint32_t j2 = tid; j2 < M2; j2 += THREADS) twt[j2] = twid(a, (i1 * j2) & (a.n - 1));
+        __syncthreads();
+#pragma unroll
+        for (int n1 = 0; n1 < R; ++n1) v[n1] = gl::mul(v[n1], twt[n2 + R * n1]);
+        dft::dft_reg<R, INV>(v);
+#pragma unroll
+        for (int k1 = 1; k1 < R; ++k1) v[k1] = gl::mul(v[k1], a.t1024[(n2 * k1) & 1023]);
+        // transposition: in round h the lanes with n2 >> 4 == h (waves 4h..4h+3)
+        // write all 32 of their values and every lane reads its 16 inputs with
+        // n2 >> 4 == h
+        const uint32_t cb = tid % 16, k1r = tid / 16;
+        uint64_t u[R];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (hw == (uint32_t)h) {  // uniform per wave
+#pragma unroll
+                for (int k1 = 0; k1 < R; ++k1) buf[xch_idx(c, k1, n2 & 15)] = v[k1];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 16; ++q) u[16 * h + q] = buf[xch_idx(cb, k1r, q)];
+            __syncthreads();
+        }
+        dft::dft_reg<R, INV>(u);  // u[k2] = row k1r + 32 k2 of column cb
+        // rows 512 h .. 512 h + 511 staged row-major, hashed one per lane, stored
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk) buf[stage_idx(k1r + R * kk, cb)] = u[16 * h + kk];
+            __syncthreads();
+            {
+                const uint32_t r = tid;  // local row
+                uint32_t cv[8];
+                row_leaf<HASH>([&](int c) { return buf[stage_idx(r, c)]; }, cv);
+                uint4* dst = reinterpret_cast<uint4*>(a.leaves + 32 * (i1 + a.L * (512 * h + r)));
+                dst[0] = make_uint4(cv[0], cv[1], cv[2], cv[3]);
+                dst[1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
+            }
+#pragma unroll
+            for (int q2 = 0; q2 < 512 * 16 / (2 * THREADS); ++q2) {
+                const uint32_t q = tid + q2 * THREADS;
+                const uint32_t r = q / 8, cp = 2 * (q % 8);
+                const uint64_t x0 = buf[stage_idx(r, cp)], x1 = buf[stage_idx(r, cp + 1)];
+                *reinterpret_cast<ulonglong2*>(a.out + (i1 + a.L * (512 * h + r)) * 16 + cp) = make_ulonglong2(x0, x1);
+            }
+            __syncthreads();  // the next half (or tile) rewrites the stage
+        }
     }
 }
 
@@ -593,6 +712,20 @@ uint32_t split_r2(uint64_t M) {
 }
 uint64_t padded(uint64_t M) { return M + M / split_r2(M) + 1; }
 
+// LDS of lde_pass_b_rows2: the exchange / stage buffer and one twiddle table
+constexpr size_t ROWS2_LDS = (size_t)(XCH_WORDS + 1024) * 8;
+
+// The fused-rows pass B form: 2 = lde_pass_b_rows2 (two workgroups per CU,
+// the default), 1 = lde_pass_b_rows (one workgroup per CU with the next
+// tile's loads in registers).  WHIR_LDE_ROWS overrides it for A/B timing.
+int rows_variant() {
+    static const int v = [] {
+        const char* e = std::getenv("WHIR_LDE_ROWS");
+        return (e && e[0] == '1') ? 1 : 2;
+    }();
+    return v;
+}
+
 using KernelFn = void (*)(LdeArgs, uint64_t);
 using RowsFn = void (*)(LdeArgs, uint64_t, uint32_t);
 using ColsFn = void (*)(LdeArgs, uint64_t, uint32_t, uint32_t);
@@ -738,6 +871,10 @@ int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* s
     a.z = scratch;
     a.tw_in_b = p.TB == 1 ? 1 : 0;
     bool fuse = leaves && s.row_major && s.C == FUSE_COLS && p.TB == 1 && p.CB == FUSE_COLS && s.out_el == 1;
+    // SHA-256 row leaves are hashed inside the persistent forward rows forms
+    // only; for other shapes the caller runs the separate leaf kernel
+    const bool sha2 = s.leaf_hash == HASH_SHA2;
+    if (sha2 && !(LDE_PB_PERSIST && p.M2 == 1024 && !p.direct && !s.inverse)) fuse = false;
     a.leaves = fuse ? leaves : nullptr;
     if (fused) *fused = fuse;
     a.tw_lo = tw.lo;
@@ -788,13 +925,19 @@ int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* s
     }
     // fused 16-column rows with M2 = 1024: the persistent prefetching pass B
     const bool rows = LDE_PB_PERSIST && fuse && p.M2 == 1024 && !p.direct;
-    RowsFn kr = s.inverse ? lde_pass_b_rows<true> : lde_pass_b_rows<false>;
+    RowsFn kr = s.inverse ? lde_pass_b_rows<true>
+                          : (sha2 ? lde_pass_b_rows<false, HASH_SHA2> : lde_pass_b_rows<false>);
+    // (SHA-256 leaves take the one-workgroup form: under the two-workgroup
+    // form's 128-VGPR budget their 64-round schedule spills)
+    RowsFn kr2 = s.inverse ? lde_pass_b_rows2<true> : lde_pass_b_rows2<false>;
     if (rows) {
         std::lock_guard<std::mutex> g(attr_mu);
-        static bool rows_attr[2] = {};
-        if (!rows_attr[inv]) {
+        static bool rows_attr[3] = {};  // forward BLAKE3, inverse, forward SHA-256
+        const int which = s.inverse ? 1 : (sha2 ? 2 : 0);
+        if (!rows_attr[which]) {
             (void)hipFuncSetAttribute((const void*)kr, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-            rows_attr[inv] = true;
+            (void)hipFuncSetAttribute((const void*)kr2, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            rows_attr[which] = true;
         }
     }
     // other M2 = 1024 shapes with 16 Z columns per tile: the persistent
@@ -858,7 +1001,13 @@ int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* s
         } else if (!p.direct) {
             hipLaunchKernelGGL(ka, ga, dim3(THREADS_A), lds_a, st, a, s0);
         }
-        if (rows) {
+        if (rows && !sha2 && (s.rows_form ? s.rows_form : rows_variant()) == 2) {
+            // two resident workgroups per CU, half a tile of LDS each
+            const uint32_t ntiles = (uint32_t)(g * p.M1);
+            const uint32_t per_cu = (uint32_t)std::min(2, resident_blocks((const void*)kr2, THREADS, ROWS2_LDS));
+            const uint32_t grid = std::min<uint32_t>(ntiles, per_cu * (uint32_t)cu_count());
+            hipLaunchKernelGGL(kr2, dim3(grid), dim3(THREADS), ROWS2_LDS, st, a, s0, ntiles);
+        } else if (rows) {
             // one resident workgroup per CU (the LDS use allows no more)
             const uint32_t ntiles = (uint32_t)(g * p.M1);
             const uint32_t grid = std::min<uint32_t>(ntiles, (uint32_t)cu_count());

@@ -17,6 +17,7 @@
 
 #include "config.h"
 #include "ctx.h"
+#include "profile.h"
 #include "transcript.h"
 
 using namespace whir_gpu;
@@ -135,6 +136,7 @@ struct whir_prover {
     bool committed = false, proved = false;
     std::vector<e3> eval_point, forms_rlc;
     size_t pow_hashes = 0;
+    double commit_ms = 0;         // host wall time of whir_prover_commit (WHIR_PROFILE's commit_total)
     uint32_t hash = HASH_BLAKE3;  // engine of the trees and PoW (params.hash_id)
     // host wall time per phase (include the waits on the device)
     double phase_ms[WHIR_PROVER_PHASES] = {};
@@ -349,13 +351,35 @@ int commit_witness(whir_prover* P, const whir_host::IrsConfig& c, const uint64_t
         Phase ph(P, ext ? PH_ROUND_COMMIT : PH_COMMIT);
         ctx->ext_scratch = P->region + P->L.scratch;
         ctx->ext_scratch_bytes = P->L.scratch_bytes;
+        // WHIR_PROFILE: the stage times of this commit from the context's HIP events
+        const bool prof_on = prof::enabled();
+        const bool was_timing = ctx->timing;
+        double before[3] = {ctx->acc_ms[0], ctx->acc_ms[1], ctx->acc_ms[2]};
+        if (prof_on) ctx->timing = true;
         rc = commit_device(ctx, d_vecs, c.num_vectors, c.vector_size, c.codeword_length, c.depth, w.matrix, w.nodes,
                            root, ext ? 1 : 0);
+        ctx->timing = was_timing;
         ctx->ext_scratch = nullptr;
         ctx->ext_scratch_bytes = 0;
         if (rc) return rc;
+        if (prof_on) {
+            // witness_encoding covers the encode and the row leaves (fused into
+            // the encode's last pass for 16-column base rows)
+            prof::record("prover", c.num_vectors * c.vector_size, "witness_encoding",
+                         (ctx->acc_ms[0] - before[0]) + (ctx->acc_ms[1] - before[1]));
+            prof::record("prover", c.codeword_length, "merkle_build_total", ctx->acc_ms[2] - before[2]);
+        }
     }
     P->ts.message(root, 32);
+    const auto t_ood = std::chrono::steady_clock::now();
+    struct OodRow {
+        size_t n;
+        std::chrono::steady_clock::time_point t0;
+        ~OodRow() {
+            prof::record("prover", n, "ood_evaluation",
+                         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+        }
+    } ood_row{c.vector_size, t_ood};
     Phase ph(P, PH_OOD);
     w.ood_points.clear();
     w.ood_values.clear();
@@ -754,11 +778,15 @@ int whir_prover_commit(whir_prover* P, const uint64_t* d_vectors, size_t num_vec
     if (num_vectors != P->cfg.initial.num_vectors) return bad_arg("num_vectors must equal batch_size");
     std::lock_guard<std::recursive_mutex> g(P->ctx->mu);
     HashScope hs(P->ctx, P->hash);
+    prof::PhaseGuard phase("commit");
+    const auto t0 = std::chrono::steady_clock::now();
     P->d_vectors = d_vectors;
     P->num_vectors = num_vectors;
     int rc = commit_witness(P, P->cfg.initial, d_vectors, false, P->initial, P->L.init_matrix, P->L.init_nodes);
     if (rc) return rc;
     P->committed = true;
+    P->commit_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    prof::record("prover", P->size, "commit_total", P->commit_ms);
     return 0;
 }
 
@@ -770,8 +798,16 @@ int whir_prover_prove(whir_prover* P, size_t num_forms, const uint64_t* points, 
     if (num_forms && (!points || !evaluations)) return bad_arg("null constraint arrays");
     std::lock_guard<std::recursive_mutex> g(P->ctx->mu);
     HashScope hs(P->ctx, P->hash);
-    int rc = prove_impl(P, num_forms, points, evaluations);
-    if (rc) return rc;
+    double prove_ms = 0;
+    {
+        prof::PhaseGuard phase("prove");
+        const auto t0 = std::chrono::steady_clock::now();
+        int rc = prove_impl(P, num_forms, points, evaluations);
+        if (rc) return rc;
+        prove_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        prof::record("prover", P->size, "prove_total", prove_ms);
+    }
+    prof::record("prover", P->size, "total_prover", P->commit_ms + prove_ms);
     P->proved = true;
     if (out_eval_point) put_e3s(P->eval_point, out_eval_point);
     if (out_forms_rlc) put_e3s(P->forms_rlc, out_forms_rlc);

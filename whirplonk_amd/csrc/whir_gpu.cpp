@@ -22,6 +22,7 @@
 #include "ctx.h"
 #include "gl.h"
 #include "kernels.h"
+#include "profile.h"
 
 using namespace whir_gpu;
 
@@ -30,6 +31,47 @@ std::string& last_error() {
     thread_local std::string e;
     return e;
 }
+std::mutex& alloc_mutex() {
+    static std::mutex m;
+    return m;
+}
+
+namespace prof {
+namespace {
+thread_local const char* t_phase = "none";
+std::mutex g_csv_mu;
+bool g_header = false;
+}  // namespace
+
+bool enabled() {
+    static const bool on = [] {
+        const char* v = std::getenv("WHIR_PROFILE");
+        return v && v[0] && v[0] != '0' && v[0] != 'f' && v[0] != 'F' && v[0] != 'n' && v[0] != 'N';
+    }();
+    return on;
+}
+
+void record(const char* mode, size_t size, const char* stage, double ms) {
+    if (!enabled()) return;
+    static const char* const keep[] = {"witness_encoding", "witness_compact", "witness_roots_prepare",
+                                       "witness_h2d", "witness_gpu_alloc", "witness_rs_encode",
+                                       "witness_leaf_hash", "witness_d2h", "witness_resize_outputs",
+                                       "merkle_leaf_total", "merkle_build_total", "ood_evaluation",
+                                       "commit_total", "prove_total", "total_prover"};
+    bool kept = false;
+    for (const char* k : keep) kept |= std::strcmp(k, stage) == 0;
+    if (!kept) return;
+    std::lock_guard<std::mutex> g(g_csv_mu);
+    if (!g_header) {
+        std::fprintf(stderr, "mode,phase,size,stage,time_ms\n");
+        g_header = true;
+    }
+    std::fprintf(stderr, "%s,%s,%zu,%s,%.6f\n", mode, t_phase, size, stage, ms);
+}
+
+PhaseGuard::PhaseGuard(const char* phase) : prev(t_phase) { t_phase = phase; }
+PhaseGuard::~PhaseGuard() { t_phase = prev; }
+}  // namespace prof
 }  // namespace whir_gpu
 
 namespace {
@@ -125,7 +167,12 @@ int rs_shape(size_t num_polys, size_t poly_size, size_t cw, size_t depth, LdeSha
 
 int run_lde(whir_ctx* ctx, LdeShape s, const uint64_t* in, uint64_t* out, uint8_t* leaves = nullptr,
             bool* fused = nullptr) {
-    if (!s.lg_m2) s.lg_m2 = tuned_split(s);
+    if (!s.lg_m2) {
+        // a tuned entry packs log2 M2 (bits 0-7) and the pass-B form (bits 8-15)
+        const uint32_t t = tuned_split(s);
+        s.lg_m2 = t & 0xFF;
+        if (!s.rows_form) s.rows_form = t >> 8;
+    }
     Twiddles tw;
     int rc = ctx->twiddles(log2u(s.n), s.inverse, tw);
     if (rc) return rc;
@@ -149,11 +196,13 @@ int run_lde(whir_ctx* ctx, LdeShape s, const uint64_t* in, uint64_t* out, uint8_
 // RS encode; for base-field input, leaves (when non-null) may be filled by the
 // fused last pass, reported through *fused.
 int rs_encode_any(whir_ctx* ctx, const uint64_t* d_in, size_t num_polys, size_t poly_size, size_t cw, size_t depth,
-                  uint64_t* d_out, int ext, uint8_t* leaves = nullptr, bool* fused = nullptr) {
+                  uint64_t* d_out, int ext, uint8_t* leaves = nullptr, bool* fused = nullptr,
+                  uint32_t leaf_hash = HASH_BLAKE3) {
     LdeShape s;
     if (fused) *fused = false;
     int rc = rs_shape(num_polys, poly_size, cw, depth, s);
     if (rc) return rc;
+    s.leaf_hash = leaf_hash;
     if (!ext) return run_lde(ctx, s, d_in, d_out, leaves, fused);
     // the three components of every Ext3 column in one transform
     s.C *= 3;
@@ -296,14 +345,16 @@ int whir_gpu::commit_device(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     int rc;
     bool fused = false;
-    const bool sha2 = ctx->hash == HASH_SHA2;  // fused leaves are BLAKE3's
+    const bool sha2 = ctx->hash == HASH_SHA2;
     if ((rc = ctx->mark(0))) return rc;
-    if ((rc = rs_encode_any(ctx, d_coeffs, num_polys, poly_size, cw, depth, d_matrix, ext, sha2 ? nullptr : d_nodes,
-                            &fused)))
+    // the row leaves go straight into the tree's first nodes, hashed inside
+    // the encode's last pass where the shape allows it
+    if ((rc = rs_encode_any(ctx, d_coeffs, num_polys, poly_size, cw, depth, d_matrix, ext, d_nodes, &fused,
+                            ctx->hash)))
         return rc;
     if ((rc = ctx->mark(1))) return rc;
     size_t row_u64 = num_polys * depth * (ext ? 3 : 1);
-    if (sha2)
+    if (!fused && sha2)
         sha256_leaves_launch(d_matrix, cw, (uint32_t)row_u64, d_nodes, ctx->stream);
     else if (!fused)
         leaf_hash_launch(d_matrix, cw, (uint32_t)row_u64, d_nodes, ctx->stream);
@@ -745,6 +796,27 @@ int whir_gpu_ntt_batch_transpose(uint64_t* values, size_t total, size_t ntt_size
 }
 int whir_gpu_intt_batch(uint64_t* values, size_t total, size_t size) { return ntt_host(values, total, size, 1); }
 
+// HIP events bracketing the stages of one host-pointer call, for the
+// WHIR_PROFILE rows (created only when profiling is on).
+struct StageEvents {
+    hipEvent_t ev[5] = {};
+    bool on = prof::enabled();
+    ~StageEvents() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    void mark(int k, hipStream_t st) {
+        if (!on) return;
+        if (!ev[k] && hipEventCreate(&ev[k]) != hipSuccess) ev[k] = nullptr;
+        if (ev[k]) (void)hipEventRecord(ev[k], st);
+    }
+    double ms(int a, int b) {
+        float t = 0;
+        if (!on || !ev[a] || !ev[b] || hipEventElapsedTime(&t, ev[a], ev[b]) != hipSuccess) return 0;
+        return t;
+    }
+};
+
 static int rs_host(const uint64_t* coeffs, uint64_t* out_matrix, uint8_t* out_hashes, size_t num_polys,
                    size_t poly_size, size_t cw, size_t depth) {
     return with_default([&](whir_ctx* ctx) {
@@ -752,20 +824,39 @@ static int rs_host(const uint64_t* coeffs, uint64_t* out_matrix, uint8_t* out_ha
         int rc = rs_shape(num_polys, poly_size, cw, depth, s);
         if (rc) return rc;
         size_t nin = num_polys * poly_size, nout = cw * num_polys * depth;
+        const auto t_alloc = std::chrono::steady_clock::now();
         if ((rc = ctx->ensure(ctx->hin, nin * 8))) return rc;
         if ((rc = ctx->ensure(ctx->hout, nout * 8))) return rc;
         if (out_hashes && (rc = ctx->ensure(ctx->hnodes, cw * 32))) return rc;
+        const double alloc_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_alloc).count();
         uint64_t* din = (uint64_t*)ctx->hin.p;
         uint64_t* dout = (uint64_t*)ctx->hout.p;
         uint8_t* dleaves = out_hashes ? (uint8_t*)ctx->hnodes.p : nullptr;
         bool fused = false;
+        StageEvents ev;
+        ev.mark(0, ctx->stream);
         HIPCHK(hipMemcpyAsync(din, coeffs, nin * 8, hipMemcpyHostToDevice, ctx->stream));
+        ev.mark(1, ctx->stream);
         if ((rc = run_lde(ctx, s, din, dout, dleaves, &fused))) return rc;
+        ev.mark(2, ctx->stream);
         if (out_hashes && !fused) leaf_hash_launch(dout, cw, (uint32_t)(num_polys * depth), dleaves, ctx->stream);
         HIPCHK(hipGetLastError());
+        ev.mark(3, ctx->stream);
         HIPCHK(hipMemcpyAsync(out_matrix, dout, nout * 8, hipMemcpyDeviceToHost, ctx->stream));
         if (out_hashes) HIPCHK(hipMemcpyAsync(out_hashes, ctx->hnodes.p, cw * 32, hipMemcpyDeviceToHost, ctx->stream));
+        ev.mark(4, ctx->stream);
         HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (ev.on) {
+            // the row leaves of 16-column rows are hashed inside the encode's
+            // last pass, so witness_rs_encode includes them and
+            // witness_leaf_hash is what remains (the separate leaf kernel)
+            prof::record("gpu", nout, "witness_gpu_alloc", alloc_ms);
+            prof::record("gpu", nout, "witness_h2d", ev.ms(0, 1));
+            prof::record("gpu", nout, "witness_rs_encode", ev.ms(1, 2));
+            if (out_hashes) prof::record("gpu", nout, "witness_leaf_hash", ev.ms(2, 3));
+            prof::record("gpu", nout, "witness_d2h", ev.ms(3, 4));
+        }
         return 0;
     });
 }
