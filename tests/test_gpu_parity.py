@@ -219,18 +219,25 @@ def test_dev_commit(orc, shape):
     assert root == wn[-32:].tobytes()
 
 
-def test_dev_commit_ext3(orc):
-    num_polys, poly_size, cw, depth = 1, 1 << 12, 1 << 11, 16
-    x = rand(99, 3 * poly_size)
+@pytest.mark.parametrize("poly_size,cw", [(1 << 12, 1 << 11), (1 << 15, 1 << 13), (1 << 16, 1 << 14),
+                                          (1 << 16, 1 << 16)])
+def test_dev_commit_ext3(orc, poly_size, cw):
+    """Ext3 round commits: the 384-byte row leaves are hashed inside the
+    encode (three 16-column sub-tiles per row) once the message length per
+    column reaches 2048; every node of the tree against the oracle."""
+    num_polys, depth = 1, 16
+    x = rand(99 + poly_size + cw, 3 * poly_size)
     with W.Context() as ctx:
         dx = ctx.upload(x)
         dm = ctx.alloc(24 * cw * depth)
         dn = ctx.alloc(32 * W.merkle_num_nodes(cw))
         root = ctx.commit(dx, num_polys, poly_size, cw, depth, dm, dn, ext3=True)
         m = ctx.download(dm, 3 * cw * depth)
+        nodes = ctx.download(dn, 32 * W.merkle_num_nodes(cw), np.uint8)
     want = orc.rs_encode_ext3(x, num_polys, poly_size, cw, depth)
     assert np.array_equal(m, want)
     wn = orc.merkle_build(orc.commit_leaves_ext3(want, cw, depth), cw)
+    assert np.array_equal(nodes, wn)
     assert root == wn[-32:].tobytes()
 
 

@@ -558,6 +558,114 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4, 4)))
     }
 }
 
+// Pass B of an Ext3 codeword of 16 Ext3 columns (split3: 48 base columns, a
+// row is 384 bytes = six BLAKE3 blocks) with M2 = 1024 and the row leaves
+// fused.  A tile (coset slot, i1) is transformed as three sub-tiles of 16
+// base columns each, in order, like lde_pass_b_rows's tile; sub-tile q's
+// columns 16q .. 16q + 15 are bytes 128q .. 128q + 127 of every row, i.e.
+// blocks 2q and 2q + 1 of its leaf, so each lane carries the chaining values
+// of its two rows in registers from one sub-tile to the next (one chunk:
+// CHUNK_START on block 0, CHUNK_END | ROOT on block 5, counter 0).  The next
+// sub-tile's loads are in flight during the current one's step 2, hashing and
+// stores; the inter-pass twiddles are computed once per tile.
+template <bool INV>
+__global__ __launch_bounds__(THREADS) void lde_pass_b_rows_e3(LdeArgs a, uint64_t s0, uint32_t ntiles) {
+    constexpr int M2 = 1024, R = 32, PAD = M2 + M2 / R + 1, SUB = 3, COLS = 3 * FUSE_COLS;
+    static_assert(PAD == 1057, "rows_idx assumes the padded 1024-point layout");
+    extern __shared__ uint64_t lds[];
+    uint64_t* twt2 = lds + 16 * PAD;  // inter-pass twiddles of this tile and the next
+    const uint32_t kmask = (1u << a.lgNK) - 1;
+    uint64_t pre[R];
+    const uint32_t nwork = ntiles * SUB;  // (tile, sub-tile) steps of the whole launch
+    auto tile_of = [&](uint32_t w) { return blockIdx.x + (w / SUB) * gridDim.x; };
+    auto fetch = [&](uint32_t tile, uint32_t sub, uint32_t tid) {
+        const uint64_t* zb = a.z + (uint64_t)(tile >> a.lgNK) * a.zstride + (uint64_t)(tile & kmask) * (COLS * M2) +
+                             (uint64_t)sub * (FUSE_COLS * M2);
+        const uint32_t off = (tid / R) * M2 + tid % R;
+#pragma unroll
+        for (int n1 = 0; n1 < R; ++n1) pre[n1] = zb[off + R * n1];
+    };
+    auto tile_i1 = [&](uint32_t t) { return s0 + (t >> a.lgNK) + a.E * (t & kmask); };
+    auto twiddles = [&](uint32_t t, uint64_t* dst, uint32_t tid) {
+        const uint64_t i1 = tile_i1(t);
+        for (uint32_t j2 = tid; j2 < M2; j2 += THREADS) dst[j2] = twid(a, (i1 * j2) & (a.n - 1));
+    };
+    // this workgroup's steps: w = SUB * (local tile number) + sub-tile
+    uint32_t w = 0;
+    if (tile_of(w) < ntiles) {
+        fetch(tile_of(w), 0, threadIdx.x);
+        twiddles(tile_of(w), twt2, threadIdx.x);
+    }
+    __syncthreads();
+    uint32_t cv[2][8];
+    for (; tile_of(w) < ntiles && w < nwork; ++w) {
+        uint32_t tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));  // lane indices re-derived per step (see lde_pass_b_rows)
+        const uint32_t tile = tile_of(w), sub = w % SUB, it = w / SUB;
+        const uint32_t c = tid / R, n2 = tid % R;
+        const uint64_t i1 = tile_i1(tile);
+        const uint64_t* twt = twt2 + (it & 1) * M2;
+        uint64_t v[R];
+#pragma unroll
+        for (int n1 = 0; n1 < R; ++n1) v[n1] = gl::mul(pre[n1], twt[n2 + R * n1]);
+        dft::dft_reg<R, INV>(v);
+#pragma unroll
+        for (int k1 = 1; k1 < R; ++k1) v[k1] = gl::mul(v[k1], a.t1024[(n2 * k1) & 1023]);
+#pragma unroll
+        for (int k1 = 0; k1 < R; ++k1) lds[c * PAD + dft::pidx<M2>(R * k1 + n2)] = v[k1];
+        // the next step's loads (and, starting a new tile, its twiddles into
+        // the other table, last read two tiles ago)
+        if (tile_of(w + 1) < ntiles) {
+            fetch(tile_of(w + 1), (w + 1) % SUB, tid);
+            if (sub == SUB - 1) twiddles(tile_of(w + 1), twt2 + ((it + 1) & 1) * M2, tid);
+        }
+        __syncthreads();
+        {
+            const uint32_t cb = tid % 16, k1 = tid / 16;
+            dft::lds_dft_step2<M2, INV>(lds + cb * PAD, k1, v);
+#pragma unroll
+            for (int k2 = 0; k2 < R; ++k2) lds[rows_idx(k1 + R * k2, cb)] = v[k2];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < M2 / THREADS; ++h) {
+            const uint32_t r = tid + h * THREADS;
+            if (sub == 0) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) cv[h][i] = b3::iv(i);
+            }
+#pragma unroll
+            for (int blk = 0; blk < 2; ++blk) {
+                uint32_t m[16];
+#pragma unroll
+                for (int cc = 0; cc < 8; ++cc) {
+                    const uint64_t x = lds[rows_idx(r, 8 * blk + cc)];
+                    m[2 * cc] = (uint32_t)x;
+                    m[2 * cc + 1] = (uint32_t)(x >> 32);
+                }
+                const uint32_t flags =
+                    (sub == 0 && blk == 0) ? b3::CHUNK_START : ((sub == SUB - 1 && blk == 1) ? (b3::CHUNK_END | b3::ROOT) : 0u);
+                b3::compress(cv[h], m, 0, 64, flags, cv[h]);
+            }
+            if (sub == SUB - 1) {
+                uint4* dst = reinterpret_cast<uint4*>(a.leaves + 32 * (i1 + a.L * r));
+                dst[0] = make_uint4(cv[h][0], cv[h][1], cv[h][2], cv[h][3]);
+                dst[1] = make_uint4(cv[h][4], cv[h][5], cv[h][6], cv[h][7]);
+            }
+        }
+        // this sub-tile's 16 columns of every row: 8 lanes per 128-byte segment
+#pragma unroll
+        for (int h = 0; h < 16 * M2 / (2 * THREADS); ++h) {
+            const uint32_t q = tid + h * THREADS;
+            const uint32_t r = q / 8, cp = 2 * (q % 8);
+            const uint64_t x0 = lds[rows_idx(r, cp)], x1 = lds[rows_idx(r, cp + 1)];
+            *reinterpret_cast<ulonglong2*>(a.out + (i1 + a.L * r) * COLS + FUSE_COLS * sub + cp) =
+                make_ulonglong2(x0, x1);
+        }
+        __syncthreads();  // the next step rewrites the LDS tile (and a twiddle table)
+    }
+}
+
 // Pass B with M2 = 1024 and 16 Z columns per tile (TB * CB = 16), no fused
 // leaves: Ext3 codewords (split3, 48 base columns), column-major output and
 // the plain encode.  Persistent with the next tile's loads in flight, as
@@ -875,8 +983,13 @@ int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* s
     // only; for other shapes the caller runs the separate leaf kernel
     const bool sha2 = s.leaf_hash == HASH_SHA2;
     if (sha2 && !(LDE_PB_PERSIST && p.M2 == 1024 && !p.direct && !s.inverse)) fuse = false;
-    a.leaves = fuse ? leaves : nullptr;
-    if (fused) *fused = fuse;
+    // Ext3 rows of 16 Ext3 columns (48 base columns): BLAKE3 leaves fused into
+    // the three-sub-tile persistent pass B (forward, M2 = 1024, with a first pass)
+    const bool fuse3 = leaves && !sha2 && s.split3 && s.row_major && s.C == 3 * FUSE_COLS && p.TB == 1 &&
+                       p.CB == FUSE_COLS && s.out_el == 1 && LDE_PB_PERSIST && p.M2 == 1024 && !p.direct &&
+                       !s.inverse;
+    a.leaves = (fuse || fuse3) ? leaves : nullptr;
+    if (fused) *fused = fuse || fuse3;
     a.tw_lo = tw.lo;
     a.tw_hi = tw.hi;
     a.t1024 = tw.t1024;
@@ -942,7 +1055,16 @@ int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* s
     }
     // other M2 = 1024 shapes with 16 Z columns per tile: the persistent
     // prefetching pass B without leaves
-    const bool cols = LDE_PB_PERSIST && !fuse && p.M2 == 1024 && !p.direct && p.TB * p.CB == 16 &&
+    RowsFn kr3 = lde_pass_b_rows_e3<false>;
+    if (fuse3) {
+        std::lock_guard<std::mutex> g(attr_mu);
+        static bool e3_attr = false;
+        if (!e3_attr) {
+            (void)hipFuncSetAttribute((const void*)kr3, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            e3_attr = true;
+        }
+    }
+    const bool cols = LDE_PB_PERSIST && !fuse && !fuse3 && p.M2 == 1024 && !p.direct && p.TB * p.CB == 16 &&
                       p.threads_b == THREADS;
     ColsFn kc = s.inverse ? lde_pass_b_cols<true> : lde_pass_b_cols<false>;
     if (cols) {
@@ -1013,6 +1135,12 @@ int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* s
             const uint32_t grid = std::min<uint32_t>(ntiles, (uint32_t)cu_count());
             const size_t lds_rows = (16 * padded(1024) + 2 * 1024) * 8;  // DFT buffer + two twiddle tables
             hipLaunchKernelGGL(kr, dim3(grid), dim3(THREADS), lds_rows, st, a, s0, ntiles);
+        } else if (fuse3) {
+            // one resident workgroup per CU (the LDS use allows no more)
+            const uint32_t ntiles = (uint32_t)(g * p.M1);
+            const uint32_t grid = std::min<uint32_t>(ntiles, (uint32_t)cu_count());
+            const size_t lds_rows = (16 * padded(1024) + 2 * 1024) * 8;  // DFT buffer + two twiddle tables
+            hipLaunchKernelGGL(kr3, dim3(grid), dim3(THREADS), lds_rows, st, a, s0, ntiles);
         } else if (cols) {
             const uint32_t ntiles = (uint32_t)(g * gb.x);
             const uint32_t grid = std::min<uint32_t>(ntiles, (uint32_t)cu_count());

@@ -204,11 +204,12 @@ int rs_encode_any(whir_ctx* ctx, const uint64_t* d_in, size_t num_polys, size_t 
     if (rc) return rc;
     s.leaf_hash = leaf_hash;
     if (!ext) return run_lde(ctx, s, d_in, d_out, leaves, fused);
-    // the three components of every Ext3 column in one transform
+    // the three components of every Ext3 column in one transform; the row
+    // leaves of 16 Ext3 columns may be hashed inside it (BLAKE3)
     s.C *= 3;
     s.in_el = 3;
     s.split3 = true;
-    return run_lde(ctx, s, d_in, d_out);
+    return run_lde(ctx, s, d_in, d_out, leaves, fused);
 }
 
 template <typename F>
