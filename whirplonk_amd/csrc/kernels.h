@@ -109,11 +109,12 @@ void sumcheck_fold_launch(const uint64_t* in, bool in_base, uint64_t n, const ui
 
 // sum_i f_i z^i over Ext3 (f base or Ext3 AoS): block b's share lands in
 // partials[3b..3b+3) and the caller sums the returned number of blocks.
-// zp.w[3l..3l+3) = z^(2^l) for l <= HORNER_MAX_LOGT, passed by value.
+// zp.w[3l..3l+3) = z^(2^l) for l < HORNER_POWS, passed by value.
 constexpr int HORNER_MAX_BLOCKS = 1024;
 constexpr uint32_t HORNER_MAX_LOGT = 18;
+constexpr uint32_t HORNER_POWS = 32;
 struct HornerPowers {
-    uint64_t w[3 * (HORNER_MAX_LOGT + 1)];
+    uint64_t w[3 * HORNER_POWS];
 };
 unsigned horner_launch(const uint64_t* f, bool f_ext3, uint64_t n, const HornerPowers& zp, uint64_t* partials,
                        hipStream_t st);
@@ -157,11 +158,15 @@ constexpr uint32_t TERM_WORDS = 11;
 // nb * 2^(nv-L) Ext3, lo_b nb * 2^L u64.
 void term_tables_launch(const uint64_t* d_desc_e, uint32_t ne, const uint64_t* d_desc_b, uint32_t nb,
                         const uint64_t* d_coords, uint32_t nv, uint32_t L, uint64_t* hi_e, uint64_t* lo_e,
-                        uint64_t* hi_b, uint64_t* lo_b, hipStream_t st);
+                        uint64_t* hi_b, uint64_t* lo_b, uint32_t* hl, hipStream_t st);
+// bytes of the limb form of the hi tables of `terms` terms (hl above; 0 when
+// the sweep does not use it, 2^L < 256): written by term_tables_launch when
+// hl is not null, read by term_accumulate_launch
+uint64_t term_limb_bytes(uint32_t nv, uint32_t L, uint32_t terms);
 // acc[i] = (accumulate ? acc[i] : 0) + sum over both term groups
 void term_accumulate_launch(uint64_t* d_acc, uint64_t n, uint32_t L, const uint64_t* hi_e, const uint64_t* lo_e,
-                            uint32_t te, const uint64_t* hi_b, const uint64_t* lo_b, uint32_t tb, bool accumulate,
-                            hipStream_t st);
+                            uint32_t te, const uint64_t* hi_b, const uint64_t* lo_b, uint32_t tb, const uint32_t* hl,
+                            bool accumulate, hipStream_t st);
 // sum_i f_i hi[i >> L] lo[i & mask] into partials[0..3) (device_sum), or
 // per block (partials[3b..3b+3)); returns the number of blocks
 unsigned mle_eval_launch(const uint64_t* f, bool f_base, uint64_t n, uint32_t L, const uint64_t* hi,

@@ -120,6 +120,12 @@ __global__ __launch_bounds__(256) void pow_batch_kernel(PowBatchArgs a, unsigned
 // (3), offset of the eq coordinates in `coords` (in Ext3 elements).
 enum : uint64_t { TERM_EQ = 0, TERM_POW = 1 };
 
+// Limb form of a hi entry x (Ext3) for term_sweep_kernel: TERM_LIMB_WORDS
+// u32, the 22/22/20-bit limbs of x.c0, x.c1, x.c2, 2 x.c1, 2 x.c2 (the
+// doubled components are the factors X^3 = 2 brings into an Ext3 product),
+// padded to one 64-byte scalar load.
+constexpr uint32_t TERM_LIMB_WORDS = 16;
+
 // hi tables: H entries per term (Ext3); lo tables: 2^L entries per term, Ext3
 // for eq terms and Ext3 powers, u64 for base-field powers.  Both term groups
 // in one launch: blockIdx.y < ne is term y of the Ext3 group (desc_e, hi_e,
@@ -129,7 +135,7 @@ __global__ __launch_bounds__(256) void term_tables_kernel(const uint64_t* __rest
                                                           const uint64_t* __restrict__ coords, uint32_t nv,
                                                           uint32_t L, uint32_t ne, uint64_t* __restrict__ hi_e,
                                                           uint64_t* __restrict__ lo_e, uint64_t* __restrict__ hi_b,
-                                                          uint64_t* __restrict__ lo_b) {
+                                                          uint64_t* __restrict__ lo_b, uint32_t* __restrict__ hl) {
     const int lo_base = blockIdx.y >= ne;
     const uint32_t t = lo_base ? blockIdx.y - ne : blockIdx.y;
     const uint64_t* d = (lo_base ? desc_b : desc_e) + (uint64_t)TERM_WORDS * t;
@@ -139,6 +145,19 @@ __global__ __launch_bounds__(256) void term_tables_kernel(const uint64_t* __rest
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const e3 s = {d[1], d[2], d[3]};
     const e3 one = {1, 0, 0};
+    // limb form of hi entry (term y of the launch, row j) for the sweep
+    auto limbs = [&](e3 x) {
+        if (!hl) return;
+        uint32_t* o = hl + TERM_LIMB_WORDS * ((uint64_t)blockIdx.y * H + j);
+        const uint64_t v[5] = {x.c0, x.c1, x.c2, gl::dbl(x.c1), gl::dbl(x.c2)};
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            o[3 * k] = (uint32_t)v[k] & 0x3FFFFF;
+            o[3 * k + 1] = (uint32_t)(v[k] >> 22) & 0x3FFFFF;
+            o[3 * k + 2] = (uint32_t)(v[k] >> 44);
+        }
+        o[15] = 0;
+    };
     if (d[0] == TERM_EQ) {
         const uint64_t* x = coords + 3 * d[10];
         if (j < H) {
@@ -150,6 +169,7 @@ __global__ __launch_bounds__(256) void term_tables_kernel(const uint64_t* __rest
                 acc = gl::e3_mul(acc, bit ? xk : gl::e3_sub(one, xk));
             }
             st3(hi, (uint64_t)t * H + j, acc);
+            limbs(acc);
         }
         if (j < LO) {
             e3 acc = one;
@@ -162,7 +182,11 @@ __global__ __launch_bounds__(256) void term_tables_kernel(const uint64_t* __rest
         }
     } else {
         const e3 z = {d[4], d[5], d[6]}, Z = {d[7], d[8], d[9]};
-        if (j < H) st3(hi, (uint64_t)t * H + j, gl::e3_mul(s, gl::e3_pow(Z, j)));
+        if (j < H) {
+            const e3 x = gl::e3_mul(s, gl::e3_pow(Z, j));
+            st3(hi, (uint64_t)t * H + j, x);
+            limbs(x);
+        }
         if (j < LO) {
             if (lo_base) {
                 lo[(uint64_t)t * LO + j] = gl::pow(z.c0, j);
@@ -174,6 +198,7 @@ __global__ __launch_bounds__(256) void term_tables_kernel(const uint64_t* __rest
 }
 
 // acc[i] (+)= sum over Ext3-lo terms of hi*lo + sum over base-lo terms of hi*lo
+// (2^L < 256: one lane per entry).
 __global__ __launch_bounds__(256) void term_accumulate_kernel(uint64_t* __restrict__ acc, uint64_t n, uint32_t L,
                                                               const uint64_t* __restrict__ hi_e,
                                                               const uint64_t* __restrict__ lo_e, uint32_t te,
@@ -189,6 +214,160 @@ __global__ __launch_bounds__(256) void term_accumulate_kernel(uint64_t* __restri
     for (uint32_t t = 0; t < te; ++t) gl::e3_mac(s, ld3(hi_e, t * H + h), ld3(lo_e, t * LO + l));
     for (uint32_t t = 0; t < tb; ++t) gl::e3_mac_base(s, ld3(hi_b, t * H + h), lo_b[t * LO + l]);
     e3 sum = gl::e3acc_reduce(s);
+    if (accumulate) sum = gl::e3_add(sum, ld3(acc, i));
+    st3(acc, i, sum);
+}
+
+// x * 2^(22 i + 32 j) summed over the six limb-product accumulators A[2 i + j]
+// (any values below 2^64), reduced mod p
+__device__ __forceinline__ uint64_t limb_fold(const uint64_t* A) {
+    uint64_t v = gl::canon(A[0]);
+    v = gl::add(v, gl::mul_pow2<32>(A[1]));
+    v = gl::add(v, gl::mul_pow2<22>(A[2]));
+    v = gl::add(v, gl::mul_pow2<54>(A[3]));
+    v = gl::add(v, gl::mul_pow2<44>(A[4]));
+    v = gl::add(v, gl::mul_pow2<76>(A[5]));
+    return v;
+}
+
+// The same sweep for 2^L >= 256.  Workgroup w covers l = 256 (w % (2^L / 256))
+// + tid of row h = w / (2^L / 256), so every hi entry it reads is the same for
+// all its lanes (workgroup-uniform addresses: scalar loads).  Terms go in
+// groups of TERM_GROUP whose lo loads are all issued before their
+// multiply-adds, so a lane waits once per group instead of once per term.
+// No per-term 128-bit products: each (uniform) hi component x is split into
+// limbs x_0 + 2^22 x_1 + 2^44 x_2 (22, 22, 20 bits) and the lane's lo value y
+// into 32-bit halves y_0, y_1; the six products x_i y_j < 2^54 add into 64-bit
+// accumulators A_ij with one mad each and no carries, and
+// sum_ij A_ij 2^(22 i + 32 j) is reduced mod p once, or once every
+// TERM_LIMB_PASS products per accumulator: a base-field term (one per STIR
+// query, usually most of the terms) adds one, an Ext3 term three (Ext3 x Ext3
+// with X^3 = 2, the doubled x components computed once per term).
+constexpr uint32_t TERM_GROUP = 4;
+constexpr uint32_t TERM_LIMB_PASS = 1024;  // 1024 * (2^22 - 1) * (2^32 - 1) < 2^64
+
+struct LimbAcc {
+    uint64_t a[3][6];  // component, slot 2 i + j
+};
+
+// A += x * y for x given by its limbs xl[0..3), y any u64
+__device__ __forceinline__ void limb_mac(uint64_t* A, const uint32_t* xl, uint64_t y) {
+    const uint32_t y0 = (uint32_t)y, y1 = (uint32_t)(y >> 32);
+    A[0] += (uint64_t)xl[0] * y0;
+    A[1] += (uint64_t)xl[0] * y1;
+    A[2] += (uint64_t)xl[1] * y0;
+    A[3] += (uint64_t)xl[1] * y1;
+    A[4] += (uint64_t)xl[2] * y0;
+    A[5] += (uint64_t)xl[2] * y1;
+}
+
+// acc += x * y, x Ext3 in limb form (TERM_LIMB_WORDS, uniform), y Ext3
+__device__ __forceinline__ void limb_mac_e3(LimbAcc& A, const uint32_t* x, e3 y) {
+    limb_mac(A.a[0], x + 0, y.c0);   // x0 y0
+    limb_mac(A.a[0], x + 12, y.c1);  // 2 x2 y1
+    limb_mac(A.a[0], x + 9, y.c2);   // 2 x1 y2
+    limb_mac(A.a[1], x + 3, y.c0);   // x1 y0
+    limb_mac(A.a[1], x + 0, y.c1);   // x0 y1
+    limb_mac(A.a[1], x + 12, y.c2);  // 2 x2 y2
+    limb_mac(A.a[2], x + 6, y.c0);   // x2 y0
+    limb_mac(A.a[2], x + 3, y.c1);   // x1 y1
+    limb_mac(A.a[2], x + 0, y.c2);   // x0 y2
+}
+
+// acc += x * y, x Ext3 in limb form (uniform), y base
+__device__ __forceinline__ void limb_mac_base(LimbAcc& A, const uint32_t* x, uint64_t y) {
+    limb_mac(A.a[0], x + 0, y);
+    limb_mac(A.a[1], x + 3, y);
+    limb_mac(A.a[2], x + 6, y);
+}
+
+// sum_ij A_ij 2^(22 i + 32 j) mod p of one component: the six shifted
+// accumulators summed exactly in 192 bits (acc128), reduced once
+__device__ __forceinline__ uint64_t limb_reduce(const uint64_t* a) {
+    gl::acc128 s = {a[0], 0, 0};
+    gl::acc_add(s, a[1] << 32, a[1] >> 32);
+    gl::acc_add(s, a[2] << 22, a[2] >> 42);
+    gl::acc_add(s, a[3] << 54, a[3] >> 10);
+    gl::acc_add(s, a[4] << 44, a[4] >> 20);
+    gl::acc_add(s, 0, a[5] << 12);
+    s.top += a[5] >> 52;
+    return gl::acc_reduce(s);
+}
+
+// the three components' sums mod p; clears A
+__device__ __forceinline__ e3 limb_flush(LimbAcc& A) {
+    const e3 r = {limb_reduce(A.a[0]), limb_reduce(A.a[1]), limb_reduce(A.a[2])};
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) A.a[c][k] = 0;
+    return r;
+}
+
+__global__ __launch_bounds__(256) void term_sweep_kernel(uint64_t* __restrict__ acc, uint64_t H, uint32_t L,
+                                                         const uint32_t* __restrict__ hl,
+                                                         const uint64_t* __restrict__ lo_e, uint32_t te,
+                                                         const uint64_t* __restrict__ lo_b, uint32_t tb,
+                                                         int accumulate) {
+    const uint64_t LO = 1ull << L;
+    const uint32_t lblocks = (uint32_t)(LO >> 8);
+    const uint64_t l = (uint64_t)(blockIdx.x % lblocks) * 256 + threadIdx.x;
+    const uint64_t h = blockIdx.x / lblocks;
+    LimbAcc A;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) A.a[c][k] = 0;
+    e3 sum = {0, 0, 0};
+    // products added to each accumulator since the last flush (uniform)
+    uint32_t used = 0;
+    auto reserve = [&](uint32_t k) {
+        if (used + k > TERM_LIMB_PASS) {
+            sum = gl::e3_add(sum, limb_flush(A));
+            used = 0;
+        }
+        used += k;
+    };
+    // Ext3 terms (few: eq constraints and out-of-domain samples)
+    uint32_t t = 0;
+    for (; t + TERM_GROUP <= te; t += TERM_GROUP) {
+        e3 lv[TERM_GROUP];
+#pragma unroll
+        for (uint32_t u = 0; u < TERM_GROUP; ++u) lv[u] = ld3(lo_e, (uint64_t)(t + u) * LO + l);
+        reserve(3 * TERM_GROUP);
+#pragma unroll
+        for (uint32_t u = 0; u < TERM_GROUP; ++u) limb_mac_e3(A, hl + TERM_LIMB_WORDS * ((uint64_t)(t + u) * H + h), lv[u]);
+    }
+    for (; t < te; ++t) {
+        reserve(3);
+        limb_mac_e3(A, hl + TERM_LIMB_WORDS * ((uint64_t)t * H + h), ld3(lo_e, (uint64_t)t * LO + l));
+    }
+    // base-field terms (their limb rows follow the Ext3 terms'): the next
+    // group's lo loads are in flight while this group's products run
+    const uint32_t* hb = hl + TERM_LIMB_WORDS * ((uint64_t)te * H);
+    const uint32_t full = tb / TERM_GROUP * TERM_GROUP;
+    uint64_t cur[TERM_GROUP];
+    if (full) {
+#pragma unroll
+        for (uint32_t u = 0; u < TERM_GROUP; ++u) cur[u] = lo_b[(uint64_t)u * LO + l];
+    }
+    for (t = 0; t < full; t += TERM_GROUP) {
+        const uint32_t tn = t + TERM_GROUP < full ? t + TERM_GROUP : t;
+        uint64_t nxt[TERM_GROUP];
+#pragma unroll
+        for (uint32_t u = 0; u < TERM_GROUP; ++u) nxt[u] = lo_b[(uint64_t)(tn + u) * LO + l];
+        reserve(TERM_GROUP);
+#pragma unroll
+        for (uint32_t u = 0; u < TERM_GROUP; ++u) limb_mac_base(A, hb + TERM_LIMB_WORDS * ((uint64_t)(t + u) * H + h), cur[u]);
+#pragma unroll
+        for (uint32_t u = 0; u < TERM_GROUP; ++u) cur[u] = nxt[u];
+    }
+    for (t = full; t < tb; ++t) {
+        reserve(1);
+        limb_mac_base(A, hb + TERM_LIMB_WORDS * ((uint64_t)t * H + h), lo_b[(uint64_t)t * LO + l]);
+    }
+    if (used) sum = gl::e3_add(sum, limb_flush(A));
+    const uint64_t i = h * LO + l;
     if (accumulate) sum = gl::e3_add(sum, ld3(acc, i));
     st3(acc, i, sum);
 }
@@ -329,18 +508,28 @@ void fill_u64_launch(uint64_t* p, uint64_t n, uint64_t v, hipStream_t st) {
 
 void term_tables_launch(const uint64_t* d_desc_e, uint32_t ne, const uint64_t* d_desc_b, uint32_t nb,
                         const uint64_t* d_coords, uint32_t nv, uint32_t L, uint64_t* hi_e, uint64_t* lo_e,
-                        uint64_t* hi_b, uint64_t* lo_b, hipStream_t st) {
+                        uint64_t* hi_b, uint64_t* lo_b, uint32_t* hl, hipStream_t st) {
     if (!ne && !nb) return;
     uint64_t span = std::max<uint64_t>(1ull << (nv - L), 1ull << L);
     hipLaunchKernelGGL(term_tables_kernel, dim3(blocks_of(span), ne + nb), dim3(256), 0, st, d_desc_e, d_desc_b,
-                       d_coords, nv, L, ne, hi_e, lo_e, hi_b, lo_b);
+                       d_coords, nv, L, ne, hi_e, lo_e, hi_b, lo_b, hl);
+}
+
+uint64_t term_limb_bytes(uint32_t nv, uint32_t L, uint32_t terms) {
+    return L >= 8 ? (uint64_t)terms * (1ull << (nv - L)) * TERM_LIMB_WORDS * 4 : 0;
 }
 
 void term_accumulate_launch(uint64_t* d_acc, uint64_t n, uint32_t L, const uint64_t* hi_e, const uint64_t* lo_e,
-                            uint32_t te, const uint64_t* hi_b, const uint64_t* lo_b, uint32_t tb, bool accumulate,
-                            hipStream_t st) {
-    hipLaunchKernelGGL(term_accumulate_kernel, dim3(blocks_of(n)), dim3(256), 0, st, d_acc, n, L, hi_e, lo_e, te, hi_b,
-                       lo_b, tb, accumulate ? 1 : 0);
+                            uint32_t te, const uint64_t* hi_b, const uint64_t* lo_b, uint32_t tb, const uint32_t* hl,
+                            bool accumulate, hipStream_t st) {
+    const uint64_t LO = 1ull << L, H = n >> L;
+    if (LO < 256) {
+        hipLaunchKernelGGL(term_accumulate_kernel, dim3(blocks_of(n)), dim3(256), 0, st, d_acc, n, L, hi_e, lo_e, te,
+                           hi_b, lo_b, tb, accumulate ? 1 : 0);
+        return;
+    }
+    hipLaunchKernelGGL(term_sweep_kernel, dim3((unsigned)((LO / 256) * H)), dim3(256), 0, st, d_acc, H, L, hl, lo_e,
+                       te, lo_b, tb, accumulate ? 1 : 0);
 }
 
 unsigned mle_eval_launch(const uint64_t* f, bool f_base, uint64_t n, uint32_t L, const uint64_t* hi,

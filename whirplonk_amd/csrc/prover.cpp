@@ -257,9 +257,9 @@ int terms_apply(whir_ctx* ctx, const Terms& T, uint64_t* d_acc, uint32_t nv, boo
     int rc;
     size_t desc_bytes = (T.desc_e.size() + T.desc_b.size() + T.coords.size() + 4) * 8;
     size_t he = (size_t)T.ne * H * 24, le = (size_t)T.ne * LO * 24, hb = (size_t)T.nb * H * 24,
-           lb = (size_t)T.nb * LO * 8;
+           lb = (size_t)T.nb * LO * 8, hlb = term_limb_bytes(nv, L, T.ne + T.nb);
     if ((rc = ctx->ensure(ctx->tdesc, desc_bytes))) return rc;
-    if ((rc = ctx->ensure(ctx->ttab, he + le + hb + lb + 256))) return rc;
+    if ((rc = ctx->ensure(ctx->ttab, he + le + hb + lb + hlb + 256))) return rc;
     uint64_t* dd = (uint64_t*)ctx->tdesc.p;
     uint64_t* de = dd;
     uint64_t* db = de + T.desc_e.size();
@@ -275,8 +275,9 @@ int terms_apply(whir_ctx* ctx, const Terms& T, uint64_t* d_acc, uint32_t nv, boo
     uint64_t* lo_e = hi_e + he / 8;
     uint64_t* hi_b = lo_e + le / 8;
     uint64_t* lo_b = hi_b + hb / 8;
-    term_tables_launch(de, T.ne, db, T.nb, dc, nv, L, hi_e, lo_e, hi_b, lo_b, ctx->stream);
-    term_accumulate_launch(d_acc, 1ull << nv, L, hi_e, lo_e, T.ne, hi_b, lo_b, T.nb, accumulate, ctx->stream);
+    uint32_t* hl = hlb ? (uint32_t*)(lo_b + lb / 8) : nullptr;
+    term_tables_launch(de, T.ne, db, T.nb, dc, nv, L, hi_e, lo_e, hi_b, lo_b, hl, ctx->stream);
+    term_accumulate_launch(d_acc, 1ull << nv, L, hi_e, lo_e, T.ne, hi_b, lo_b, T.nb, hl, accumulate, ctx->stream);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -968,7 +969,7 @@ int whir_dev_mle_evaluate(whir_ctx* ctx, const uint64_t* d_f, int f_is_base, siz
     if ((rc = ctx->stage_desc(all, dd))) return rc;
     uint64_t* hi = (uint64_t*)ctx->ttab.p;
     uint64_t* lo = hi + 3 * H;
-    term_tables_launch(dd, 1, nullptr, 0, dd + T.desc_e.size(), nv, L, hi, lo, nullptr, nullptr, ctx->stream);
+    term_tables_launch(dd, 1, nullptr, 0, dd + T.desc_e.size(), nv, L, hi, lo, nullptr, nullptr, nullptr, ctx->stream);
     unsigned blocks = mle_eval_launch(d_f, f_is_base != 0, 1ull << nv, L, hi, lo, ctx->hpart_d, ctx->stream, false);
     HIPCHK(hipGetLastError());
     e3 v;

@@ -119,37 +119,107 @@ __global__ __launch_bounds__(256) void sumcheck_fold_kernel(const uint64_t* __re
     }
 }
 
-// Thread t of T = 2^logT threads evaluates the interleaved sub-polynomial
-// sum_q f[t + T q] Z^q (Z = z^T) by Horner, scales by z^t, and each block's
-// total goes to partials[block].  The powers z^(2^l), l <= logT <= HORNER_MAX_LOGT,
-// come by value in the kernel arguments (read with scalar loads).
-__global__ __launch_bounds__(256) void horner_kernel(const uint64_t* __restrict__ f, int f_ext3, uint64_t n,
-                                                     HornerPowers zp, uint32_t logT,
-                                                     uint64_t* __restrict__ partials) {
+// sum_i f_i z^i with i = t + T q, lane t < T = 2^logT (= gridDim.x * 256):
+//   lane t: acc_t = sum_q f[t + T q] P[q],  P[q] = z^(T q),
+// from a per-workgroup LDS table of the powers (256 of them per chunk of q;
+// the chunk's sum is scaled by z^(256 T c)), so each coefficient costs one
+// multiply-add without reduction and a lane's loads do not wait on one
+// another.  acc_t is then scaled by z^t = z^tau z^(256 b) (t = 256 b + tau):
+// z^tau = A[tau % 16] B[tau / 16] from two 16-entry tables, and z^(256 b)
+// once per workgroup on its total.  The tables and z^(256 b) are products of
+// the powers z^(2^l) (l < HORNER_POWS, by value) over the bits of the
+// exponent: each table entry by its own lane, z^(256 b) as a product tree
+// over one wave's lanes (one factor per bit of b).
+template <bool EXT>
+__global__ __launch_bounds__(256) void horner_kernel(const uint64_t* __restrict__ f, uint64_t n, HornerPowers zp,
+                                                     uint32_t logT, uint64_t* __restrict__ partials) {
+    __shared__ e3 P[256];
+    __shared__ e3 A[16], B[16];
+    __shared__ e3 zb;
+    const uint32_t tid = threadIdx.x;
     const uint64_t T = 1ull << logT;
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    e3 acc[1] = {{0, 0, 0}};
-    if (t < n) {
-        e3 Z = {0, 0, 0};
+    const uint64_t Q = (n + T - 1) >> logT;
+    const e3 one = {1, 0, 0};
+    auto zpow2 = [&](uint32_t l) -> e3 { return {zp.w[3 * l], zp.w[3 * l + 1], zp.w[3 * l + 2]}; };
+    if (tid < 64) {
+        // wave 0: z^(256 b) = product over the set bits j of b of z^(2^(8 + j))
+        e3 v = one;
 #pragma unroll
-        for (uint32_t l = 0; l <= HORNER_MAX_LOGT; ++l)
-            if (l == logT) Z = {zp.w[3 * l], zp.w[3 * l + 1], zp.w[3 * l + 2]};
-        uint64_t nq = (n - t + T - 1) / T;
-        for (uint64_t q = nq; q-- > 0;) {
-            uint64_t i = t + T * q;
-            acc[0] = gl::e3_mul(Z, acc[0]);  // the invariant factor first: its doublings hoist
-            if (f_ext3) {
-                acc[0] = gl::e3_add(acc[0], load_e3(f, i));
-            } else {
-                acc[0].c0 = gl::add(acc[0].c0, f[i]);
+        for (uint32_t j = 0; j < HORNER_MAX_LOGT - 8; ++j)
+            if (tid == j && j + 8 < logT && ((blockIdx.x >> j) & 1)) v = zpow2(8 + j);
+#pragma unroll
+        for (int m = 1; m < 16; m <<= 1) {
+            const e3 o = {red::shfl_xor64(v.c0, m), red::shfl_xor64(v.c1, m), red::shfl_xor64(v.c2, m)};
+            v = gl::e3_mul(v, o);
+        }
+        if (tid == 0) zb = v;
+    } else if (tid < 96) {
+        // A[k] = z^k, B[k] = z^(16 k), k < 16
+        const uint32_t k = tid & 15, l0 = tid < 80 ? 0 : 4;
+        e3 a = one;
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j)
+            if ((k >> j) & 1) a = gl::e3_mul(a, zpow2(l0 + j));
+        if (tid < 80)
+            A[k] = a;
+        else
+            B[k] = a;
+    }
+    // P[q] = z^(T q), q < min(Q, 256): lane q, product over the bits of q
+    const uint32_t qmax = (uint32_t)(Q < 256 ? Q : 256);
+    if (tid < qmax) {
+        e3 p = one;
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j)
+            if ((tid >> j) & 1) p = gl::e3_mul(p, zpow2(logT + j));
+        P[tid] = p;
+    }
+    __syncthreads();
+    const uint64_t t = (uint64_t)blockIdx.x * 256 + tid;
+    e3 acc = {0, 0, 0};
+    e3 zc = {1, 0, 0};  // z^(256 T c) for chunk c
+    constexpr uint32_t U = EXT ? 8 : 16;  // loads in flight per lane
+    for (uint64_t c0 = 0; c0 < Q; c0 += 256) {
+        const uint32_t qn = (uint32_t)(Q - c0 < 256 ? Q - c0 : 256);
+        gl::e3acc s = gl::e3acc_zero();
+        // full groups of U coefficients (loads first), then the remainder
+        uint32_t q = 0;
+        for (; q + U <= qn; q += U) {
+            e3 v[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const uint64_t i = t + T * (c0 + q + u);
+                if constexpr (EXT)
+                    v[u] = i < n ? load_e3(f, i) : e3{0, 0, 0};
+                else
+                    v[u] = {i < n ? f[i] : 0, 0, 0};
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                if constexpr (EXT)
+                    gl::e3_mac(s, P[q + u], v[u]);
+                else
+                    gl::e3_mac_base(s, P[q + u], v[u].c0);
             }
         }
-#pragma unroll
-        for (uint32_t l = 0; l < HORNER_MAX_LOGT; ++l)
-            if (l < logT && ((t >> l) & 1)) acc[0] = gl::e3_mul(acc[0], e3{zp.w[3 * l], zp.w[3 * l + 1], zp.w[3 * l + 2]});
+        for (; q < qn; ++q) {
+            const uint64_t i = t + T * (c0 + q);
+            if (i < n) {
+                if constexpr (EXT)
+                    gl::e3_mac(s, P[q], load_e3(f, i));
+                else
+                    gl::e3_mac_base(s, P[q], f[i]);
+            }
+        }
+        e3 part = gl::e3acc_reduce(s);
+        if (c0) part = gl::e3_mul(part, zc);
+        acc = gl::e3_add(acc, part);
+        if (c0 + 256 < Q) zc = gl::e3_mul(zc, zpow2(logT + 8));
     }
-    block_sum<1>(acc);
-    if (threadIdx.x == 0) store_e3(partials, blockIdx.x, acc[0]);
+    acc = gl::e3_mul(acc, gl::e3_mul(A[tid & 15], B[tid >> 4]));
+    e3 a1[1] = {acc};
+    block_sum<1>(a1);
+    if (tid == 0) store_e3(partials, blockIdx.x, gl::e3_mul(a1[0], zb));
 }
 
 unsigned grid_for(uint64_t work, unsigned cap) {
@@ -198,10 +268,14 @@ void sumcheck_fold2_launch(const uint64_t* a, bool a_base, uint64_t* a_out, cons
 unsigned horner_launch(const uint64_t* f, bool f_ext3, uint64_t n, const HornerPowers& zp, uint64_t* partials,
                        hipStream_t st) {
     // T = blocks*256 threads, a power of two, about 16+ terms per thread
+    // (logT + 8 < HORNER_POWS: the kernel reads z^(2^l) up to l = logT + 8)
     uint32_t logT = 8;
     while (logT < HORNER_MAX_LOGT && (1ull << (logT + 4)) < n) ++logT;
     unsigned blocks = (unsigned)((1ull << logT) / 256);
-    hipLaunchKernelGGL(horner_kernel, dim3(blocks), dim3(256), 0, st, f, f_ext3 ? 1 : 0, n, zp, logT, partials);
+    if (f_ext3)
+        hipLaunchKernelGGL(horner_kernel<true>, dim3(blocks), dim3(256), 0, st, f, n, zp, logT, partials);
+    else
+        hipLaunchKernelGGL(horner_kernel<false>, dim3(blocks), dim3(256), 0, st, f, n, zp, logT, partials);
     return blocks;
 }
 

@@ -689,7 +689,7 @@ int whir_dev_horner(whir_ctx* ctx, const uint64_t* d_f, int f_is_ext3, size_t n,
     if (rc) return rc;
     HornerPowers zp;
     gl::e3 p = {z[0], z[1], z[2]};
-    for (uint32_t l = 0; l <= HORNER_MAX_LOGT; ++l) {
+    for (uint32_t l = 0; l < HORNER_POWS; ++l) {
         zp.w[3 * l] = p.c0;
         zp.w[3 * l + 1] = p.c1;
         zp.w[3 * l + 2] = p.c2;
