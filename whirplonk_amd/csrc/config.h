@@ -30,20 +30,20 @@ struct IrsConfig {
         return 1.0 / (2.0 * johnson_slack * std::sqrt(rate()));
     }
     double rbr_queries() const {
-        double per = unique_decoding() ? ((1.0 + rate()) * 0.5) : (std::sqrt(rate()) + johnson_slack);
-        return (double)in_domain * (-std::log2(per));
+        double per_query_agreement = unique_decoding() ? ((1.0 + rate()) * 0.5) : (std::sqrt(rate()) + johnson_slack);
+        return (double)in_domain * (-std::log2(per_query_agreement));
     }
     double rbr_fold_prox_gaps(double fbits) const {
-        double lir = -std::log2(rate());
-        double lk = std::log2((double)message_length());
-        double err;
+        double log_inv_rate_bits = -std::log2(rate());
+        double log_message_length = std::log2((double)message_length());
+        double error_bits;
         if (unique_decoding()) {
-            err = lk + lir;
+            error_bits = log_message_length + log_inv_rate_bits;
         } else {
             const double LOG2_10 = 3.321928094887362;
-            err = 7.0 * LOG2_10 + 3.5 * lir + 2.0 * lk;
+            error_bits = 7.0 * LOG2_10 + 3.5 * log_inv_rate_bits + 2.0 * log_message_length;
         }
-        return fbits - err;
+        return fbits - error_bits;
     }
 
     static IrsConfig derive(double security, bool unique_dec, size_t num_vec, size_t vec_size, size_t depth,
@@ -52,21 +52,21 @@ struct IrsConfig {
         c.num_vectors = num_vec;
         c.vector_size = vec_size;
         c.depth = depth;
-        size_t msg = vec_size / depth;
-        c.codeword_length = (size_t)std::ceil((double)msg / rate_val);
-        double ar = (double)msg / (double)c.codeword_length;
-        c.johnson_slack = unique_dec ? 0.0 : std::sqrt(ar) / 20.0;
+        size_t message_len = vec_size / depth;
+        c.codeword_length = (size_t)std::ceil((double)message_len / rate_val);
+        double actual_rate = (double)message_len / (double)c.codeword_length;
+        c.johnson_slack = unique_dec ? 0.0 : std::sqrt(actual_rate) / 20.0;
         if (unique_dec) {
             c.out_domain = 0;
         } else {
-            double L = 1.0 / (2.0 * c.johnson_slack * std::sqrt(ar));
-            double lc2 = L * (L - 1.0) / 2.0;
-            double lps = fbits - std::log2((double)(vec_size - 1));
-            c.out_domain = (size_t)std::ceil((security + std::log2(lc2)) / lps);
+            double L = 1.0 / (2.0 * c.johnson_slack * std::sqrt(actual_rate));
+            double list_pairs = L * (L - 1.0) / 2.0;
+            double bits_per_sample = fbits - std::log2((double)(vec_size - 1));
+            c.out_domain = (size_t)std::ceil((security + std::log2(list_pairs)) / bits_per_sample);
             if (c.out_domain < 1) c.out_domain = 1;
         }
-        double per = unique_dec ? ((1.0 + ar) * 0.5) : (std::sqrt(ar) + c.johnson_slack);
-        c.in_domain = (size_t)std::ceil(security / (-std::log2(per)));
+        double per_query_agreement = unique_dec ? ((1.0 + actual_rate) * 0.5) : (std::sqrt(actual_rate) + c.johnson_slack);
+        c.in_domain = (size_t)std::ceil(security / (-std::log2(per_query_agreement)));
         return c;
     }
 };
@@ -118,67 +118,67 @@ struct WhirConfig {
                                       size_t{1} << p.initial_folding_factor, initial_rate, fbits);
         double start_fold_bits;
         {
-            double pg = c.initial.rbr_fold_prox_gaps(fbits);
-            double ll = std::log2(c.initial.list_size());
-            start_fold_bits = std::max(security - std::min(pg, fbits - ll - 1.0), 0.0);
+            double prox_gap_bits = c.initial.rbr_fold_prox_gaps(fbits);
+            double list_bits = std::log2(c.initial.list_size());
+            start_fold_bits = std::max(security - std::min(prox_gap_bits, fbits - list_bits - 1.0), 0.0);
         }
         double skip_bits;
         {
-            double pg = c.initial.rbr_fold_prox_gaps(fbits) + std::log2((double)p.initial_folding_factor);
-            skip_bits = std::max(security - pg, 0.0);
+            double prox_gap_bits = c.initial.rbr_fold_prox_gaps(fbits) + std::log2((double)p.initial_folding_factor);
+            skip_bits = std::max(security - prox_gap_bits, 0.0);
         }
         c.initial_sumcheck.initial_size = size;
         c.initial_sumcheck.num_rounds = p.initial_folding_factor;
         c.initial_sumcheck.round_pow = pow_threshold(start_fold_bits);
         c.initial_skip_pow = pow_threshold(skip_bits);
 
-        size_t rd = 0;
-        size_t ids = c.initial.in_domain;
-        double qe = c.initial.rbr_queries();
+        size_t round_index = 0;
+        size_t prev_in_domain = c.initial.in_domain;
+        double prev_query_error = c.initial.rbr_queries();
         num_vars -= p.initial_folding_factor;
         while (num_vars >= p.folding_factor) {
-            size_t rff = rd == 0 ? p.initial_folding_factor : p.folding_factor;
-            size_t nli = log_inv_rate + (rff - 1);
-            double rr = std::pow(0.5, (double)nli);
-            RoundConfig rc;
-            rc.irs = IrsConfig::derive(protocol_security, p.unique_decoding, 1, size_t{1} << num_vars,
-                                       size_t{1} << p.folding_factor, rr, fbits);
-            double cbe;
+            size_t round_fold = round_index == 0 ? p.initial_folding_factor : p.folding_factor;
+            size_t next_log_inv_rate = log_inv_rate + (round_fold - 1);
+            double round_rate = std::pow(0.5, (double)next_log_inv_rate);
+            RoundConfig round;
+            round.irs = IrsConfig::derive(protocol_security, p.unique_decoding, 1, size_t{1} << num_vars,
+                                          size_t{1} << p.folding_factor, round_rate, fbits);
+            double combination_bits;
             {
-                double ll = std::log2(rc.irs.list_size());
-                size_t cnt = rc.irs.out_domain + ids;
-                cbe = fbits - (std::log2((double)cnt) + ll + 1.0);
+                double list_bits = std::log2(round.irs.list_size());
+                size_t constraint_count = round.irs.out_domain + prev_in_domain;
+                combination_bits = fbits - (std::log2((double)constraint_count) + list_bits + 1.0);
             }
-            double pb = std::max(security - std::min(qe, cbe), 0.0);
-            double fpb;
+            double round_pow_bits = std::max(security - std::min(prev_query_error, combination_bits), 0.0);
+            double fold_pow_bits;
             {
-                double pg = rc.irs.rbr_fold_prox_gaps(fbits);
-                double ll = std::log2(rc.irs.list_size());
-                fpb = std::max(security - std::min(pg, fbits - ll - 1.0), 0.0);
+                double prox_gap_bits = round.irs.rbr_fold_prox_gaps(fbits);
+                double list_bits = std::log2(round.irs.list_size());
+                fold_pow_bits = std::max(security - std::min(prox_gap_bits, fbits - list_bits - 1.0), 0.0);
             }
-            rc.sumcheck.initial_size = size_t{1} << num_vars;
-            rc.sumcheck.num_rounds = p.folding_factor;
-            rc.sumcheck.round_pow = pow_threshold(fpb);
-            rc.pow = pow_threshold(pb);
-            ++rd;
+            round.sumcheck.initial_size = size_t{1} << num_vars;
+            round.sumcheck.num_rounds = p.folding_factor;
+            round.sumcheck.round_pow = pow_threshold(fold_pow_bits);
+            round.pow = pow_threshold(round_pow_bits);
+            ++round_index;
             num_vars -= p.folding_factor;
-            log_inv_rate = nli;
-            ids = rc.irs.in_domain;
-            qe = rc.irs.rbr_queries();
-            c.rounds.push_back(rc);
+            log_inv_rate = next_log_inv_rate;
+            prev_in_domain = round.irs.in_domain;
+            prev_query_error = round.irs.rbr_queries();
+            c.rounds.push_back(round);
         }
-        double rbr_e = c.rounds.empty() ? c.initial.rbr_queries() : c.rounds.back().irs.rbr_queries();
-        double fpb2 = std::max(security - rbr_e, 0.0);
-        double ffpb = std::max(security - fbits + 1.0, 0.0);
+        double last_query_error = c.rounds.empty() ? c.initial.rbr_queries() : c.rounds.back().irs.rbr_queries();
+        double final_pow_bits = std::max(security - last_query_error, 0.0);
+        double final_fold_pow_bits = std::max(security - fbits + 1.0, 0.0);
         c.final_sumcheck.initial_size = size_t{1} << num_vars;
         c.final_sumcheck.num_rounds = num_vars;
-        c.final_sumcheck.round_pow = pow_threshold(ffpb);
-        c.final_pow = pow_threshold(fpb2);
+        c.final_sumcheck.round_pow = pow_threshold(final_fold_pow_bits);
+        c.final_pow = pow_threshold(final_pow_bits);
         return c;
     }
 
     // Flat form: [n_rounds | initial irs (7) | initial sumcheck size, rounds, pow |
-    // skip pow | per round: irs (7), sumcheck size, rounds, pow, round pow |
+    // skip pow | per_query_agreement round: irs (7), sumcheck size, rounds, pow, round pow |
     // final sumcheck size, rounds, pow | final pow]; irs = num_vectors,
     // vector_size, codeword_length, depth, in_domain, out_domain, slack bits.
     std::vector<uint64_t> flat() const {
