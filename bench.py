@@ -30,6 +30,8 @@ Beside the headline line's workload it measures, in the same JSON line:
     the reference's own prover on the host (bit-exact proof bytes);
   * full_prove_sha2: the same prove with SHA-256 trees, leaves and PoW (the
     CLI's --hash Sha2);
+  * full_prove_nv24: the prove at num_variables 24, rate 1/4 (the served
+    mix's largest request) on its own, at most 10 proofs;
   * batched: 64 polynomials of 2^14..2^22 coefficients (rate 1/4) committed by
     one graph launch, against one-at-a-time commits and the CPU roots;
   * served: commit+prove requests (sizes 2^16..2^24 with weights
@@ -39,7 +41,8 @@ Beside the headline line's workload it measures, in the same JSON line:
     with p50/p99 latency, and the plain one-at-a-time GPU path on the same
     requests.
 
-    python bench.py [--gpus 1] [--steps 30] [--warmup 3] [--no-cpu] [--legs commit,prove,prove_sha2,batched,served]
+    python bench.py [--gpus 1] [--steps 30] [--warmup 3] [--no-cpu]
+                    [--legs commit,prove,prove_sha2,prove_nv24,batched,served]
 """
 from __future__ import annotations
 
@@ -253,23 +256,24 @@ def prove_inputs(nv, seed, nforms=3):
     return coeffs, pts.reshape(nforms, nv, 3)
 
 
-def prove_leg(steps: int, warmup: int, cpu: bool, hash_id: int = 1):
-    """Full WHIR prove at 2^22 (rate 1/2, folding factor 4, 3 MLE constraints),
-    the CLI's parameters and domain separator; inputs resident in HBM.
-    hash_id 2 is the CLI's --hash Sha2 (SHA-256 trees, leaves and PoW)."""
+def prove_leg(steps: int, warmup: int, cpu: bool, hash_id: int = 1, nv: int = PROVE_NV, rate: int = 1):
+    """Full WHIR prove at 2^nv (default 2^22, rate 1/2; folding factor 4, 3 MLE
+    constraints), the CLI's parameters and domain separator; inputs resident
+    in HBM.  hash_id 2 is the CLI's --hash Sha2 (SHA-256 trees, leaves and
+    PoW); rate is the starting log inverse rate."""
     import whirplonk_amd as W
 
-    coeffs, pts = prove_inputs(PROVE_NV, PROVE_SEED)
-    params = W.Params.default(starting_log_inv_rate=1, hash_id=hash_id)
+    coeffs, pts = prove_inputs(nv, PROVE_SEED)
+    params = W.Params.default(starting_log_inv_rate=rate, hash_id=hash_id)
     pid, sid = W.domain_separator_cli(params)
     ctx = W.Context(0)
     d = ctx.upload(coeffs)
     ev = np.array([ctx.mle_evaluate(d, True, pts[f]) for f in range(len(pts))], dtype=np.uint64)
-    ws_bytes = W.workspace_bytes(1 << PROVE_NV, params)
+    ws_bytes = W.workspace_bytes(1 << nv, params)
     ws = ctx.alloc(ws_bytes)
 
     def one():
-        with W.Prover(ctx, params, 1 << PROVE_NV, pid, sid, workspace=(ws, ws_bytes)) as pr:
+        with W.Prover(ctx, params, 1 << nv, pid, sid, workspace=(ws, ws_bytes)) as pr:
             pr.commit(d, 1)
             pr.prove(pts, ev)
             return pr.proof(), pr.phase_times(), pr.pow_hashes()
@@ -283,7 +287,7 @@ def prove_leg(steps: int, warmup: int, cpu: bool, hash_id: int = 1):
     ctx.sync()
     ms = 1000.0 * (time.perf_counter() - t0) / steps
     ctx.close()
-    out = {"workload": "full WHIR prove nv=22 rate 1/2 folding_factor 4, 3 MLE constraints, CLI parameters"
+    out = {"workload": f"full WHIR prove nv={nv} rate 1/{1 << rate} folding_factor 4, 3 MLE constraints, CLI parameters"
                        + (", --hash Sha2" if hash_id == 2 else ""),
            "ms_per_proof": round(ms, 3), "proofs_per_s": round(1000.0 / ms, 2),
            "phase_ms": {k: round(v, 3) for k, v in phases.items()}, "pow_hashes": int(pow_h),
@@ -295,7 +299,7 @@ def prove_leg(steps: int, warmup: int, cpu: bool, hash_id: int = 1):
 
         if oracle.ref() is not None:
             t0 = time.perf_counter()
-            _, n2, h2 = oracle.ref_prove(PROVE_NV, coeffs, pts, hash_id=hash_id, rate=1)
+            _, n2, h2 = oracle.ref_prove(nv, coeffs, pts, hash_id=hash_id, rate=rate)
             out["cpu_reference_ms"] = round(1000.0 * (time.perf_counter() - t0), 1)
             restore_affinity()
             out["cpu_threads"] = int(oracle.ref().ref_num_threads())
@@ -598,12 +602,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-host", action="store_true", help="skip the host-resident commit timing")
-    ap.add_argument("--legs", default="commit,prove,prove_sha2,batched,served")
+    ap.add_argument("--legs", default="commit,prove,prove_sha2,prove_nv24,batched,served")
     ap.add_argument("--requests", type=int, default=2000, help="served requests")
     ap.add_argument("--offered", type=float, default=0.0,
                     help="served offered rate (requests/s, Poisson); 0: 0.8x the measured plain-path rate")
-    ap.add_argument("--slots", type=int, default=16)
-    ap.add_argument("--budget-gb", type=float, default=64.0)
+    # 24 slots within 96 GiB of the 288 GB of HBM (capacity measured 2-4 %
+    # above 16 slots in 64 GiB, with a longer tail under heavy load)
+    ap.add_argument("--slots", type=int, default=24)
+    ap.add_argument("--budget-gb", type=float, default=96.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 512 MiB device copy kernel (reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -629,6 +635,11 @@ def main():
     out["full_prove_sha2"] = (prove_leg(args.steps, args.warmup, not args.no_cpu, hash_id=2)
                               if "prove_sha2" in legs else None)
     done("prove_sha2")
+    # the served mix's largest request on its own (no CPU reference: it is
+    # checked against the reference prover in the served leg's samples)
+    out["full_prove_nv24"] = (prove_leg(min(args.steps, 10), args.warmup, False, nv=24, rate=2)
+                              if "prove_nv24" in legs else None)
+    done("prove_nv24")
     out["batched"] = batched_leg(args.steps, not args.no_cpu) if "batched" in legs else None
     done("batched")
     out["served"] = (served_leg(args.requests, args.offered, args.slots, args.budget_gb, not args.no_cpu)

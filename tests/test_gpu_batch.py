@@ -57,6 +57,13 @@ def test_batch_roots_and_replays(ctx):
         # the codeword of one polynomial against the oracle
         k = sizes.index(1 << 12)
         assert np.array_equal(b.matrix(k), oracle.rs_encode(polys[k], 1, sizes[k], (sizes[k] // 16) << 2, 16))
+        # whole node arrays (the trees are built level by level across the
+        # batch): one tree per step shape of the build, against the oracle
+        for nv in sorted({s.bit_length() - 1 for s in sizes}):
+            k = sizes.index(1 << nv)
+            rows = sizes[k] // 4
+            want = oracle.merkle_build(oracle.commit_leaves_base(b.matrix(k), rows, 16), rows)
+            assert np.array_equal(b.nodes(k), want), nv
         # graph replays, with the program's own eager work in between
         scratch = ctx.upload(polys[0])
         d_m = ctx.alloc(8 * (sizes[0] // 4) * 16)

@@ -693,6 +693,11 @@ class Batch:
         rows = load().whir_batch_rows(self.h, k)
         return self.ctx.download(load().whir_batch_matrix(self.h, k), rows * 16)
 
+    def nodes(self, k: int) -> np.ndarray:
+        """Polynomial k's flat Merkle node array (32 bytes per node)."""
+        rows = load().whir_batch_rows(self.h, k)
+        return self.ctx.download(load().whir_batch_nodes(self.h, k), 4 * merkle_num_nodes(rows)).view(np.uint8)
+
     def run(self, graph: bool = True):
         _check((load().whir_batch_run if graph else load().whir_batch_run_eager)(self.h))
 

@@ -103,6 +103,11 @@ struct whir_batch {
     size_t roots_off = 0, idx_off = 0, total = 0;
     std::vector<size_t> scratch_off, scratch_bytes;
     std::vector<std::vector<size_t>> branch_polys;
+    // BLAKE3 batches: the trees are built after the encodes, level by level
+    // across all polynomials (a few launches for the whole batch)
+    bool multi_tree = false;
+    size_t segs_off = 0;
+    std::vector<MerkleMultiLaunch> tree_launches;
     uint8_t* region = nullptr;
     std::vector<hipStream_t> branches;
     std::vector<hipEvent_t> events;
@@ -134,7 +139,8 @@ struct whir_batch {
             ctx->ext_scratch_bytes = scratch_bytes[b];
             for (size_t k : branch_polys[b]) {
                 rc = commit_device(ctx, (const uint64_t*)(region + coeff_off[k]), 1, sizes[k], rows[k], depth,
-                                   (uint64_t*)(region + matrix_off[k]), region + nodes_off[k], nullptr, 0);
+                                   (uint64_t*)(region + matrix_off[k]), region + nodes_off[k], nullptr, 0,
+                                   !multi_tree);
                 if (rc) break;
             }
         }
@@ -158,6 +164,7 @@ struct whir_batch {
             HIPCHK(hipEventRecord(events[1 + b], branches[b]));
             HIPCHK(hipStreamWaitEvent(main, events[1 + b], 0));
         }
+        if (multi_tree) merkle_multi_launch(region, (const MerkleSeg*)(region + segs_off), tree_launches, main);
         gather_hashes_launch(region, (const uint64_t*)(region + idx_off), sizes.size(), region + roots_off, main);
         HIPCHK(hipGetLastError());
         return 0;
@@ -262,6 +269,12 @@ int whir_batch_create(whir_ctx* ctx, size_t count, const uint64_t* sizes, uint32
     }
     B->roots_off = put(count * 32);
     B->idx_off = put(count * 8);
+    std::vector<MerkleSeg> segs;
+    if (B->hash == HASH_BLAKE3) {
+        std::vector<uint64_t> noff(B->nodes_off.begin(), B->nodes_off.end());
+        B->multi_tree = merkle_multi_plan(noff.data(), B->rows.data(), count, segs, B->tree_launches) == 0;
+    }
+    if (B->multi_tree) B->segs_off = put(segs.size() * sizeof(MerkleSeg));
     for (size_t b = 0; b < nb; ++b) {
         size_t need = 0;
         for (size_t k : B->branch_polys[b]) {
@@ -277,6 +290,9 @@ int whir_batch_create(whir_ctx* ctx, size_t count, const uint64_t* sizes, uint32
     for (size_t k = 0; k < count; ++k)
         idx[k] = (B->nodes_off[k] + 32 * (whir_merkle_num_nodes(B->rows[k]) - 1)) / 32;
     HIPCHK(hipMemcpyAsync(B->region + B->idx_off, idx.data(), count * 8, hipMemcpyHostToDevice, ctx->stream));
+    if (B->multi_tree && !segs.empty())
+        HIPCHK(hipMemcpyAsync(B->region + B->segs_off, segs.data(), segs.size() * sizeof(MerkleSeg),
+                              hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     for (size_t b = 0; b < nb; ++b) {
         hipStream_t st;

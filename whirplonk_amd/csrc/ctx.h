@@ -326,8 +326,9 @@ int lde_device(whir_ctx* ctx, const LdeShape& s, const uint64_t* in, uint64_t* o
                bool* fused = nullptr);
 
 // RS encode + leaves + Merkle tree of num_polys vectors already on the device;
-// the root is copied to out_root (host) when non-null.
+// the root is copied to out_root (host) when non-null.  With tree false only
+// the encode and the leaves (in d_nodes' first cw nodes) run.
 int commit_device(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_polys, size_t poly_size, size_t cw,
-                  size_t depth, uint64_t* d_matrix, uint8_t* d_nodes, uint8_t* out_root, int ext);
+                  size_t depth, uint64_t* d_matrix, uint8_t* d_nodes, uint8_t* out_root, int ext, bool tree = true);
 
 }  // namespace whir_gpu

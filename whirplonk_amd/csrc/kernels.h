@@ -1,5 +1,6 @@
 // kernels.h — host-side launchers of the HIP kernels (internal to the library).
 #pragma once
+#include <vector>
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdint.h>
@@ -81,6 +82,23 @@ void sha256_leaves_launch(const uint64_t* matrix, uint64_t rows, uint32_t row_u6
                           hipStream_t st);
 void sha256_hash_many_launch(const uint8_t* in, uint64_t size, uint64_t count, uint8_t* out, hipStream_t st);
 void sha256_merkle_build_launch(uint8_t* nodes, uint64_t n_leaves, hipStream_t st, uint8_t* root_copy = nullptr);
+// Many BLAKE3 trees built level by level together (a batch's trees): the
+// plan lists, per launch, one segment per tree that takes part (its node
+// array as a byte offset from a common base, the child level's node offset
+// and width, the launch's first workgroup for it).  Leaf counts must be
+// powers of two.  Upload `segs` to the device and pass its address to
+// merkle_multi_launch, which issues the plan's launches in order.
+struct MerkleSeg {
+    uint64_t nodes, child_off, width;
+    uint32_t block0, pad;
+};
+struct MerkleMultiLaunch {
+    uint32_t kind, seg0, nseg, blocks;
+};
+int merkle_multi_plan(const uint64_t* nodes_off, const uint64_t* leaves, size_t count, std::vector<MerkleSeg>& segs,
+                      std::vector<MerkleMultiLaunch>& launches);
+void merkle_multi_launch(uint8_t* base, const MerkleSeg* d_segs, const std::vector<MerkleMultiLaunch>& launches,
+                         hipStream_t st);
 // out[i] = nodes[idx[i]] (32 bytes each)
 void gather_hashes_launch(const uint8_t* nodes, const uint64_t* idx, uint64_t count, uint8_t* out, hipStream_t st);
 void fill_zero_launch(uint8_t* p, uint64_t bytes, hipStream_t st);

@@ -266,6 +266,145 @@ __global__ __launch_bounds__(512) void merkle_top3_kernel(uint8_t* __restrict__ 
     }
 }
 
+// ------------------------------------------------ many trees per launch
+// The trees of a batch built level by level together: each launch covers one
+// step (the same kernel body as above) of every tree that has it, so a batch
+// of T trees costs a handful of launches instead of T times as many.  A
+// launch's segment table lists, per tree, its node array (byte offset from
+// `base`), the child level's offset and width, and the launch's first
+// workgroup for it; a workgroup finds its tree by binary search over the
+// table (workgroup-uniform, scalar loads).
+__device__ __forceinline__ MerkleSeg find_seg(const MerkleSeg* segs, uint32_t nseg) {
+    uint32_t lo = 0, hi = nseg;  // segs[lo].block0 <= blockIdx.x < segs[hi].block0
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (segs[mid].block0 <= blockIdx.x)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return segs[lo];
+}
+
+__global__ __launch_bounds__(256) void merkle_2level_multi_kernel(uint8_t* __restrict__ base,
+                                                                  const MerkleSeg* __restrict__ segs, uint32_t nseg) {
+    const MerkleSeg g = find_seg(segs, nseg);
+    uint8_t* nodes = base + g.nodes;
+    const uint64_t i = (uint64_t)(blockIdx.x - g.block0) * blockDim.x + threadIdx.x;
+    if (i >= g.width / 4) return;
+    uint32_t m[16], h0[8], h1[8], p[8];
+    load64(nodes + 32 * (g.child_off + 4 * i), m);
+    b3::hash64(m, h0);
+    load64(nodes + 32 * (g.child_off + 4 * i + 2), m);
+    b3::hash64(m, h1);
+    const uint64_t parent_off = g.child_off + g.width;
+    store_hash(nodes + 32 * (parent_off + 2 * i), h0);
+    store_hash(nodes + 32 * (parent_off + 2 * i + 1), h1);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        m[k] = h0[k];
+        m[8 + k] = h1[k];
+    }
+    b3::hash64(m, p);
+    store_hash(nodes + 32 * (parent_off + g.width / 2 + i), p);
+}
+
+__global__ __launch_bounds__(256) void merkle_level_multi_kernel(uint8_t* __restrict__ base,
+                                                                 const MerkleSeg* __restrict__ segs, uint32_t nseg) {
+    const MerkleSeg g = find_seg(segs, nseg);
+    uint8_t* nodes = base + g.nodes;
+    const uint64_t i = (uint64_t)(blockIdx.x - g.block0) * blockDim.x + threadIdx.x;
+    if (i >= g.width / 2) return;
+    uint32_t m[16], h[8];
+    load64(nodes + 32 * (g.child_off + 2 * i), m);
+    b3::hash64(m, h);
+    store_hash(nodes + 32 * (g.child_off + g.width + i), h);
+}
+
+// merkle_top3_kernel's three levels for each tree: workgroup w of a tree
+// reduces its children [256 w, 256 w + 256) to 32 parents
+__global__ __launch_bounds__(512) void merkle_top3_multi_kernel(uint8_t* __restrict__ base,
+                                                                const MerkleSeg* __restrict__ segs, uint32_t nseg) {
+    __shared__ uint32_t lvl[256 * 8];
+    const MerkleSeg g = find_seg(segs, nseg);
+    uint8_t* nodes = base + g.nodes;
+    const uint64_t w = blockIdx.x - g.block0;
+    for (uint32_t i = threadIdx.x; i < 256; i += blockDim.x) {
+        const uint4* src = reinterpret_cast<const uint4*>(nodes + 32 * (g.child_off + 256 * w + i));
+        uint4 a = src[0], b = src[1];
+        uint32_t* d = lvl + 8 * i;
+        d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w;
+        d[4] = b.x; d[5] = b.y; d[6] = b.z; d[7] = b.w;
+    }
+    __syncthreads();
+    uint64_t off = g.child_off, width = g.width;
+    uint32_t seg = 256;
+    for (int lev = 0; lev < 3; ++lev) {
+        const uint64_t parent_off = off + width;
+        const uint32_t np = seg / 2;
+        const uint32_t i = threadIdx.x >> 2, k = threadIdx.x & 3;
+        uint32_t lo = 0, hi = 0;
+        if (i < np) {
+            uint32_t m[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) m[q] = lvl[16 * i + q];
+            b3::hash64_quad(m, k, lo, hi);
+            uint32_t* dst = reinterpret_cast<uint32_t*>(nodes + 32 * (parent_off + w * np + i));
+            dst[k] = lo;
+            dst[4 + k] = hi;
+        }
+        __syncthreads();
+        if (i < np) {
+            lvl[8 * i + k] = lo;
+            lvl[8 * i + 4 + k] = hi;
+        }
+        __syncthreads();
+        off = parent_off;
+        width /= 2;
+        seg = np;
+    }
+}
+
+// the top (child level of at most 256 nodes) of every tree: one workgroup
+// per tree, quad mode, each level staged in LDS
+__global__ __launch_bounds__(1024) void merkle_top_multi_kernel(uint8_t* __restrict__ base,
+                                                                const MerkleSeg* __restrict__ segs, uint32_t nseg) {
+    __shared__ uint32_t lvl[256 * 8];
+    const MerkleSeg g = find_seg(segs, nseg);
+    uint8_t* nodes = base + g.nodes;
+    uint64_t child_off = g.child_off, child_width = g.width;
+    for (uint64_t i = threadIdx.x; i < child_width; i += blockDim.x) {
+        const uint4* src = reinterpret_cast<const uint4*>(nodes + 32 * (child_off + i));
+        uint4 a = src[0], b = src[1];
+        uint32_t* d = lvl + 8 * i;
+        d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w;
+        d[4] = b.x; d[5] = b.y; d[6] = b.z; d[7] = b.w;
+    }
+    __syncthreads();
+    while (child_width > 1) {
+        const uint64_t width = child_width / 2, parent_off = child_off + child_width;
+        const uint32_t i = threadIdx.x >> 2, k = threadIdx.x & 3;
+        uint32_t lo = 0, hi = 0;
+        if (i < width) {
+            uint32_t m[16];
+#pragma unroll
+            for (int q = 0; q < 16; ++q) m[q] = lvl[16 * i + q];
+            b3::hash64_quad(m, k, lo, hi);
+            uint32_t* dst = reinterpret_cast<uint32_t*>(nodes + 32 * (parent_off + i));
+            dst[k] = lo;
+            dst[4 + k] = hi;
+        }
+        __syncthreads();
+        if (i < width) {
+            lvl[8 * i + k] = lo;
+            lvl[8 * i + 4 + k] = hi;
+        }
+        __syncthreads();
+        child_off = parent_off;
+        child_width = width;
+    }
+}
+
 __global__ __launch_bounds__(256) void gather_hashes_kernel(const uint8_t* __restrict__ nodes,
                                                             const uint64_t* __restrict__ idx, uint64_t count,
                                                             uint8_t* __restrict__ out) {
@@ -350,6 +489,76 @@ void merkle_build_launch(uint8_t* nodes, uint64_t n_leaves, hipStream_t st, uint
     }
     if (width > 1)
         hipLaunchKernelGGL(merkle_top_kernel, dim3(1), dim3(TOP_WIDTH / 2), 0, st, nodes, off, width, root_copy);
+}
+
+int merkle_multi_plan(const uint64_t* nodes_off, const uint64_t* leaves, size_t count,
+                      std::vector<MerkleSeg>& segs, std::vector<MerkleMultiLaunch>& launches) {
+    segs.clear();
+    launches.clear();
+    // per tree: its current child level (offset, width); every tree runs the
+    // steps of merkle_build_launch, one launch per step across the trees
+    std::vector<uint64_t> off(count, 0), width(count);
+    for (size_t k = 0; k < count; ++k) {
+        if (!leaves[k] || (leaves[k] & (leaves[k] - 1))) return -1;  // powers of two only (no padding)
+        width[k] = leaves[k];
+    }
+    auto add_launch = [&](uint32_t kind, auto take, auto blocks, auto advance) {
+        MerkleMultiLaunch L{kind, (uint32_t)segs.size(), 0, 0};
+        for (size_t k = 0; k < count; ++k) {
+            if (!take(width[k])) continue;
+            segs.push_back(MerkleSeg{nodes_off[k], off[k], width[k], L.blocks});
+            L.blocks += blocks(width[k]);
+            ++L.nseg;
+            advance(off[k], width[k]);
+        }
+        if (L.nseg) launches.push_back(L);
+        return L.nseg;
+    };
+    // two levels per step while the child level has at least 4 * TOP_WIDTH nodes
+    while (add_launch(
+        0, [](uint64_t w) { return w >= 4 * TOP_WIDTH; }, [](uint64_t w) { return (uint32_t)blocks_for(w / 4, 256); },
+        [](uint64_t& o, uint64_t& w) {
+            o += w + w / 2;
+            w /= 4;
+        })) {
+    }
+    while (add_launch(
+        1, [](uint64_t w) { return w > TOP_WIDTH; }, [](uint64_t w) { return (uint32_t)blocks_for(w / 2, 256); },
+        [](uint64_t& o, uint64_t& w) {
+            o += w;
+            w /= 2;
+        })) {
+    }
+    add_launch(
+        2, [](uint64_t w) { return w > 256; }, [](uint64_t w) { return (uint32_t)(w / 256); },
+        [](uint64_t& o, uint64_t& w) {
+            o += w + w / 2 + w / 4;
+            w /= 8;
+        });
+    add_launch(
+        3, [](uint64_t w) { return w > 1; }, [](uint64_t) { return 1u; }, [](uint64_t&, uint64_t& w) { w = 1; });
+    return 0;
+}
+
+void merkle_multi_launch(uint8_t* base, const MerkleSeg* d_segs, const std::vector<MerkleMultiLaunch>& launches,
+                         hipStream_t st) {
+    for (const MerkleMultiLaunch& L : launches) {
+        const MerkleSeg* t = d_segs + L.seg0;
+        switch (L.kind) {
+            case 0:
+                hipLaunchKernelGGL(merkle_2level_multi_kernel, dim3(L.blocks), dim3(256), 0, st, base, t, L.nseg);
+                break;
+            case 1:
+                hipLaunchKernelGGL(merkle_level_multi_kernel, dim3(L.blocks), dim3(256), 0, st, base, t, L.nseg);
+                break;
+            case 2:
+                hipLaunchKernelGGL(merkle_top3_multi_kernel, dim3(L.blocks), dim3(512), 0, st, base, t, L.nseg);
+                break;
+            default:
+                hipLaunchKernelGGL(merkle_top_multi_kernel, dim3(L.blocks), dim3(1024), 0, st, base, t, L.nseg);
+                break;
+        }
+    }
 }
 
 void gather_hashes_launch(const uint8_t* nodes, const uint64_t* idx, uint64_t count, uint8_t* out, hipStream_t st) {

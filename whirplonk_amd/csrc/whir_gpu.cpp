@@ -342,7 +342,7 @@ int whir_gpu::pow_run(hipStream_t st, unsigned long long* slots, int& set, const
 }
 
 int whir_gpu::commit_device(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_polys, size_t poly_size, size_t cw,
-                      size_t depth, uint64_t* d_matrix, uint8_t* d_nodes, uint8_t* out_root, int ext) {
+                      size_t depth, uint64_t* d_matrix, uint8_t* d_nodes, uint8_t* out_root, int ext, bool tree) {
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     int rc;
     bool fused = false;
@@ -360,6 +360,10 @@ int whir_gpu::commit_device(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_
     else if (!fused)
         leaf_hash_launch(d_matrix, cw, (uint32_t)row_u64, d_nodes, ctx->stream);
     if ((rc = ctx->mark(2))) return rc;
+    if (!tree) {
+        HIPCHK(hipGetLastError());
+        return (rc = ctx->collect(0, 1, 0)) ? rc : ctx->collect(1, 2, 1);
+    }
     // from two leaves up the tree's last kernel also writes the root to mapped
     // host memory, so reading it costs no copy
     const bool mapped_root = out_root && cw >= 2;
