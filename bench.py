@@ -124,10 +124,9 @@ def gpu_leg(steps: int, warmup: int, coeffs, b, r):
     d_fa = ctx.alloc(24 * (poly // 2))
     d_fb = ctx.alloc(24 * (poly // 2))
 
-    # The sumcheck round (seeded r) shares only the read-only coefficients with
-    # the commit, so it runs on a second context's stream, from a worker
-    # thread, while the commit runs: the HBM-bound round fills what the
-    # VALU-bound LDE leaves of the memory system.
+    # One step: the commit (root to the host), then the first sumcheck round
+    # over the same coefficients with the seeded challenge (c0, c2 to the
+    # host), both on the context's one stream.
     def step():
         root = ctx.commit(d_f, 1, poly, cw, depth, d_m, d_n)
         c0, c2 = ctx.sumcheck_round(d_f, True, d_b, poly, r, d_fa, d_fb)
@@ -578,8 +577,8 @@ def commit_leg(args):
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u64",
         "data": "synthetic: seeded reference-LCG uniform Goldilocks inputs (rejection of x >= p)",
-        "config": {"workload": "plain GPU commit nv=24 rate 1/4 folding_factor 4 + first Ext3 sumcheck round "
-                               "(the round on a second stream, overlapping the commit)",
+        "config": {"workload": "plain GPU commit nv=24 rate 1/4 folding_factor 4, then the first Ext3 sumcheck "
+                               "round (one stream)",
                    "num_variables": NV, "codeword_rows": cw, "columns": depth, "committed_elems": n_cw,
                    "hash": "blake3", "seed": hex(SEED)},
         "stage_ms": stage_ms,
