@@ -102,10 +102,13 @@ def test_eq_golden(ctx, fixtures):
         ctx.free(d)
 
 
-@pytest.mark.parametrize("n,count", [(1, 1), (2, 3), (8, 3), (1024, 5), (4096, 60), (1 << 18, 253)])
-def test_geometric_accumulate(ctx, n, count):
+# (1024, 1500) and (2048, 1100, all base) pass the sweep's 1024 limb products
+# per accumulator, so its mid-sweep reductions run
+@pytest.mark.parametrize("n,count,base_every", [(1, 1, 2), (2, 3, 2), (8, 3, 2), (1024, 5, 2), (4096, 60, 2),
+                                                (1 << 18, 253, 2), (1024, 1500, 2), (2048, 1100, 1)])
+def test_geometric_accumulate(ctx, n, count, base_every):
     rnd = random.Random(n * 7 + count)
-    pts = _e3s(rnd, count, base_every=2)  # half the points in the base field
+    pts = _e3s(rnd, count, base_every=base_every)  # every base_every-th point in the base field
     scs = _e3s(rnd, count)
     base = _e3s(rnd, n).reshape(-1)
     d = ctx.upload(base)
