@@ -94,6 +94,7 @@ RS_CASES = [
     (1, 1 << 16, 1 << 14, 16),                                                      # nv=16, rate 1/4, ff=4
     (1, 16, 2048, 16), (1, 1 << 12, 1 << 12, 16), (5, 512, 256, 4), (2, 1 << 14, 1 << 16, 8),
     (1, 1 << 20, 1 << 18, 4),
+    (1, 1 << 10, 1 << 12, 16), (2, 1 << 9, 1 << 14, 16), (1, 1 << 8, 1 << 15, 8),     # one pass, m = 64, 32, 32
 ]
 
 
@@ -108,7 +109,10 @@ def test_rs_encode(orc, shape):
     assert np.array_equal(leaves, orc.commit_leaves_base(want, cw, num_polys * depth))
 
 
-@pytest.mark.parametrize("shape", [(1, 64, 256, 4), (2, 256, 1024, 16), (1, 16, 2048, 16), (3, 1024, 512, 4)])
+@pytest.mark.parametrize("shape", [(1, 64, 256, 4), (2, 256, 1024, 16), (1, 16, 2048, 16), (3, 1024, 512, 4),
+                                   # the last rounds' shapes: m = 4, 16, 32, 64 per column, 48 base columns
+                                   (1, 64, 1 << 12, 16), (1, 256, 1 << 14, 16), (1, 512, 1 << 12, 16),
+                                   (1, 1024, 1 << 13, 16)])
 def test_rs_encode_ext3(orc, shape):
     num_polys, poly_size, cw, depth = shape
     x = rand(77 + sum(shape), 3 * num_polys * poly_size)

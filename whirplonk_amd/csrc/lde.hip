@@ -735,6 +735,42 @@ __global__ __launch_bounds__(THREADS) void lde_pass_b_cols(LdeArgs a, uint64_t s
     }
 }
 
+// One-pass encode of a shape with m = M <= 64 coefficients per column and
+// many output cosets (the last WHIR rounds: m = 4 .. 64, thousands of
+// cosets): lane (s, c) computes the whole coset s of column c in registers,
+//   Y[s + E k][c] = DFT_M( f_c[j] * w_n^{s j} )[k],   k < M,
+// the workgroup's SPB = 256 / C cosets (at most 2048 / M) sharing one LDS
+// table of their twist factors w_n^{s j}, built one entry per lane, so a
+// table entry serves the C columns of its coset.  A coset's M output rows are
+// written C contiguous values at a time (row-major) or one column run
+// (column-major).
+template <int M>
+__global__ __launch_bounds__(256) void lde_small_direct(LdeArgs a, uint32_t spb) {
+    __shared__ uint64_t tw[2048];
+    const uint32_t C = (uint32_t)a.C;
+    const uint64_t s0 = (uint64_t)blockIdx.x * spb;
+    for (uint32_t q = threadIdx.x; q < spb * M; q += blockDim.x) {
+        const uint64_t s = s0 + q / M, j = q % M;
+        tw[q] = s < a.E ? twid(a, (s * j) & (a.n - 1)) : 1;
+    }
+    __syncthreads();
+    const uint32_t c = threadIdx.x % C, sl = threadIdx.x / C;
+    const uint64_t s = s0 + sl;
+    if (sl >= spb || s >= a.E) return;
+    uint64_t x[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) x[j] = a.in[in_index(a, c, j)];
+#pragma unroll
+    for (int j = 1; j < M; ++j) x[j] = gl::mul(x[j], tw[sl * M + j]);
+    dft::dft_reg<M, false>(x);
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+        const uint64_t i = s + a.E * k;
+        const uint64_t idx = a.row_major ? (i * C + c) : (c * a.n + i);
+        a.out[idx * a.out_el + a.out_comp] = x[k];
+    }
+}
+
 // m == 1: every codeword entry of column c is its single coefficient.
 __global__ __launch_bounds__(256) void lde_broadcast_kernel(LdeArgs a) {
     const uint64_t total = a.n * a.C;
@@ -1079,6 +1115,21 @@ int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* s
         const uint64_t total = s.n * s.C;
         const unsigned blocks = (unsigned)std::min<uint64_t>((total + 255) / 256, 8192);
         hipLaunchKernelGGL(lde_broadcast_kernel, dim3(blocks), dim3(256), 0, st, a);
+        return 0;
+    }
+    // forward one-pass shapes of at most 64 coefficients per column without
+    // fused leaves: whole cosets in registers, one lane per (coset, column)
+    if (p.direct && !fuse && !s.inverse && s.m >= 2 && s.m <= 32 && p.E >= 2 && s.C <= 256) {
+        const uint32_t spb = (uint32_t)std::min<uint64_t>(256 / s.C, 2048 / s.m);
+        const unsigned blocks = (unsigned)((p.E + spb - 1) / spb);
+        switch (s.m) {
+            case 2: hipLaunchKernelGGL(lde_small_direct<2>, dim3(blocks), dim3(256), 0, st, a, spb); break;
+            case 4: hipLaunchKernelGGL(lde_small_direct<4>, dim3(blocks), dim3(256), 0, st, a, spb); break;
+            case 8: hipLaunchKernelGGL(lde_small_direct<8>, dim3(blocks), dim3(256), 0, st, a, spb); break;
+            case 16: hipLaunchKernelGGL(lde_small_direct<16>, dim3(blocks), dim3(256), 0, st, a, spb); break;
+            case 32: hipLaunchKernelGGL(lde_small_direct<32>, dim3(blocks), dim3(256), 0, st, a, spb); break;
+            default: hipLaunchKernelGGL(lde_small_direct<64>, dim3(blocks), dim3(256), 0, st, a, spb); break;
+        }
         return 0;
     }
     // M1 >= 16 with a full tile: the persistent register-tile pass A
