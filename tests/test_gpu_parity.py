@@ -323,6 +323,17 @@ def test_ood_horner(orc, n):
     assert np.array_equal(got[:3], orc.horner_ext3_ext3(fe, pts[:3]))
 
 
+def test_ood_horner_power_chunks(orc):
+    """More coefficients than 256 power-table entries per lane (2^26 + 5 at
+    2^18 lanes: 257 per lane): the sum runs over two chunks of the table, the
+    second scaled by z^(256 T)."""
+    n = (1 << 26) + 5
+    f = rand(n, n)
+    pts = rand(n + 9, 3)
+    got = W.ood_evaluate_from_base(f, pts, 1, n)
+    assert np.array_equal(got[:3], orc.horner_base_ext3(f, pts[:3]))
+
+
 def test_ood_multi_vector_order(orc):
     nv, vs = 3, 1000
     f = rand(3, nv * vs)
