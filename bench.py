@@ -569,6 +569,8 @@ def commit_leg(args):
                    "sample": f"full workload (nv=24 commit + first sumcheck round), reference C++ CPU path "
                              f"(oracle/_ref, OpenMP {c['threads']} threads), 1 warm-up + median of {c['reps']}",
                    "seconds_per_step": round(c["seconds"], 4),
+                   "blake3": "SIMD: the reference's vendored BLAKE3 with its SSE2 / SSE4.1 / AVX2 kernels "
+                             "(runtime dispatch, AVX-512 off), as its CMake build compiles it",
                    "omp_proc_bind": os.environ.get("OMP_PROC_BIND"), **host_cpu_info()}
             root_ok = (c["root"] == root and np.array_equal(c["c0"], c0) and np.array_equal(c["c2"], c2))
 
