@@ -151,8 +151,8 @@ def gpu_leg(steps: int, warmup: int, coeffs, b, r):
 
 
 def host_resident_leg(coeffs, steps: int = 10):
-    """The commit as BASELINE.md section 3 times it: coefficients in pinned host
-    memory to the root in host memory (the 128 MiB upload included), and the
+    """The commit timed from the coefficients in pinned host memory to the root
+    in host memory (the 128 MiB upload included), and the
     host-pointer drop-in whir_gpu_interleaved_rs_encode_blake3_matrix_leaves
     (coefficients in, codeword and leaves back to pageable host memory)."""
     import whirplonk_amd as W
