@@ -124,6 +124,28 @@ def test_sha256_dev_commit(ctx, shape):
     assert root_b == oracle.merkle_build(oracle.commit_leaves_base(want, cw, cols), cw)[-32:].tobytes()
 
 
+@pytest.mark.parametrize("poly_size,cw", [(1 << 12, 1 << 11), (1 << 15, 1 << 13), (1 << 16, 1 << 16)])
+def test_sha256_dev_commit_ext3(ctx, poly_size, cw):
+    """Ext3 round commits with SHA-256 trees: from 2048 coefficients per column
+    the 384-byte row leaves are hashed inside the encode (six data blocks over
+    the three 16-column sub-tiles, then the padding block); every node of the
+    tree against the oracle."""
+    depth = 16
+    x = rand(17 + poly_size + cw, 3 * poly_size)
+    print("sha256 ext3 commit", poly_size, cw, flush=True)
+    dx = ctx.upload(x)
+    dm = ctx.alloc(24 * cw * depth)
+    dn = ctx.alloc(32 * W.merkle_num_nodes(cw))
+    root = ctx.commit(dx, 1, poly_size, cw, depth, dm, dn, ext3=True)
+    m = ctx.download(dm, 3 * cw * depth)
+    nodes = ctx.download(dn, 32 * W.merkle_num_nodes(cw), np.uint8)
+    want = oracle.rs_encode_ext3(x, 1, poly_size, cw, depth)
+    assert np.array_equal(m, want)
+    wn = oracle.sha256_merkle_build(oracle.sha256_leaves(want, cw, 3 * depth), cw)
+    assert np.array_equal(nodes, wn)
+    assert root == wn[-32:].tobytes()
+
+
 def test_sha256_commit_golden_whir(ctx, fixtures):
     """The reference's golden_whir proofs (dump_whir.cpp: SHA-256 trees, depth
     1, codeword 2x the vector size): the device commit of their input vectors

@@ -564,11 +564,12 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4, 4)))
 // base columns each, in order, like lde_pass_b_rows's tile; sub-tile q's
 // columns 16q .. 16q + 15 are bytes 128q .. 128q + 127 of every row, i.e.
 // blocks 2q and 2q + 1 of its leaf, so each lane carries the chaining values
-// of its two rows in registers from one sub-tile to the next (one chunk:
-// CHUNK_START on block 0, CHUNK_END | ROOT on block 5, counter 0).  The next
-// sub-tile's loads are in flight during the current one's step 2, hashing and
-// stores; the inter-pass twiddles are computed once per tile.
-template <bool INV>
+// of its two rows in registers from one sub-tile to the next (BLAKE3: one
+// chunk, CHUNK_START on block 0, CHUNK_END | ROOT on block 5, counter 0;
+// SHA-256: the state over the six data blocks, then the padding block).  The
+// next sub-tile's loads are in flight during the current one's step 2,
+// hashing and stores; the inter-pass twiddles are computed once per tile.
+template <bool INV, uint32_t HASH = HASH_BLAKE3>
 __global__ __launch_bounds__(THREADS) void lde_pass_b_rows_e3(LdeArgs a, uint64_t s0, uint32_t ntiles) {
     constexpr int M2 = 1024, R = 32, PAD = M2 + M2 / R + 1, SUB = 3, COLS = 3 * FUSE_COLS;
     static_assert(PAD == 1057, "rows_idx assumes the padded 1024-point layout");
@@ -630,27 +631,59 @@ __global__ __launch_bounds__(THREADS) void lde_pass_b_rows_e3(LdeArgs a, uint64_
 #pragma unroll
         for (int h = 0; h < M2 / THREADS; ++h) {
             const uint32_t r = tid + h * THREADS;
-            if (sub == 0) {
+            if constexpr (HASH == HASH_SHA2) {
+                // SHA-256 of the 384-byte row: six data blocks (two per
+                // sub-tile, the limbs' bytes as big-endian words), then the
+                // constant padding block of a 3072-bit message
+                if (sub == 0) s256::init(cv[h]);
 #pragma unroll
-                for (int i = 0; i < 8; ++i) cv[h][i] = b3::iv(i);
-            }
+                for (int blk = 0; blk < 2; ++blk) {
+                    uint32_t wd[16];
 #pragma unroll
-            for (int blk = 0; blk < 2; ++blk) {
-                uint32_t m[16];
-#pragma unroll
-                for (int cc = 0; cc < 8; ++cc) {
-                    const uint64_t x = lds[rows_idx(r, 8 * blk + cc)];
-                    m[2 * cc] = (uint32_t)x;
-                    m[2 * cc + 1] = (uint32_t)(x >> 32);
+                    for (int cc = 0; cc < 8; ++cc) {
+                        const uint64_t x = lds[rows_idx(r, 8 * blk + cc)];
+                        wd[2 * cc] = s256::bswap((uint32_t)x);
+                        wd[2 * cc + 1] = s256::bswap((uint32_t)(x >> 32));
+                    }
+                    s256::compress(cv[h], wd);
                 }
-                const uint32_t flags =
-                    (sub == 0 && blk == 0) ? b3::CHUNK_START : ((sub == SUB - 1 && blk == 1) ? (b3::CHUNK_END | b3::ROOT) : 0u);
-                b3::compress(cv[h], m, 0, 64, flags, cv[h]);
-            }
-            if (sub == SUB - 1) {
-                uint4* dst = reinterpret_cast<uint4*>(a.leaves + 32 * (i1 + a.L * r));
-                dst[0] = make_uint4(cv[h][0], cv[h][1], cv[h][2], cv[h][3]);
-                dst[1] = make_uint4(cv[h][4], cv[h][5], cv[h][6], cv[h][7]);
+                if (sub == SUB - 1) {
+                    uint32_t wd[16];
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) wd[i] = 0;
+                    wd[0] = 0x80000000u;
+                    wd[15] = 384 * 8;
+                    s256::compress(cv[h], wd);
+                    uint32_t out[8];
+                    s256::digest(cv[h], out);
+                    uint4* dst = reinterpret_cast<uint4*>(a.leaves + 32 * (i1 + a.L * r));
+                    dst[0] = make_uint4(out[0], out[1], out[2], out[3]);
+                    dst[1] = make_uint4(out[4], out[5], out[6], out[7]);
+                }
+            } else {
+                if (sub == 0) {
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) cv[h][i] = b3::iv(i);
+                }
+#pragma unroll
+                for (int blk = 0; blk < 2; ++blk) {
+                    uint32_t m[16];
+#pragma unroll
+                    for (int cc = 0; cc < 8; ++cc) {
+                        const uint64_t x = lds[rows_idx(r, 8 * blk + cc)];
+                        m[2 * cc] = (uint32_t)x;
+                        m[2 * cc + 1] = (uint32_t)(x >> 32);
+                    }
+                    const uint32_t flags = (sub == 0 && blk == 0)
+                                               ? b3::CHUNK_START
+                                               : ((sub == SUB - 1 && blk == 1) ? (b3::CHUNK_END | b3::ROOT) : 0u);
+                    b3::compress(cv[h], m, 0, 64, flags, cv[h]);
+                }
+                if (sub == SUB - 1) {
+                    uint4* dst = reinterpret_cast<uint4*>(a.leaves + 32 * (i1 + a.L * r));
+                    dst[0] = make_uint4(cv[h][0], cv[h][1], cv[h][2], cv[h][3]);
+                    dst[1] = make_uint4(cv[h][4], cv[h][5], cv[h][6], cv[h][7]);
+                }
             }
         }
         // this sub-tile's 16 columns of every row: 8 lanes per 128-byte segment
@@ -1019,9 +1052,10 @@ int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* s
     // only; for other shapes the caller runs the separate leaf kernel
     const bool sha2 = s.leaf_hash == HASH_SHA2;
     if (sha2 && !(LDE_PB_PERSIST && p.M2 == 1024 && !p.direct && !s.inverse)) fuse = false;
-    // Ext3 rows of 16 Ext3 columns (48 base columns): BLAKE3 leaves fused into
-    // the three-sub-tile persistent pass B (forward, M2 = 1024, with a first pass)
-    const bool fuse3 = leaves && !sha2 && s.split3 && s.row_major && s.C == 3 * FUSE_COLS && p.TB == 1 &&
+    // Ext3 rows of 16 Ext3 columns (48 base columns): the leaves (either
+    // engine) fused into the three-sub-tile persistent pass B (forward,
+    // M2 = 1024, with a first pass)
+    const bool fuse3 = leaves && s.split3 && s.row_major && s.C == 3 * FUSE_COLS && p.TB == 1 &&
                        p.CB == FUSE_COLS && s.out_el == 1 && LDE_PB_PERSIST && p.M2 == 1024 && !p.direct &&
                        !s.inverse;
     a.leaves = (fuse || fuse3) ? leaves : nullptr;
@@ -1091,13 +1125,13 @@ int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* s
     }
     // other M2 = 1024 shapes with 16 Z columns per tile: the persistent
     // prefetching pass B without leaves
-    RowsFn kr3 = lde_pass_b_rows_e3<false>;
+    RowsFn kr3 = sha2 ? lde_pass_b_rows_e3<false, HASH_SHA2> : lde_pass_b_rows_e3<false>;
     if (fuse3) {
         std::lock_guard<std::mutex> g(attr_mu);
-        static bool e3_attr = false;
-        if (!e3_attr) {
+        static bool e3_attr[2] = {};
+        if (!e3_attr[sha2]) {
             (void)hipFuncSetAttribute((const void*)kr3, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-            e3_attr = true;
+            e3_attr[sha2] = true;
         }
     }
     const bool cols = LDE_PB_PERSIST && !fuse && !fuse3 && p.M2 == 1024 && !p.direct && p.TB * p.CB == 16 &&
