@@ -351,10 +351,14 @@ def batched_leg(steps: int, cpu: bool):
             ctx.commit(d[k], 1, sz, (sz // 16) * 4, 16, *bufs[sz])
     ctx.sync()
     plain_ms = 1000.0 * (time.perf_counter() - t0) / steps
+    form, per_poly_ms, level_ms = b.form()
     b.close()
     ctx.close()
     out = {"workload": "batched commit: 64 polynomials nv uniform in [14,22] (seeded), rate 1/4, folding_factor 4, "
-                       "one HIP graph launch (4 branch streams, autotuned radix split)",
+                       "one HIP graph launch (autotuned form: level-synchronous encodes over all polynomials, "
+                       "or per-polynomial commits on 4 branch streams with autotuned radix splits)",
+           "form": "level-synchronous" if form == 1 else "per-polynomial",
+           "form_replay_ms": {"level_synchronous": round(level_ms, 3), "per_polynomial": round(per_poly_ms, 3)},
            "polynomials": len(sizes), "total_coefficients": sum(sizes), "committed_elems": committed,
            "ms_per_batch": round(ms, 3), "elem_per_s": round(committed / ms * 1000.0, 1),
            "plain_ms_per_batch": round(plain_ms, 3), "speedup_vs_plain": round(plain_ms / ms, 3),

@@ -414,6 +414,22 @@ int whir_tuning_import(const uint64_t* records, size_t words);
 typedef struct whir_batch whir_batch;
 int whir_batch_create(whir_ctx* ctx, size_t count, const uint64_t* sizes, uint32_t log_inv_rate,
                       uint32_t folding_factor, uint32_t num_branches, int autotune, whir_batch** out);
+/* whir_batch_create with the batch form chosen by the caller: 1 the
+ * level-synchronous form (BLAKE3 batches, folding factor 4, rate <= 1/2: every
+ * polynomial with 2^10 to 2^18 coefficients per column encoded, leaves
+ * included, by one first-pass and one second-pass launch together, then the
+ * trees level by level across all polynomials), 0 one commit per polynomial
+ * on the branch streams, -1 the library's choice (with autotune, both forms
+ * are captured and the one whose graph replays faster is kept; without it,
+ * the level-synchronous form where it applies).  All forms give the same
+ * roots.  No reference counterpart: the reference commits one polynomial per
+ * call (cuda_integration.hpp:555-621). */
+int whir_batch_create_form(whir_ctx* ctx, size_t count, const uint64_t* sizes, uint32_t log_inv_rate,
+                           uint32_t folding_factor, uint32_t num_branches, int autotune, int form,
+                           whir_batch** out);
+/* The form a batch runs (1 level-synchronous, 0 per polynomial) and, when
+ * create timed both, their replay times in ms (else -1). */
+int whir_batch_form(whir_batch* batch, double* per_poly_ms, double* level_ms);
 int whir_batch_destroy(whir_batch* batch);
 uint64_t* whir_batch_coeffs(whir_batch* batch, size_t k);
 uint64_t* whir_batch_matrix(whir_batch* batch, size_t k);

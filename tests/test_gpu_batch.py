@@ -81,6 +81,36 @@ def test_batch_roots_and_replays(ctx):
         assert r[1] == _commit_ref(newc) and r[0] == eager[0] and r[2:] == eager[2:]
 
 
+@pytest.mark.parametrize("rate,lgs", [(2, list(range(22, 13, -1)) + [14, 12]), (1, [18, 15, 14, 11]),
+                                       (3, [21, 17, 14])])
+def test_batch_level_synchronous(ctx, rate, lgs):
+    """The level-synchronous form (one first-pass launch and one second-pass
+    launch over all polynomials, trees level by level): every root equals the
+    reference build's commit of that polynomial alone, on the eager run and on
+    graph replays, at every first-pass size (2^11 .. 2^18 coefficients per
+    column) and one-pass size (2^10), beside a polynomial too small for it
+    (committed per polynomial in the same graph); whole node arrays and
+    codewords against the oracle for a few."""
+    sizes = [1 << lg for lg in lgs]
+    polys = _polys(sizes, 0x1E7E1 + rate)
+    print("level batch rate", rate, sizes, flush=True)
+    want = [_commit_ref(c, rate=rate) for c in polys]
+    with W.Batch(ctx, sizes, log_inv_rate=rate, folding_factor=4, branches=2, autotune=False, form=1) as b:
+        assert b.form()[0] == 1
+        for k, c in enumerate(polys):
+            b.upload(k, c)
+        b.run(graph=False)
+        assert b.roots() == want
+        for rep in range(2):
+            b.run(graph=True)
+            assert b.roots() == want, rep
+        for k in sorted(range(len(sizes)), key=lambda k: sizes[k])[:3]:
+            rows = (sizes[k] // 16) << rate
+            m = oracle.rs_encode(polys[k], 1, sizes[k], rows, 16)
+            assert np.array_equal(b.matrix(k), m), sizes[k]
+            assert np.array_equal(b.nodes(k), oracle.merkle_build(oracle.commit_leaves_base(m, rows, 16), rows))
+
+
 def _force(rec_key, lg):
     W.tuning_import(np.array(list(rec_key) + [lg], dtype=np.uint64))
 
@@ -165,7 +195,21 @@ def test_batch_workload_64_polys(ctx):
     polys = _polys(sizes, BATCH_SEED)
     print("batch workload", sum(sizes), "coefficients", flush=True)
     want = [_commit_ref(c) for c in polys]
+    # both batch forms, each eager and on graph replays
+    for form in (0, 1):
+        with W.Batch(ctx, sizes, log_inv_rate=2, folding_factor=4, branches=4, autotune=False, form=form) as b:
+            assert b.form()[0] == form
+            for k, c in enumerate(polys):
+                b.upload(k, c)
+            b.run(graph=False)
+            assert b.roots() == want, form
+            for rep in range(2):
+                b.run(graph=True)
+                assert b.roots() == want, (form, rep)
     with W.Batch(ctx, sizes, log_inv_rate=2, folding_factor=4, branches=4, autotune=True) as b:
+        f, per_ms, level_ms = b.form()
+        print("batch forms: per-polynomial", per_ms, "ms, level-synchronous", level_ms, "ms, chose", f, flush=True)
+        assert per_ms > 0 and level_ms > 0 and f == (1 if level_ms <= per_ms else 0)
         for k, c in enumerate(polys):
             b.upload(k, c)
         b.run(graph=False)

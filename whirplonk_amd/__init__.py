@@ -127,6 +127,9 @@ _SIGS = [
     ("whir_tuning_import", _int, [_u64p, _sz]),
     ("whir_batch_create", _int, [_vp, _sz, _u64p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _int,
                                  ctypes.POINTER(_vp)]),
+    ("whir_batch_create_form", _int, [_vp, _sz, _u64p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _int,
+                                      _int, ctypes.POINTER(_vp)]),
+    ("whir_batch_form", _int, [_vp, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     ("whir_batch_destroy", _int, [_vp]),
     ("whir_batch_coeffs", _vp, [_vp, _sz]),
     ("whir_batch_matrix", _vp, [_vp, _sz]),
@@ -665,14 +668,24 @@ class Batch:
     """Batched commits of many polynomials in one graph launch (whir_batch_*)."""
 
     def __init__(self, ctx: "Context", sizes, log_inv_rate: int = 2, folding_factor: int = 4,
-                 branches: int = 8, autotune: bool = True):
+                 branches: int = 8, autotune: bool = True, form: int = -1):
+        """form: 1 level-synchronous encodes, 0 one commit per polynomial,
+        -1 the library's choice (timed when autotune is set)."""
         self.ctx = ctx
         self.sizes = [int(s) for s in sizes]
         sz = _u64(self.sizes)
         h = _vp()
-        _check(load().whir_batch_create(ctx.h, len(self.sizes), _p64(sz), log_inv_rate, folding_factor, branches,
-                                        int(autotune), ctypes.byref(h)))
+        _check(load().whir_batch_create_form(ctx.h, len(self.sizes), _p64(sz), log_inv_rate, folding_factor,
+                                             branches, int(autotune), int(form), ctypes.byref(h)))
         self.h = h
+
+    def form(self):
+        """(form, per-polynomial ms, level-synchronous ms): the form the batch
+        runs (1 level-synchronous, 0 per polynomial) and, when create timed
+        both, their graph replay times (else -1)."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        f = load().whir_batch_form(self.h, ctypes.byref(a), ctypes.byref(b))
+        return f, a.value, b.value
 
     def close(self):
         if self.h:

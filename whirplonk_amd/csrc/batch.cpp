@@ -108,6 +108,18 @@ struct whir_batch {
     bool multi_tree = false;
     size_t segs_off = 0;
     std::vector<MerkleMultiLaunch> tree_launches;
+    // level-synchronous encodes (BLAKE3, folding factor 4, 2^10 <= m <= 2^18):
+    // the polynomials in `level_polys` are encoded, leaves included, by one
+    // first-pass and one second-pass launch together (lde_multi_launch); the
+    // others keep their per-polynomial commits on the branch streams
+    std::vector<size_t> level_polys;
+    std::vector<std::vector<size_t>> level_branch_polys;  // the rest, per branch
+    LdeMultiPlan lplan;
+    size_t lsegs_off = 0;
+    uint32_t nlseg = 0;
+    const uint64_t* t1024 = nullptr;
+    bool use_level = false;  // the form the graph was captured with
+    double form_ms[2] = {-1.0, -1.0};  // measured replay: per-polynomial form, level-synchronous form
     uint8_t* region = nullptr;
     std::vector<hipStream_t> branches;
     std::vector<hipEvent_t> events;
@@ -127,17 +139,20 @@ struct whir_batch {
         if (region) (void)hipFree(region);
     }
 
-    // enqueue every commit: branch b runs its polynomials in order on `on[b]`
+    // enqueue every per-polynomial commit: branch b runs its polynomials in
+    // order on `on[b]` (with the level-synchronous form, only those the
+    // multi-polynomial launches do not cover)
     int enqueue(const std::vector<hipStream_t>& on, hipStream_t main) {
         hipStream_t keep = ctx->stream;
         const uint32_t keep_hash = ctx->hash;
         ctx->hash = hash;
         int rc = 0;
-        for (size_t b = 0; b < branch_polys.size() && !rc; ++b) {
+        const auto& per_branch = use_level ? level_branch_polys : branch_polys;
+        for (size_t b = 0; b < per_branch.size() && !rc; ++b) {
             ctx->stream = on[b];
             ctx->ext_scratch = scratch_bytes[b] ? region + scratch_off[b] : nullptr;
             ctx->ext_scratch_bytes = scratch_bytes[b];
-            for (size_t k : branch_polys[b]) {
+            for (size_t k : per_branch[b]) {
                 rc = commit_device(ctx, (const uint64_t*)(region + coeff_off[k]), 1, sizes[k], rows[k], depth,
                                    (uint64_t*)(region + matrix_off[k]), region + nodes_off[k], nullptr, 0,
                                    !multi_tree);
@@ -160,6 +175,7 @@ struct whir_batch {
         for (size_t b = 0; b < nb; ++b) HIPCHK(hipStreamWaitEvent(branches[b], events[0], 0));
         int rc = enqueue(branches, main);
         if (rc) return rc;
+        if (use_level) lde_multi_launch((const LdeMultiSeg*)(region + lsegs_off), nlseg, lplan, t1024, main);
         for (size_t b = 0; b < nb; ++b) {
             HIPCHK(hipEventRecord(events[1 + b], branches[b]));
             HIPCHK(hipStreamWaitEvent(main, events[1 + b], 0));
@@ -167,6 +183,49 @@ struct whir_batch {
         if (multi_tree) merkle_multi_launch(region, (const MerkleSeg*)(region + segs_off), tree_launches, main);
         gather_hashes_launch(region, (const uint64_t*)(region + idx_off), sizes.size(), region + roots_off, main);
         HIPCHK(hipGetLastError());
+        return 0;
+    }
+
+    // record the current form into `graph` / `exec`
+    int capture() {
+        const bool timing = ctx->timing;
+        ctx->timing = false;
+        HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int rc = record(ctx->stream);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        ctx->timing = timing;
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail("hipStreamEndCapture", e);
+        graph = g;
+        HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        return 0;
+    }
+
+    // median of 3 replays (after one warm-up) of the captured graph, HIP events
+    int time_replays(double* ms_out) {
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        std::vector<float> t;
+        hipError_t e = hipSuccess;
+        for (int r = 0; r < 4 && e == hipSuccess; ++r) {
+            (void)hipEventRecord(e0, ctx->stream);
+            e = hipGraphLaunch(exec, ctx->stream);
+            (void)hipEventRecord(e1, ctx->stream);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (r) t.push_back(ms);
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (e != hipSuccess) return fail("batch replay", e);
+        std::sort(t.begin(), t.end());
+        *ms_out = t[t.size() / 2];
         return 0;
     }
 };
@@ -217,7 +276,13 @@ int whir_tuning_import(const uint64_t* rec, size_t words) {
 
 int whir_batch_create(whir_ctx* ctx, size_t count, const uint64_t* sizes, uint32_t log_inv_rate,
                       uint32_t folding_factor, uint32_t num_branches, int autotune, whir_batch** out) {
+    return whir_batch_create_form(ctx, count, sizes, log_inv_rate, folding_factor, num_branches, autotune, -1, out);
+}
+
+int whir_batch_create_form(whir_ctx* ctx, size_t count, const uint64_t* sizes, uint32_t log_inv_rate,
+                           uint32_t folding_factor, uint32_t num_branches, int autotune, int form, whir_batch** out) {
     if (!ctx || !out || !count || !sizes) return bad_arg("null argument");
+    if (form < -1 || form > 1) return bad_arg("batch form must be -1, 0 or 1");
     if (num_branches == 0) num_branches = 1;
     const uint32_t depth = 1u << folding_factor;
     std::unique_ptr<whir_batch> B(new whir_batch());
@@ -275,6 +340,46 @@ int whir_batch_create(whir_ctx* ctx, size_t count, const uint64_t* sizes, uint32
         B->multi_tree = merkle_multi_plan(noff.data(), B->rows.data(), count, segs, B->tree_launches) == 0;
     }
     if (B->multi_tree) B->segs_off = put(segs.size() * sizeof(MerkleSeg));
+    // level-synchronous encodes: BLAKE3 trees built level by level, 16-column
+    // rows (folding factor 4), at least two cosets, 2^10 <= m <= 2^18;
+    // segments in decreasing m (the largest tiles first)
+    std::vector<LdeMultiSeg> lsegs;
+    if (B->multi_tree && depth == 16 && log_inv_rate >= 1) {
+        for (size_t k : order) {
+            const uint64_t m = sizes[k] / depth;
+            if (m >= 1024 && m <= (1ull << 18)) B->level_polys.push_back(k);
+        }
+    }
+    for (size_t k : B->level_polys) {
+        LdeShape s = commit_shape(sizes[k], depth, log_inv_rate);
+        Twiddles tw;
+        if ((rc = ctx->twiddles(log2u(s.n), false, tw))) return rc;
+        LdeMultiSeg g{};
+        g.in = (const uint64_t*)(uintptr_t)B->coeff_off[k];  // offsets for now, rebased below
+        g.out = (uint64_t*)(uintptr_t)B->matrix_off[k];
+        g.leaves = (uint8_t*)(uintptr_t)B->nodes_off[k];
+        const size_t zw = lde_multi_scratch_elems(s.m, s.n);
+        g.z = zw ? (uint64_t*)(uintptr_t)put(zw * 8) : nullptr;
+        g.tw_lo = tw.lo;
+        g.tw_hi = tw.hi;
+        g.lg_m = (uint32_t)log2u(s.m);
+        g.lg_n = (uint32_t)log2u(s.n);
+        B->t1024 = tw.t1024;
+        lsegs.push_back(g);
+    }
+    if (!lsegs.empty()) {
+        B->nlseg = (uint32_t)lsegs.size();
+        B->lsegs_off = put(lsegs.size() * sizeof(LdeMultiSeg));
+    }
+    // the per-polynomial commits left to the branches in that form
+    B->level_branch_polys.assign(nb, {});
+    {
+        std::vector<bool> lev(count, false);
+        for (size_t k : B->level_polys) lev[k] = true;
+        for (size_t b = 0; b < nb; ++b)
+            for (size_t k : B->branch_polys[b])
+                if (!lev[k]) B->level_branch_polys[b].push_back(k);
+    }
     for (size_t b = 0; b < nb; ++b) {
         size_t need = 0;
         for (size_t k : B->branch_polys[b]) {
@@ -293,6 +398,17 @@ int whir_batch_create(whir_ctx* ctx, size_t count, const uint64_t* sizes, uint32
     if (B->multi_tree && !segs.empty())
         HIPCHK(hipMemcpyAsync(B->region + B->segs_off, segs.data(), segs.size() * sizeof(MerkleSeg),
                               hipMemcpyHostToDevice, ctx->stream));
+    if (!lsegs.empty()) {
+        for (auto& g : lsegs) {  // offsets -> device addresses
+            g.in = (const uint64_t*)(B->region + (uintptr_t)g.in);
+            g.out = (uint64_t*)(B->region + (uintptr_t)g.out);
+            g.leaves = B->region + (uintptr_t)g.leaves;
+            if (g.z) g.z = (uint64_t*)(B->region + (uintptr_t)g.z);
+        }
+        if (lde_multi_prepare(lsegs, B->lplan)) return bad_arg("batch shapes rejected by the level-synchronous plan");
+        HIPCHK(hipMemcpyAsync(B->region + B->lsegs_off, lsegs.data(), lsegs.size() * sizeof(LdeMultiSeg),
+                              hipMemcpyHostToDevice, ctx->stream));
+    }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     for (size_t b = 0; b < nb; ++b) {
         hipStream_t st;
@@ -301,18 +417,33 @@ int whir_batch_create(whir_ctx* ctx, size_t count, const uint64_t* sizes, uint32
     }
     B->events.assign(nb + 1, nullptr);
     for (auto& e : B->events) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    // capture
-    const bool timing = ctx->timing;
-    ctx->timing = false;
-    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    rc = B->record(ctx->stream);
-    hipGraph_t graph = nullptr;
-    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
-    ctx->timing = timing;
-    if (rc) return rc;
-    if (e != hipSuccess) return fail("hipStreamEndCapture", e);
-    B->graph = graph;
-    HIPCHK(hipGraphInstantiate(&B->exec, graph, nullptr, nullptr, 0));
+    // capture the level-synchronous form where it applies; with autotune,
+    // also the per-polynomial form, and keep the one whose replays are faster
+    B->use_level = !B->level_polys.empty() && form != 0;
+    if ((rc = B->capture())) return rc;
+    if (autotune && B->use_level && form == -1) {
+        if ((rc = B->time_replays(&B->form_ms[1]))) return rc;
+        hipGraph_t g1 = B->graph;
+        hipGraphExec_t x1 = B->exec;
+        B->graph = nullptr;
+        B->exec = nullptr;
+        B->use_level = false;
+        if ((rc = B->capture()) || (rc = B->time_replays(&B->form_ms[0]))) {
+            (void)hipGraphExecDestroy(x1);
+            (void)hipGraphDestroy(g1);
+            return rc;
+        }
+        if (B->form_ms[1] <= B->form_ms[0]) {
+            (void)hipGraphExecDestroy(B->exec);
+            (void)hipGraphDestroy(B->graph);
+            B->graph = g1;
+            B->exec = x1;
+            B->use_level = true;
+        } else {
+            (void)hipGraphExecDestroy(x1);
+            (void)hipGraphDestroy(g1);
+        }
+    }
     *out = B.release();
     return 0;
 }
@@ -333,6 +464,13 @@ uint8_t* whir_batch_nodes(whir_batch* B, size_t k) {
 }
 size_t whir_batch_rows(whir_batch* B, size_t k) { return (B && k < B->sizes.size()) ? B->rows[k] : 0; }
 size_t whir_batch_device_bytes(whir_batch* B) { return B ? B->total : 0; }
+
+int whir_batch_form(whir_batch* B, double* per_poly_ms, double* level_ms) {
+    if (!B) return -1;
+    if (per_poly_ms) *per_poly_ms = B->form_ms[0];
+    if (level_ms) *level_ms = B->form_ms[1];
+    return B->use_level ? 1 : 0;
+}
 
 int whir_batch_run(whir_batch* B) {
     if (!B) return bad_arg("null batch");

@@ -69,6 +69,32 @@ int lde_splits(const LdeShape& s, uint32_t* lg_m2, int cap);
 int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* scratch,
                const Twiddles& tw, hipStream_t st, uint8_t* leaves = nullptr, bool* fused = nullptr);
 
+// Encodes of many 16-column base polynomials (folding factor 4) with their
+// BLAKE3 row leaves, level-synchronous: one first-pass launch over every
+// polynomial with m >= 2048 coefficients per column, then one second-pass
+// launch over all of them (M2 = 1024 rows per tile, m = M1 * 1024,
+// 1024 <= m <= 2^18, n = E * m with 2 <= E, n <= 2^26).  A segment is one
+// polynomial; lde_multi_prepare fills the tile ranges (segments must be in
+// decreasing m) and returns the launch sizes.
+struct LdeMultiSeg {
+    const uint64_t* in;      // coefficients: column c < 16 at in + c * m
+    uint64_t* out;           // codeword, row-major n x 16
+    uint8_t* leaves;         // n x 32 bytes (the tree's leaf level)
+    uint64_t* z;             // first-pass scratch, E * 16 * m u64 (unused when m == 1024)
+    const uint64_t* tw_lo;   // the size-n twiddle tables (Twiddles::lo, ::hi)
+    const uint64_t* tw_hi;
+    uint32_t lg_m, lg_n;
+    uint32_t tile_a0, tile_b0;  // the segment's first tile of each launch
+};
+struct LdeMultiPlan {
+    uint32_t tiles_a = 0, tiles_b = 0;
+};
+int lde_multi_prepare(std::vector<LdeMultiSeg>& segs, LdeMultiPlan& plan);
+// u64 of first-pass scratch a segment of this shape needs
+size_t lde_multi_scratch_elems(uint64_t m, uint64_t n);
+void lde_multi_launch(const LdeMultiSeg* d_segs, uint32_t nseg, const LdeMultiPlan& plan, const uint64_t* t1024,
+                      hipStream_t st);
+
 // leaves[i] = BLAKE3(row i), row = row_u64 little-endian u64 words.
 void leaf_hash_launch(const uint64_t* matrix, uint64_t rows, uint32_t row_u64, uint8_t* leaves, hipStream_t st);
 // out[i] = BLAKE3(in[i*size .. (i+1)*size]) for arbitrary byte size.

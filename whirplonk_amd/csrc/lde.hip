@@ -999,6 +999,229 @@ bool make_plan(const LdeShape& s, Plan& p) {
     return true;
 }
 
+// ------------------------------------------------- batches (LdeMultiSeg)
+// Level-synchronous encodes of many 16-column polynomials: every first-pass
+// tile of every polynomial in one launch, then every second-pass tile in one
+// launch.  The same transform as lde_pass_a_regs + lde_pass_b_rows2 (M2 =
+// 1024, inter-pass twiddle in the second pass, all E cosets of a polynomial
+// in one scratch region), with the tile's polynomial looked up in a segment
+// table: a workgroup walks global tiles t, t + gridDim.x, ... and finds the
+// segment of t by binary search over the segments' first tiles (uniform, so
+// scalar loads).  Both kernels run two 512-lane workgroups per CU (at most
+// 128 VGPRs) and keep no register prefetch: the other workgroup computes
+// while one waits for its loads.
+
+__device__ __forceinline__ uint64_t twid_seg(const LdeMultiSeg& g, uint64_t e) {
+    uint64_t x = ((__attribute__((address_space(1))) const uint64_t*)g.tw_lo)[e & 2047];
+    uint64_t h = e >> 11;
+    return h ? gl::mul(x, ((__attribute__((address_space(1))) const uint64_t*)g.tw_hi)[h]) : x;
+}
+
+// The segment table's pointers are loaded from memory, so the compiler
+// cannot tell they address global memory and would use flat instructions
+// (which also count against the LDS wait counter); these casts state it.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+template <typename T>
+__device__ __forceinline__ gptr<T> global_ptr(T* p) {
+    return (gptr<T>)p;
+}
+
+// the last segment whose first tile (of pass B, or of pass A) is <= t
+template <bool PASS_B>
+__device__ __forceinline__ uint32_t seg_of(const LdeMultiSeg* segs, uint32_t nseg, uint32_t t) {
+    uint32_t lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        const uint32_t first = PASS_B ? segs[mid].tile_b0 : segs[mid].tile_a0;
+        if (first <= t)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// first-pass tiles of one polynomial: columns c < 16 split into runs of TA
+// consecutive j2 (TA * R2 = 512 lanes)
+template <int M1>
+constexpr uint32_t multi_a_tiles() {
+    return 16 * (1024 / (THREADS / dft::Split<M1>::R2));
+}
+template <int M1>
+constexpr size_t multi_a_lds_words() {
+    return (size_t)(THREADS / dft::Split<M1>::R2) * (dft::Split<M1>::PADDED + 1) + M1;
+}
+constexpr size_t max_sz(size_t a, size_t b) { return a > b ? a : b; }
+constexpr size_t MULTI_A_LDS =
+    8 * max_sz(max_sz(max_sz(multi_a_lds_words<2>(), multi_a_lds_words<4>()),
+                      max_sz(multi_a_lds_words<8>(), multi_a_lds_words<16>())),
+               max_sz(max_sz(multi_a_lds_words<32>(), multi_a_lds_words<64>()),
+                      max_sz(multi_a_lds_words<128>(), multi_a_lds_words<256>())));
+
+// One first-pass tile (column c, j2 in [J, J + TA)) for all E cosets: lane
+// (t, n2) holds x[t][R2 n1 + n2] in registers across the cosets; per coset:
+// twist w_L^{s j1} (LDS table), DFT_R1 in registers, t1024 twiddles, DFT_R2
+// from LDS, Z_s[k][c][j2] written (16 * 1024 values per k).
+template <int M1>
+__device__ __forceinline__ void multi_a_tile(const LdeMultiSeg& g, uint32_t lt, const uint64_t* __restrict__ t1024,
+                                             uint64_t* lds) {
+    using S = dft::Split<M1>;
+    constexpr int R1 = S::R1, R2 = S::R2, PAD = S::PADDED + 1, TA = THREADS / R2;
+    constexpr uint32_t M2 = 1024, NJ = M2 / TA, ZROW = 16 * M2;
+    static_assert(R1 <= R2 && TA * R2 == THREADS && TA <= (int)M2, "first-pass tile shape");
+    uint64_t* twist = lds + TA * PAD;
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));  // lane indices re-derived per tile (see lde_pass_b_rows)
+    const uint32_t c = lt / NJ, J = (lt % NJ) * TA;
+    const uint32_t t = tid % TA, n2 = tid / TA;
+    const uint64_t m = 1ull << g.lg_m, nmask = (1ull << g.lg_n) - 1;
+    const uint32_t E = 1u << (g.lg_n - g.lg_m);
+    const gptr<const uint64_t> col = global_ptr(g.in + c * m + J);  // uniform base, 32-bit lane offsets
+    const uint32_t off = t + M2 * n2;
+    // the tile's inputs stay in registers across the cosets, except at the
+    // largest size, where holding them would spill under the 128-VGPR budget
+    // (its later cosets re-read the tile, from the cache)
+    constexpr bool KEEP = M1 < 256;
+    uint64_t pre[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) pre[n1] = col[off + M2 * R2 * n1];
+    for (uint32_t s = 0; s < E; ++s) {
+        if (!KEEP && s != 0) {
+            uint32_t tid2 = threadIdx.x;
+            asm volatile("" : "+v"(tid2));  // no addresses kept from the first load
+            const uint32_t off2 = tid2 % TA + M2 * (tid2 / TA);
+#pragma unroll
+            for (int n1 = 0; n1 < R1; ++n1) pre[n1] = col[off2 + M2 * R2 * n1];
+        }
+        if (s != 0) {
+            for (uint32_t j1 = tid; j1 < (uint32_t)M1; j1 += THREADS)
+                twist[j1] = twid_seg(g, ((uint64_t)s * j1 * M2) & nmask);
+            __syncthreads();
+        }
+        uint64_t v[R1];
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) v[n1] = s != 0 ? gl::mul(pre[n1], twist[R2 * n1 + n2]) : pre[n1];
+        dft::dft_reg<R1, false>(v);
+        if constexpr (R2 > 1) {
+#pragma unroll
+            for (int k1 = 1; k1 < R1; ++k1) v[k1] = gl::mul(v[k1], t1024[((n2 * k1) * (1024 / M1)) & 1023]);
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) lds[t * PAD + dft::pidx<M1>(R2 * k1 + n2)] = v[k1];
+        __syncthreads();
+        const gptr<uint64_t> z = global_ptr(g.z + (uint64_t)s * (16 * m) + (uint64_t)c * M2 + J);  // uniform
+        for (uint32_t w = tid; w < (uint32_t)(TA * R1); w += THREADS) {
+            const uint32_t t2 = w % TA, k1 = w / TA;
+            uint64_t u[R2];
+            dft::lds_dft_step2<M1, false>(lds + t2 * PAD, k1, u);
+            const uint32_t zo = k1 * ZROW + t2;  // < 2^28: 32-bit lane offsets
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) z[zo + (uint32_t)(R1 * k2) * ZROW] = u[k2];
+        }
+        __syncthreads();  // the next coset (or tile) rewrites the LDS tile and the twist table
+    }
+}
+
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4, 4))) void lde_multi_a(
+    const LdeMultiSeg* __restrict__ segs, uint32_t nseg, uint32_t ntiles, const uint64_t* __restrict__ t1024) {
+    extern __shared__ uint64_t lds[];
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const LdeMultiSeg g = segs[seg_of<false>(segs, nseg, t)];
+        const uint32_t lt = t - g.tile_a0;
+        switch (g.lg_m) {
+            case 11: multi_a_tile<2>(g, lt, t1024, lds); break;
+            case 12: multi_a_tile<4>(g, lt, t1024, lds); break;
+            case 13: multi_a_tile<8>(g, lt, t1024, lds); break;
+            case 14: multi_a_tile<16>(g, lt, t1024, lds); break;
+            case 15: multi_a_tile<32>(g, lt, t1024, lds); break;
+            case 16: multi_a_tile<64>(g, lt, t1024, lds); break;
+            case 17: multi_a_tile<128>(g, lt, t1024, lds); break;
+            case 18: multi_a_tile<256>(g, lt, t1024, lds); break;
+            default: break;  // lde_multi_prepare admits no other size
+        }
+    }
+}
+
+// Second pass of every polynomial: tile = (coset s, first-pass output index
+// k), 16 columns x 1024 rows i = i1 + L r (i1 = s + E k, L = E M1), as
+// lde_pass_b_rows2 (two-round transposition, rows staged in two halves for
+// the BLAKE3 leaves and the 16-byte stores).  A polynomial with m = 1024 has
+// no first pass: its tiles read the coefficients themselves (k = 0, L = E,
+// the inter-pass twiddle w_n^{s j2} being the coset twist).
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(4, 4))) void lde_multi_b(
+    const LdeMultiSeg* __restrict__ segs, uint32_t nseg, uint32_t ntiles, const uint64_t* __restrict__ t1024) {
+    constexpr int M2 = 1024, R = 32;
+    extern __shared__ uint64_t lds[];
+    uint64_t* buf = lds;              // exchange, then row stage
+    uint64_t* twt = lds + XCH_WORDS;  // inter-pass twiddles of the tile
+    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        uint32_t tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));  // lane indices re-derived per tile (see lde_pass_b_rows)
+        const LdeMultiSeg g = segs[seg_of<true>(segs, nseg, tile)];
+        const uint32_t lt = tile - g.tile_b0;
+        const uint32_t lgM1 = g.lg_m - 10, lgE = g.lg_n - g.lg_m, lgL = lgE + lgM1;
+        const uint32_t s = lt >> lgM1, k = lt & ((1u << lgM1) - 1);
+        const uint64_t i1 = s + ((uint64_t)k << lgE);
+        const uint64_t m = 1ull << g.lg_m, nmask = (1ull << g.lg_n) - 1;
+        const uint32_t c = (tid >> 4) & 15, hw = tid >> 8, n2 = (tid & 15) | (hw << 4);
+        const gptr<const uint64_t> src =
+            global_ptr(lgM1 ? g.z + (uint64_t)s * (16 * m) + (uint64_t)k * (16 * M2) + (uint64_t)c * M2
+                            : g.in + (uint64_t)c * m);
+        uint64_t v[R];
+#pragma unroll
+        for (int n1 = 0; n1 < R; ++n1) v[n1] = src[n2 + R * n1];
+        // the previous tile's last reads of twt and buf ended at its final barrier
+        for (uint32_t j2 = tid; j2 < (uint32_t)M2; j2 += THREADS) twt[j2] = twid_seg(g, (i1 * j2) & nmask);
+        __syncthreads();
+#pragma unroll
+        for (int n1 = 0; n1 < R; ++n1) v[n1] = gl::mul(v[n1], twt[n2 + R * n1]);
+        dft::dft_reg<R, false>(v);
+#pragma unroll
+        for (int k1 = 1; k1 < R; ++k1) v[k1] = gl::mul(v[k1], t1024[(n2 * k1) & 1023]);
+        const uint32_t cb = tid % 16, k1r = tid / 16;
+        uint64_t u[R];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (hw == (uint32_t)h) {  // uniform per wave
+#pragma unroll
+                for (int k1 = 0; k1 < R; ++k1) buf[xch_idx(c, k1, n2 & 15)] = v[k1];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 16; ++q) u[16 * h + q] = buf[xch_idx(cb, k1r, q)];
+            __syncthreads();
+        }
+        dft::dft_reg<R, false>(u);  // u[k2] = row k1r + 32 k2 of column cb
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int kk = 0; kk < 16; ++kk) buf[stage_idx(k1r + R * kk, cb)] = u[16 * h + kk];
+            __syncthreads();
+            {
+                const uint32_t r = tid;  // local row
+                uint32_t cv[8];
+                row_leaf<HASH_BLAKE3>([&](int cc) { return buf[stage_idx(r, cc)]; }, cv);
+                const uint64_t i = i1 + ((uint64_t)(512 * h + r) << lgL);
+                const gptr<u32x4> dst = global_ptr(reinterpret_cast<u32x4*>(g.leaves + 32 * i));
+                dst[0] = u32x4{cv[0], cv[1], cv[2], cv[3]};
+                dst[1] = u32x4{cv[4], cv[5], cv[6], cv[7]};
+            }
+#pragma unroll
+            for (int q2 = 0; q2 < 512 * 16 / (2 * THREADS); ++q2) {
+                const uint32_t q = tid + q2 * THREADS;
+                const uint32_t r = q / 8, cp = 2 * (q % 8);
+                const uint64_t x0 = buf[stage_idx(r, cp)], x1 = buf[stage_idx(r, cp + 1)];
+                const uint64_t i = i1 + ((uint64_t)(512 * h + r) << lgL);
+                *global_ptr(reinterpret_cast<u64x2*>(g.out + i * 16 + cp)) = u64x2{x0, x1};
+            }
+            __syncthreads();  // the next half (or tile) rewrites the stage
+        }
+    }
+}
+
 }  // namespace
 
 int lde_splits(const LdeShape& s, uint32_t* lg_m2, int cap) {
@@ -1235,6 +1458,69 @@ int lde_launch(const LdeShape& s, const uint64_t* in, uint64_t* out, uint64_t* s
         }
     }
     return 0;
+}
+
+size_t lde_multi_scratch_elems(uint64_t m, uint64_t n) { return m > 1024 ? (size_t)(16 * n) : 0; }
+
+namespace {
+// kernel attributes and occupancy, queried once on the host (before any
+// graph capture: lde_multi_prepare runs when a batch is planned)
+struct MultiGrid {
+    uint32_t per_cu_a = 1, per_cu_b = 1;
+};
+const MultiGrid& multi_grid() {
+    static MultiGrid g;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        (void)hipFuncSetAttribute((const void*)lde_multi_a, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void*)lde_multi_b, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        g.per_cu_a = (uint32_t)std::min(2, resident_blocks((const void*)lde_multi_a, THREADS, MULTI_A_LDS));
+        g.per_cu_b = (uint32_t)std::min(2, resident_blocks((const void*)lde_multi_b, THREADS, ROWS2_LDS));
+    });
+    return g;
+}
+}  // namespace
+
+int lde_multi_prepare(std::vector<LdeMultiSeg>& segs, LdeMultiPlan& plan) {
+    (void)multi_grid();
+    uint64_t ta = 0, tb = 0;
+    for (size_t i = 0; i < segs.size(); ++i) {
+        LdeMultiSeg& g = segs[i];
+        if (g.lg_m < 10 || g.lg_m > 18 || g.lg_n <= g.lg_m || g.lg_n > 26) return 1;
+        if (i && g.lg_m > segs[i - 1].lg_m) return 1;  // decreasing m
+        if (!g.in || !g.out || !g.leaves || !g.tw_lo || !g.tw_hi || (g.lg_m > 10 && !g.z)) return 1;
+        g.tile_a0 = (uint32_t)ta;
+        g.tile_b0 = (uint32_t)tb;
+        static constexpr uint32_t per_a[9] = {0,
+                                              multi_a_tiles<2>(),
+                                              multi_a_tiles<4>(),
+                                              multi_a_tiles<8>(),
+                                              multi_a_tiles<16>(),
+                                              multi_a_tiles<32>(),
+                                              multi_a_tiles<64>(),
+                                              multi_a_tiles<128>(),
+                                              multi_a_tiles<256>()};
+        ta += per_a[g.lg_m - 10];
+        tb += 1ull << (g.lg_n - 10);  // E cosets x M1 first-pass outputs = n / 1024
+    }
+    if (ta >= (1ull << 31) || tb >= (1ull << 31)) return 1;
+    plan.tiles_a = (uint32_t)ta;
+    plan.tiles_b = (uint32_t)tb;
+    return 0;
+}
+
+void lde_multi_launch(const LdeMultiSeg* d_segs, uint32_t nseg, const LdeMultiPlan& plan, const uint64_t* t1024,
+                      hipStream_t st) {
+    const MultiGrid& mg = multi_grid();
+    if (!nseg) return;
+    if (plan.tiles_a) {
+        const uint32_t grid = std::min<uint32_t>(plan.tiles_a, mg.per_cu_a * (uint32_t)cu_count());
+        hipLaunchKernelGGL(lde_multi_a, dim3(grid), dim3(THREADS), MULTI_A_LDS, st, d_segs, nseg, plan.tiles_a, t1024);
+    }
+    if (plan.tiles_b) {
+        const uint32_t grid = std::min<uint32_t>(plan.tiles_b, mg.per_cu_b * (uint32_t)cu_count());
+        hipLaunchKernelGGL(lde_multi_b, dim3(grid), dim3(THREADS), ROWS2_LDS, st, d_segs, nseg, plan.tiles_b, t1024);
+    }
 }
 
 namespace {
