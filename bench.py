@@ -443,6 +443,8 @@ def served_leg(n_req: int, offered: float, slots: int, budget_gb: float, cpu: bo
         st1 = srv.stats()
         pow_launches = st1["pow_launches"] - st0["pow_launches"]
         pow_searches = st1["pow_searches"] - st0["pow_searches"]
+        enc_launches = st1["encode_launches"] - st0["encode_launches"]
+        encodes = st1["encodes"] - st0["encodes"]
         if offered <= 0:  # 0.8x the measured plain-path capacity
             offered = round(0.8 * n_req / plain_s, 1)
         same_cap = all(res[i][:2] == ref[i] for i in range(n_req))
@@ -492,6 +494,7 @@ def served_leg(n_req: int, offered: float, slots: int, budget_gb: float, cpu: bo
         "heavy_p50_ms": pct(lat_h, 0.5), "heavy_p99_ms": pct(lat_h, 0.99),
         "peak_hbm_bytes": st["peak"],
         "pow_searches_per_launch": round(pow_searches / max(pow_launches, 1), 2),
+        "encodes_per_launch": round(encodes / max(enc_launches, 1), 2),
         "bit_exact_vs_plain": bool(same_cap and same_p and same_h),
         "bit_exact_vs_cpu": None,
     })

@@ -475,8 +475,11 @@ int whir_host_free(void* ptr);
  * queue wait, service time and end-to-end latency in ms. */
 int whir_server_wait(whir_server* server, uint64_t id, uint8_t* narg, size_t narg_cap, size_t* narg_len,
                      uint8_t* hints, size_t hints_cap, size_t* hints_len, double* out_ms);
-/* out[9]: completed, running, queued, arena bytes in use, peak bytes in use,
- * budget, slots, batched proof-of-work launches, searches they ran. */
+/* out[11]: completed, running, queued, arena bytes in use, peak bytes in use,
+ * budget, slots, batched proof-of-work launches, searches they ran, batched
+ * encode launches (initial commits of BLAKE3 requests with 2^14 to 2^22
+ * coefficients at folding factor 4, several requests' encodes in one
+ * level-synchronous launch pair), encodes they ran. */
 int whir_server_stats(whir_server* server, uint64_t* out);
 /* Test hook for the serving loop's error path (no counterpart in the
  * reference, whose GPU layer has no serving loop): request `id` (one that

@@ -60,8 +60,11 @@ def test_mix_equals_alone(slots):
         st = srv.stats()
         assert st["completed"] == len(inputs)
         assert st["peak"] <= budget and st["in_use"] == 0
-        # every proof-of-work search went through the batcher
+        # every proof-of-work search went through the batcher, and the
+        # initial commits' encodes of the 2^14 requests through the other
         assert st["pow_searches"] >= st["pow_launches"] > 0
+        assert st["encodes"] == sizes.count(14) and st["encodes"] >= st["encode_launches"]
+        assert (st["encode_launches"] > 0) == (14 in sizes)
 
 
 def test_served_vs_reference():
@@ -134,6 +137,10 @@ def test_served_workload_mix():
         got = [srv.wait(i)[:2] for i in ids]
         st = srv.stats()
     assert st["completed"] == len(sizes) and st["in_use"] == 0
+    # the initial commits of every request up to 2^22 coefficients went
+    # through the encode batcher
+    print("encodes", st["encodes"], "in", st["encode_launches"], "launches", flush=True)
+    assert st["encodes"] == sum(1 for nv in sizes if nv <= 22) and st["encode_launches"] > 0
     for i in range(len(sizes)):
         assert got[i] == alone[i][0], (i, sizes[i])
     if oracle.ref() is None:

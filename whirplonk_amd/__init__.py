@@ -804,7 +804,8 @@ class Server:
                                        ctypes.byref(nh), ms))
         return a[:na.value].tobytes(), b[:nh.value].tobytes(), (ms[0], ms[1], ms[2])
 
-    STATS = ("completed", "running", "queued", "in_use", "peak", "budget", "slots", "pow_launches", "pow_searches")
+    STATS = ("completed", "running", "queued", "in_use", "peak", "budget", "slots", "pow_launches", "pow_searches",
+             "encode_launches", "encodes")
 
     def stats(self) -> dict:
         out = np.zeros(len(self.STATS), dtype=np.uint64)

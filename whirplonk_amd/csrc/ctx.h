@@ -179,6 +179,12 @@ struct whir_ctx {
     // its requests): the hook returns the nonce for (challenge, threshold).
     int (*pow_hook)(void* arg, const uint8_t* ch, uint64_t threshold, uint64_t* nonce) = nullptr;
     void* pow_hook_arg = nullptr;
+    // ... and the encodes of its requests' initial commits (BLAKE3, 16-column
+    // rows, 2^10 <= m <= 2^18): the hook encodes `seg` (leaves included) with
+    // the encodes of other requests, after the work enqueued so far on
+    // `stream`, and returns once `stream` waits for the result.
+    int (*encode_hook)(void* arg, const whir_gpu::LdeMultiSeg& seg, const uint64_t* t1024, hipStream_t st) = nullptr;
+    void* encode_hook_arg = nullptr;
     // waits for all work enqueued on `stream` (through the mapped signal word)
     whir_gpu::StreamSignal sig;
     // Failure injection for tests of the callers' error paths: when nonzero,

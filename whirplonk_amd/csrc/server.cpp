@@ -11,7 +11,9 @@
 // block across all WHIR rounds.  Same-stage work is batched across requests
 // where it has no data dependence on a request's device state: every
 // proof-of-work search of the running requests goes through one batcher that
-// launches the searches pending at that moment as one kernel.  Results are
+// launches the searches pending at that moment as one kernel, and the encodes
+// of their initial commits through another that runs the encodes pending at
+// that moment as one level-synchronous launch pair.  Results are
 // byte-identical to the plain single-request path (the same prover code runs
 // on the same inputs; a search's nonce does not depend on its batch).
 #include "../../include/whir_gpu.h"
@@ -238,6 +240,133 @@ private:
     uint64_t launches_ = 0, jobs_ = 0;
 };
 
+// The encodes of the running requests' initial commits, batched: a worker
+// that reaches its commit waits (on the host) for its stream's earlier work
+// (the coefficient upload), queues the encode (one LdeMultiSeg: its
+// coefficients, codeword, leaf level and scratch in its arena block) and, if
+// the launch path is free, leads: it runs every queued encode (up to
+// MAX_SEGS) as one level-synchronous launch pair on the batcher's stream and
+// waits for it, while encodes that arrive meanwhile queue for the next
+// launch.  Streams never wait on one another on the device (a stream waiting
+// on another's event would hold its hardware queue, which other slots'
+// streams share).  As with the searches, no request waits for a batch to
+// fill, and an encode's output does not depend on its batch.
+class EncodeBatcher {
+public:
+    static constexpr uint32_t MAX_SEGS = 16;
+
+    int init(int device) {
+        HIPCHK(hipSetDevice(device));
+        HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+        HIPCHK(hipMalloc((void**)&table_, MAX_SEGS * sizeof(LdeMultiSeg)));
+        HIPCHK(hipHostMalloc((void**)&staging_, MAX_SEGS * sizeof(LdeMultiSeg), hipHostMallocDefault));
+        return signal_init(sig_);
+    }
+    ~EncodeBatcher() {
+        if (st_) (void)hipStreamSynchronize(st_);
+        if (table_) (void)hipFree(table_);
+        if (staging_) (void)hipHostFree(staging_);
+        signal_free(sig_);
+        if (st_) (void)hipStreamDestroy(st_);
+    }
+    static int hook(void* self, const LdeMultiSeg& seg, const uint64_t* t1024, hipStream_t st) {
+        return static_cast<EncodeBatcher*>(self)->encode(seg, t1024, st);
+    }
+    void stats(uint64_t* launches, uint64_t* jobs) {
+        std::lock_guard<std::mutex> lk(mu_);
+        *launches = launches_;
+        *jobs = jobs_;
+    }
+
+private:
+    struct Waiter {
+        LdeMultiSeg seg;
+        const uint64_t* t1024;
+        bool done = false;
+        int rc = 0;
+        std::string error;
+    };
+
+    // one completion word per worker thread, for its own stream
+    static int thread_signal(StreamSignal** out) {
+        struct Holder {
+            StreamSignal s;
+            ~Holder() { signal_free(s); }
+        };
+        thread_local Holder h;
+        if (int rc = signal_init(h.s)) return rc;
+        *out = &h.s;
+        return 0;
+    }
+
+    // runs on the leader's thread with the path held: launch and wait
+    int lead(std::vector<Waiter*>& batch) {
+        std::stable_sort(batch.begin(), batch.end(),
+                         [](const Waiter* a, const Waiter* b) { return a->seg.lg_m > b->seg.lg_m; });
+        std::vector<LdeMultiSeg> segs;
+        for (Waiter* w : batch) segs.push_back(w->seg);
+        LdeMultiPlan plan;
+        if (lde_multi_prepare(segs, plan)) return bad_arg("encode batch rejected by the level-synchronous plan");
+        std::memcpy(staging_, segs.data(), segs.size() * sizeof(LdeMultiSeg));  // the last launch is done
+        HIPCHK(hipMemcpyAsync(table_, staging_, segs.size() * sizeof(LdeMultiSeg), hipMemcpyHostToDevice, st_));
+        lde_multi_launch(table_, (uint32_t)segs.size(), plan, batch[0]->t1024, st_);
+        HIPCHK(hipGetLastError());
+        return signal_wait(sig_, st_);
+    }
+
+    int encode(const LdeMultiSeg& seg, const uint64_t* t1024, hipStream_t st) {
+        StreamSignal* own = nullptr;
+        if (int rc = thread_signal(&own)) return rc;
+        if (int rc = signal_wait(*own, st)) return rc;  // the coefficients are on the device
+        Waiter w;
+        w.seg = seg;
+        w.t1024 = t1024;
+        std::unique_lock<std::mutex> lk(mu_);
+        queue_.push_back(&w);
+        while (!w.done) {
+            if (busy_ || queue_.empty()) {
+                cv_.wait(lk);
+                continue;
+            }
+            busy_ = true;
+            std::vector<Waiter*> batch;
+            while (!queue_.empty() && batch.size() < MAX_SEGS) {
+                batch.push_back(queue_.front());
+                queue_.pop_front();
+            }
+            lk.unlock();
+            int rc = lead(batch);
+            std::string err = rc ? last_error() : std::string();
+            if (rc && hipStreamSynchronize(st_) != hipSuccess && err.empty()) err = "encode batch stream failed";
+            lk.lock();
+            busy_ = false;
+            ++launches_;
+            jobs_ += batch.size();
+            for (Waiter* x : batch) {
+                x->rc = rc;
+                x->error = err;
+                x->done = true;
+            }
+            cv_.notify_all();
+        }
+        if (w.rc) {
+            last_error() = w.error;
+            return w.rc;
+        }
+        return 0;
+    }
+
+    hipStream_t st_ = nullptr;
+    LdeMultiSeg* table_ = nullptr;    // device copy of the launch's segments
+    LdeMultiSeg* staging_ = nullptr;  // pinned source of that copy
+    StreamSignal sig_;
+    bool busy_ = false;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::deque<Waiter*> queue_;
+    uint64_t launches_ = 0, jobs_ = 0;
+};
+
 struct Slot {
     whir_ctx* ctx = nullptr;
     uint8_t* pinned = nullptr;
@@ -261,6 +390,7 @@ struct whir_server {
     std::map<uint64_t, std::unique_ptr<Request>> reqs;
     std::map<uint64_t, uint32_t> inject;  // request id -> device wait that fails
     PowBatcher pow;
+    EncodeBatcher enc;
     uint64_t next_id = 1;
     size_t completed = 0, running = 0;
     bool stop = false;
@@ -419,7 +549,8 @@ int whir_server_create(int device, const whir_params* params, uint32_t num_slots
     S->budget = (hbm_budget_bytes + 255) & ~size_t{255};
     HIPCHK(hipMalloc((void**)&S->region, S->budget));
     S->arena.init(S->region, S->budget);
-    if ((rc = S->pow.init(device, params->hash_id == WHIR_HASH_SHA2 ? HASH_SHA2 : HASH_BLAKE3))) {
+    if ((rc = S->pow.init(device, params->hash_id == WHIR_HASH_SHA2 ? HASH_SHA2 : HASH_BLAKE3)) ||
+        (rc = S->enc.init(device))) {
         (void)hipFree(S->region);
         return rc;
     }
@@ -432,6 +563,8 @@ int whir_server_create(int device, const whir_params* params, uint32_t num_slots
         }
         sl->ctx->pow_hook = &PowBatcher::hook;
         sl->ctx->pow_hook_arg = &S->pow;
+        sl->ctx->encode_hook = &EncodeBatcher::hook;
+        sl->ctx->encode_hook_arg = &S->enc;
         S->slots.push_back(std::move(sl));
     }
     for (auto& sl : S->slots) sl->th = std::thread(&whir_server::worker, S.get(), sl.get());
@@ -545,6 +678,7 @@ int whir_server_stats(whir_server* S, uint64_t* out) {
     out[5] = S->budget;
     out[6] = S->slots.size();
     S->pow.stats(&out[7], &out[8]);
+    S->enc.stats(&out[9], &out[10]);
     return 0;
 }
 

@@ -349,10 +349,30 @@ int whir_gpu::commit_device(whir_ctx* ctx, const uint64_t* d_coeffs, size_t num_
     const bool sha2 = ctx->hash == HASH_SHA2;
     if ((rc = ctx->mark(0))) return rc;
     // the row leaves go straight into the tree's first nodes, hashed inside
-    // the encode's last pass where the shape allows it
-    if ((rc = rs_encode_any(ctx, d_coeffs, num_polys, poly_size, cw, depth, d_matrix, ext, d_nodes, &fused,
-                            ctx->hash)))
+    // the encode's last pass where the shape allows it; a serving loop may
+    // take over the encode of a base commit to batch it with other requests'
+    const uint64_t m = depth ? poly_size / depth : 0;
+    const bool hookable = ctx->encode_hook && !ext && num_polys == 1 && depth == 16 && !sha2 && m >= 1024 &&
+                          m <= (1ull << 18) && is_pow2(m) && is_pow2(cw) && cw >= 2 * m && cw <= (1ull << 26) &&
+                          ctx->ext_scratch && ctx->ext_scratch_bytes >= lde_multi_scratch_elems(m, cw) * 8;
+    if (hookable) {
+        Twiddles tw;
+        if ((rc = ctx->twiddles(log2u(cw), false, tw))) return rc;
+        LdeMultiSeg g{};
+        g.in = d_coeffs;
+        g.out = d_matrix;
+        g.leaves = d_nodes;
+        g.z = (uint64_t*)ctx->ext_scratch;
+        g.tw_lo = tw.lo;
+        g.tw_hi = tw.hi;
+        g.lg_m = (uint32_t)log2u(m);
+        g.lg_n = (uint32_t)log2u(cw);
+        if ((rc = ctx->encode_hook(ctx->encode_hook_arg, g, tw.t1024, ctx->stream))) return rc;
+        fused = true;
+    } else if ((rc = rs_encode_any(ctx, d_coeffs, num_polys, poly_size, cw, depth, d_matrix, ext, d_nodes, &fused,
+                                   ctx->hash))) {
         return rc;
+    }
     if ((rc = ctx->mark(1))) return rc;
     size_t row_u64 = num_polys * depth * (ext ? 3 : 1);
     if (!fused && sha2)
