@@ -153,3 +153,29 @@ def test_whir_profile_csv_rows():
     env.pop("WHIR_PROFILE")
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120, env=env, cwd=root)
     assert r.returncode == 0 and "mode,phase,size,stage,time_ms" not in r.stderr
+
+
+@pytest.mark.parametrize("nbytes", [(4 << 20) - 8, 4 << 20, (16 << 20) + 8, (37 << 20) + 24, 96 << 20])
+def test_staged_host_copies(nbytes):
+    """Copies between pageable host memory and the device go through two
+    pinned staging chunks (16 MiB) above 4 MiB: sizes below, at and between
+    the chunk boundaries come back unchanged, from pageable and from pinned
+    host memory, and a device-to-device check sees the uploaded bytes."""
+    n = nbytes // 8
+    src, _ = W.lcg_uniform(nbytes, n)
+    print("staged copy", nbytes, flush=True)
+    with W.Context() as ctx:
+        d = ctx.upload(src)
+        assert np.array_equal(ctx.download(d, n), src)
+        # the encode-to-bytes drop-in: host -> device -> host through the staged paths
+        out = np.empty(nbytes, dtype=np.uint8)
+        W._check(W.load().whir_gpu_encode_to_bytes(W._p64(src), W._p8(out), n))
+        assert np.array_equal(out.view(np.uint64), src)
+        pin = W.Pinned(n)
+        pin.array[:] = src[::-1]
+        W._check(W.load().whir_memcpy_h2d(ctx.h, d, pin.ptr, nbytes))
+        pin.array[:] = 0
+        W._check(W.load().whir_memcpy_d2h(ctx.h, pin.ptr, d, nbytes))
+        assert np.array_equal(pin.array, src[::-1])
+        assert np.array_equal(ctx.download(d, n), src[::-1])
+        ctx.free(d)

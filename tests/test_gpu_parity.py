@@ -112,7 +112,9 @@ def test_rs_encode(orc, shape):
 @pytest.mark.parametrize("shape", [(1, 64, 256, 4), (2, 256, 1024, 16), (1, 16, 2048, 16), (3, 1024, 512, 4),
                                    # the last rounds' shapes: m = 4, 16, 32, 64 per column, 48 base columns
                                    (1, 64, 1 << 12, 16), (1, 256, 1 << 14, 16), (1, 512, 1 << 12, 16),
-                                   (1, 1024, 1 << 13, 16)])
+                                   (1, 1024, 1 << 13, 16),
+                                   # 24 MiB AoS codeword: split into its components out of two staging chunks
+                                   (1, 1 << 14, 1 << 16, 16)])
 def test_rs_encode_ext3(orc, shape):
     num_polys, poly_size, cw, depth = shape
     x = rand(77 + sum(shape), 3 * num_polys * poly_size)

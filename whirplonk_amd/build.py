@@ -20,7 +20,8 @@ LIB = os.path.join(OUT, "libwhir_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["lde.hip", "merkle.hip", "sha256.hip", "sumcheck.hip", "prove.hip", "whir_gpu.cpp", "prover.cpp", "server.cpp", "batch.cpp"]
+SOURCES = ["lde.hip", "merkle.hip", "sha256.hip", "sumcheck.hip", "prove.hip", "whir_gpu.cpp", "prover.cpp", "server.cpp",
+           "batch.cpp", "hostcopy.cpp"]
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

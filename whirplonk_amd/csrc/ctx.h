@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <cstring>
@@ -229,6 +230,12 @@ struct whir_ctx {
         return 0;
     }
 
+    // Two pinned chunks for copies to and from pageable host memory
+    // (whir_gpu::copy_h2d / copy_d2h); each chunk's last DMA is recorded in
+    // its event before the host reuses the chunk.
+    void* stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+
     // Grow-only mapped staging for openings: the gather kernels read the
     // indices from it and write the rows and hashes into it.
     void* hio = nullptr;
@@ -320,6 +327,20 @@ struct whir_ctx {
 };
 
 namespace whir_gpu {
+
+// Host <-> device copies on ctx's stream for the host-pointer entry points.
+// Pageable host memory goes through the context's pinned staging chunks (DMA
+// of one chunk overlapping the host copy of the other); pinned memory and
+// small copies take one DMA.  copy_h2d returns once the source may be
+// reused (the last DMAs may still run, in stream order); copy_d2h returns
+// once the destination is written.
+int copy_h2d(whir_ctx* ctx, void* dst_dev, const void* src_host, size_t bytes);
+int copy_d2h(whir_ctx* ctx, void* dst_host, const void* src_dev, size_t bytes);
+// Staged D2H of `bytes` (a multiple of `unit`) where consume(off, p, len)
+// moves bytes [off, off + len) of the source out of the pinned chunk p; it is
+// called for pieces of whole units, several at once on the copy threads.
+using D2hConsume = std::function<void(size_t off, const uint8_t* p, size_t len)>;
+int copy_d2h_each(whir_ctx* ctx, const void* src_dev, size_t bytes, size_t unit, const D2hConsume& consume);
 
 // Autotuned LDE splits (log2 M2) per shape; 0 = not tuned.
 uint32_t tuned_split(const LdeShape& s);

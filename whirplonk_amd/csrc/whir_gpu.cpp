@@ -485,6 +485,10 @@ int whir_ctx_destroy(whir_ctx* ctx) {
     if (ctx->hio) (void)hipHostFree(ctx->hio);
     if (ctx->hdesc) (void)hipHostFree(ctx->hdesc);
     if (ctx->hdesc_ev) (void)hipEventDestroy(ctx->hdesc_ev);
+    for (int k = 0; k < 2; ++k) {
+        if (ctx->stage[k]) (void)hipHostFree(ctx->stage[k]);
+        if (ctx->stage_ev[k]) (void)hipEventDestroy(ctx->stage_ev[k]);
+    }
     for (auto* e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
@@ -516,14 +520,13 @@ int whir_dev_free(whir_ctx* ctx, void* p) {
 }
 int whir_memcpy_h2d(whir_ctx* ctx, void* dst, const void* src, size_t bytes) {
     if (!bytes) return 0;
-    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    if (int rc = copy_h2d(ctx, dst, src, bytes)) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return 0;
 }
 int whir_memcpy_d2h(whir_ctx* ctx, void* dst, const void* src, size_t bytes) {
     if (!bytes) return 0;
-    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (int rc = copy_d2h(ctx, dst, src, bytes)) return rc;
     return 0;
 }
 
@@ -787,12 +790,10 @@ static int ntt_host(uint64_t* values, size_t total, size_t size, int inverse) {
         int rc = ctx->ensure(ctx->hin, total * 8);
         if (rc) return rc;
         uint64_t* d = (uint64_t*)ctx->hin.p;
-        HIPCHK(hipMemcpyAsync(d, values, total * 8, hipMemcpyHostToDevice, ctx->stream));
+        if ((rc = copy_h2d(ctx, d, values, total * 8))) return rc;
         rc = whir_dev_ntt_batch(ctx, d, total, size, inverse);
         if (rc) return rc;
-        HIPCHK(hipMemcpyAsync(values, d, total * 8, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        return 0;
+        return copy_d2h(ctx, values, d, total * 8);
     });
 }
 
@@ -812,11 +813,9 @@ int whir_gpu_ntt_batch_transpose(uint64_t* values, size_t total, size_t ntt_size
         if ((rc = ctx->ensure(ctx->hout, total * 8))) return rc;
         uint64_t* din = (uint64_t*)ctx->hin.p;
         uint64_t* dout = (uint64_t*)ctx->hout.p;
-        HIPCHK(hipMemcpyAsync(din, values, total * 8, hipMemcpyHostToDevice, ctx->stream));
+        if ((rc = copy_h2d(ctx, din, values, total * 8))) return rc;
         if ((rc = ntt_transpose(ctx, din, true, dout, total, ntt_size, rows, cols))) return rc;
-        HIPCHK(hipMemcpyAsync(values, dout, total * 8, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        return 0;
+        return copy_d2h(ctx, values, dout, total * 8);
     });
 }
 int whir_gpu_intt_batch(uint64_t* values, size_t total, size_t size) { return ntt_host(values, total, size, 1); }
@@ -861,15 +860,15 @@ static int rs_host(const uint64_t* coeffs, uint64_t* out_matrix, uint8_t* out_ha
         bool fused = false;
         StageEvents ev;
         ev.mark(0, ctx->stream);
-        HIPCHK(hipMemcpyAsync(din, coeffs, nin * 8, hipMemcpyHostToDevice, ctx->stream));
+        if ((rc = copy_h2d(ctx, din, coeffs, nin * 8))) return rc;
         ev.mark(1, ctx->stream);
         if ((rc = run_lde(ctx, s, din, dout, dleaves, &fused))) return rc;
         ev.mark(2, ctx->stream);
         if (out_hashes && !fused) leaf_hash_launch(dout, cw, (uint32_t)(num_polys * depth), dleaves, ctx->stream);
         HIPCHK(hipGetLastError());
         ev.mark(3, ctx->stream);
-        HIPCHK(hipMemcpyAsync(out_matrix, dout, nout * 8, hipMemcpyDeviceToHost, ctx->stream));
-        if (out_hashes) HIPCHK(hipMemcpyAsync(out_hashes, ctx->hnodes.p, cw * 32, hipMemcpyDeviceToHost, ctx->stream));
+        if ((rc = copy_d2h(ctx, out_matrix, dout, nout * 8))) return rc;
+        if (out_hashes && (rc = copy_d2h(ctx, out_hashes, ctx->hnodes.p, cw * 32))) return rc;
         ev.mark(4, ctx->stream);
         HIPCHK(hipStreamSynchronize(ctx->stream));
         if (ev.on) {
@@ -913,7 +912,7 @@ int whir_gpu_interleaved_rs_encode_blake3_ext3_matrix_leaves(const uint64_t* c0,
         uint64_t* dout = (uint64_t*)ctx->hout.p;
         const uint64_t* src[3] = {c0, c1, c2};
         for (int c = 0; c < 3; ++c)
-            HIPCHK(hipMemcpyAsync(din + c * nin, src[c], nin * 8, hipMemcpyHostToDevice, ctx->stream));
+            if ((rc = copy_h2d(ctx, din + c * nin, src[c], nin * 8))) return rc;
         s.out_el = 3;
         for (uint32_t c = 0; c < 3; ++c) {
             s.out_comp = c;
@@ -921,16 +920,19 @@ int whir_gpu_interleaved_rs_encode_blake3_ext3_matrix_leaves(const uint64_t* c0,
         }
         leaf_hash_launch(dout, cw, (uint32_t)(3 * num_polys * depth), (uint8_t*)ctx->hnodes.p, ctx->stream);
         HIPCHK(hipGetLastError());
-        std::vector<uint64_t> aos(3 * nout);
-        HIPCHK(hipMemcpyAsync(aos.data(), dout, 3 * nout * 8, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipMemcpyAsync(out_hashes, ctx->hnodes.p, cw * 32, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        for (size_t i = 0; i < nout; ++i) {
-            o0[i] = aos[3 * i];
-            o1[i] = aos[3 * i + 1];
-            o2[i] = aos[3 * i + 2];
-        }
-        return 0;
+        // the AoS codeword is split into its three component arrays straight
+        // out of the staging chunks, by the copy threads
+        rc = copy_d2h_each(ctx, dout, 3 * nout * 8, 24, [=](size_t off, const uint8_t* p, size_t len) {
+            const uint64_t* q = (const uint64_t*)p;
+            const size_t i0 = off / 24, cnt = len / 24;
+            for (size_t i = 0; i < cnt; ++i) {
+                o0[i0 + i] = q[3 * i];
+                o1[i0 + i] = q[3 * i + 1];
+                o2[i0 + i] = q[3 * i + 2];
+            }
+        });
+        if (rc) return rc;
+        return copy_d2h(ctx, out_hashes, ctx->hnodes.p, cw * 32);
     });
 }
 
@@ -947,10 +949,10 @@ static int host_merkle_tree(uint32_t hash, const uint8_t* leaves, size_t n, uint
         int rc = ctx->ensure(ctx->hnodes, nn * 32);
         if (rc) return rc;
         uint8_t* d = (uint8_t*)ctx->hnodes.p;
-        if (n) HIPCHK(hipMemcpyAsync(d, leaves, n * 32, hipMemcpyHostToDevice, ctx->stream));
+        if (n && (rc = copy_h2d(ctx, d, leaves, n * 32))) return rc;
         build_tree(hash, d, n, ctx->stream);
         HIPCHK(hipGetLastError());
-        if (out_nodes) HIPCHK(hipMemcpyAsync(out_nodes, d, nn * 32, hipMemcpyDeviceToHost, ctx->stream));
+        if (out_nodes && (rc = copy_d2h(ctx, out_nodes, d, nn * 32))) return rc;
         if (out_root) HIPCHK(hipMemcpyAsync(out_root, d + 32 * (nn - 1), 32, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
         return 0;
@@ -1008,7 +1010,7 @@ static int host_open_path(uint32_t hash, const uint8_t* leaves, size_t n, const 
         if ((rc = ctx->ensure(ctx->hidx, nh * 8))) return rc;
         if ((rc = ctx->ensure(ctx->hout, nh * 32))) return rc;
         uint8_t* d = (uint8_t*)ctx->hnodes.p;
-        if (n) HIPCHK(hipMemcpyAsync(d, leaves, n * 32, hipMemcpyHostToDevice, ctx->stream));
+        if (n && (rc = copy_h2d(ctx, d, leaves, n * 32))) return rc;
         build_tree(hash, d, n, ctx->stream);
         if (nh) {
             HIPCHK(hipMemcpyAsync(ctx->hidx.p, idx.data(), nh * 8, hipMemcpyHostToDevice, ctx->stream));
@@ -1049,8 +1051,8 @@ static int ood_host(const uint64_t* const* comps, int ext, const uint64_t* point
                 for (int c = 0; c < 3; ++c) aos[3 * i + c] = comps[c][i];
             HIPCHK(hipMemcpyAsync(d, aos.data(), 3 * cnt * 8, hipMemcpyHostToDevice, ctx->stream));
             HIPCHK(hipStreamSynchronize(ctx->stream));
-        } else {
-            HIPCHK(hipMemcpyAsync(d, comps[0], cnt * 8, hipMemcpyHostToDevice, ctx->stream));
+        } else if ((rc = copy_h2d(ctx, d, comps[0], cnt * 8))) {
+            return rc;
         }
         for (size_t p = 0; p < np; ++p)
             for (size_t v = 0; v < nv; ++v) {
@@ -1079,10 +1081,8 @@ int whir_gpu_encode_to_bytes(const uint64_t* values, uint8_t* out, size_t count)
         int rc = ctx->ensure(ctx->hin, count * 8);
         if (rc) return rc;
         if (!count) return 0;
-        HIPCHK(hipMemcpyAsync(ctx->hin.p, values, count * 8, hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(hipMemcpyAsync(out, ctx->hin.p, count * 8, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        return 0;
+        if ((rc = copy_h2d(ctx, ctx->hin.p, values, count * 8))) return rc;
+        return copy_d2h(ctx, out, ctx->hin.p, count * 8);
     });
 }
 
@@ -1091,15 +1091,13 @@ static int host_hash_many(uint32_t hash, size_t size, const uint8_t* input, size
         int rc;
         if ((rc = ctx->ensure(ctx->hin, size * count + 16))) return rc;
         if ((rc = ctx->ensure(ctx->hout, count * 32))) return rc;
-        if (size * count) HIPCHK(hipMemcpyAsync(ctx->hin.p, input, size * count, hipMemcpyHostToDevice, ctx->stream));
+        if (size * count && (rc = copy_h2d(ctx, ctx->hin.p, input, size * count))) return rc;
         if (hash == HASH_SHA2)
             sha256_hash_many_launch((uint8_t*)ctx->hin.p, size, count, (uint8_t*)ctx->hout.p, ctx->stream);
         else
             hash_many_launch((uint8_t*)ctx->hin.p, size, count, (uint8_t*)ctx->hout.p, ctx->stream);
         HIPCHK(hipGetLastError());
-        if (count) HIPCHK(hipMemcpyAsync(output, ctx->hout.p, count * 32, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        return 0;
+        return copy_d2h(ctx, output, ctx->hout.p, count * 32);
     });
 }
 
@@ -1118,8 +1116,7 @@ int whir_gpu_sumcheck_compute(const uint64_t* a, int a_is_base, const uint64_t* 
         int rc;
         if ((rc = ctx->ensure(ctx->hin, na * 8))) return rc;
         if ((rc = ctx->ensure(ctx->hout, 3 * n * 8))) return rc;
-        HIPCHK(hipMemcpyAsync(ctx->hin.p, a, na * 8, hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(hipMemcpyAsync(ctx->hout.p, b, 3 * n * 8, hipMemcpyHostToDevice, ctx->stream));
+        if ((rc = copy_h2d(ctx, ctx->hin.p, a, na * 8)) || (rc = copy_h2d(ctx, ctx->hout.p, b, 3 * n * 8))) return rc;
         return whir_dev_sumcheck_compute(ctx, (uint64_t*)ctx->hin.p, a_is_base, (uint64_t*)ctx->hout.p, n, c0, c2);
     });
 }
@@ -1130,12 +1127,10 @@ int whir_gpu_sumcheck_fold(const uint64_t* v, int v_is_base, size_t n, const uin
         int rc;
         if ((rc = ctx->ensure(ctx->hin, nv * 8))) return rc;
         if ((rc = ctx->ensure(ctx->hout, 3 * (n / 2) * 8 + 8))) return rc;
-        HIPCHK(hipMemcpyAsync(ctx->hin.p, v, nv * 8, hipMemcpyHostToDevice, ctx->stream));
+        if ((rc = copy_h2d(ctx, ctx->hin.p, v, nv * 8))) return rc;
         rc = whir_dev_sumcheck_fold(ctx, (uint64_t*)ctx->hin.p, v_is_base, n, r, (uint64_t*)ctx->hout.p);
         if (rc) return rc;
-        HIPCHK(hipMemcpyAsync(out, ctx->hout.p, 3 * (n / 2) * 8, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        return 0;
+        return copy_d2h(ctx, out, ctx->hout.p, 3 * (n / 2) * 8);
     });
 }
 
