@@ -186,7 +186,7 @@ def host_resident_leg(coeffs, steps: int = 10):
             "ms_per_commit": round(ms, 3), "elem_per_s": round(cw * depth / ms * 1000.0, 1), "root": root.hex(),
             "dropin_rs_encode_leaves_ms": round(drop_ms, 2),
             "dropin_note": "whir_gpu_interleaved_rs_encode_blake3_matrix_leaves: 128 MiB in, 512 MiB codeword "
-                           "and 128 MiB leaves out to pageable host memory"}
+                           "and 128 MiB leaves out to pageable host memory (through pinned staging chunks)"}
 
 
 def host_cpu_info():
