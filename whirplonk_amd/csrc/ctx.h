@@ -4,12 +4,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <condition_variable>
 #include <cstdint>
+#include <cstring>
 #include <functional>
 #include <map>
 #include <mutex>
-#include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "gl.h"
@@ -54,10 +57,34 @@ struct DevBuf {
 // earlier work: about 6 us from a short kernel's end to the host seeing it,
 // against about 11 us through hipStreamSynchronize.  h / d: host and device
 // addresses of the word; seq: the last number enqueued.
+class WaitHub;
 struct StreamSignal {
     uint64_t* h = nullptr;
     uint64_t* d = nullptr;
     uint64_t seq = 0;
+    WaitHub* hub = nullptr;  // when set, waits sleep on the hub instead of spinning
+};
+
+// Waits that sleep instead of spinning, for hosts with more waiting threads
+// than cores (a serving loop's slots): one poller thread reads the completion
+// word of every registered wait and wakes each waiter, through its own
+// condition variable, once its word has reached the wanted value.  The
+// poller sleeps while nothing is registered.
+class WaitHub {
+public:
+    WaitHub();
+    ~WaitHub();
+    // true once *word >= want, false if that was not seen within `timeout`
+    bool wait(const uint64_t* word, uint64_t want, std::chrono::microseconds timeout);
+
+private:
+    struct Waiter;
+    void poll();
+    std::mutex mu_;
+    std::condition_variable cv_;
+    std::vector<Waiter*> waits_;
+    bool stop_ = false;
+    std::thread poller_;
 };
 int signal_init(StreamSignal& s);  // allocates the word once
 void signal_free(StreamSignal& s);

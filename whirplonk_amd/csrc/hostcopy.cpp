@@ -213,3 +213,65 @@ int copy_d2h(whir_ctx* ctx, void* dst, const void* src, size_t bytes) {
 }
 
 }  // namespace whir_gpu
+
+// ------------------------------------------------------------------ WaitHub
+
+namespace whir_gpu {
+
+struct WaitHub::Waiter {
+    const uint64_t* word;
+    uint64_t want;
+    bool done = false;
+    std::condition_variable cv;
+};
+
+WaitHub::WaitHub() : poller_([this] { poll(); }) {}
+
+WaitHub::~WaitHub() {
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        stop_ = true;
+    }
+    cv_.notify_all();
+    poller_.join();
+}
+
+bool WaitHub::wait(const uint64_t* word, uint64_t want, std::chrono::microseconds timeout) {
+    Waiter w;
+    w.word = word;
+    w.want = want;
+    std::unique_lock<std::mutex> lk(mu_);
+    if (__atomic_load_n(word, __ATOMIC_ACQUIRE) >= want) return true;
+    waits_.push_back(&w);
+    cv_.notify_all();  // the poller, if it sleeps
+    if (w.cv.wait_for(lk, timeout, [&] { return w.done; })) return true;
+    waits_.erase(std::find(waits_.begin(), waits_.end(), &w));  // still registered: the poller did not see it
+    return __atomic_load_n(word, __ATOMIC_ACQUIRE) >= want;
+}
+
+void WaitHub::poll() {
+    std::unique_lock<std::mutex> lk(mu_);
+    while (!stop_) {
+        if (waits_.empty()) {
+            cv_.wait(lk);
+            continue;
+        }
+        for (size_t i = 0; i < waits_.size();) {
+            Waiter* w = waits_[i];
+            if (__atomic_load_n(w->word, __ATOMIC_ACQUIRE) >= w->want) {
+                w->done = true;
+                w->cv.notify_one();
+                waits_[i] = waits_.back();
+                waits_.pop_back();
+            } else {
+                ++i;
+            }
+        }
+        // let waiters register and the woken ones take the lock
+        lk.unlock();
+        for (int k = 0; k < 16; ++k) __builtin_ia32_pause();
+        lk.lock();
+    }
+}
+
+}  // namespace whir_gpu

@@ -119,7 +119,7 @@ class PowBatcher {
 public:
     static constexpr int LANES = 2;  // launches in flight at once
 
-    int init(int device, uint32_t hash) {
+    int init(int device, uint32_t hash, WaitHub* hub) {
         HIPCHK(hipSetDevice(device));
         hash_ = hash;
         cap_ = (size_t)POW_BATCH_MAX << (POW_MAX_LG_LANES - 8);  // one minimum per block
@@ -132,6 +132,7 @@ public:
             l.hmin = (uint64_t*)h;
             HIPCHK(hipHostGetDevicePointer((void**)&l.hmin_d, h, 0));
             if (int rc = signal_init(l.sig)) return rc;
+            l.sig.hub = hub;
             HIPCHK(hipStreamSynchronize(l.st));
         }
         return 0;
@@ -255,12 +256,15 @@ class EncodeBatcher {
 public:
     static constexpr uint32_t MAX_SEGS = 16;
 
-    int init(int device) {
+    int init(int device, WaitHub* hub) {
         HIPCHK(hipSetDevice(device));
         HIPCHK(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
         HIPCHK(hipMalloc((void**)&table_, MAX_SEGS * sizeof(LdeMultiSeg)));
         HIPCHK(hipHostMalloc((void**)&staging_, MAX_SEGS * sizeof(LdeMultiSeg), hipHostMallocDefault));
-        return signal_init(sig_);
+        hub_ = hub;
+        if (int rc = signal_init(sig_)) return rc;
+        sig_.hub = hub;
+        return 0;
     }
     ~EncodeBatcher() {
         if (st_) (void)hipStreamSynchronize(st_);
@@ -288,13 +292,14 @@ private:
     };
 
     // one completion word per worker thread, for its own stream
-    static int thread_signal(StreamSignal** out) {
+    int thread_signal(StreamSignal** out) {
         struct Holder {
             StreamSignal s;
             ~Holder() { signal_free(s); }
         };
         thread_local Holder h;
         if (int rc = signal_init(h.s)) return rc;
+        h.s.hub = hub_;
         *out = &h.s;
         return 0;
     }
@@ -360,6 +365,7 @@ private:
     LdeMultiSeg* table_ = nullptr;    // device copy of the launch's segments
     LdeMultiSeg* staging_ = nullptr;  // pinned source of that copy
     StreamSignal sig_;
+    WaitHub* hub_ = nullptr;
     bool busy_ = false;
     std::mutex mu_;
     std::condition_variable cv_;
@@ -389,6 +395,9 @@ struct whir_server {
     std::deque<Request*> queue;
     std::map<uint64_t, std::unique_ptr<Request>> reqs;
     std::map<uint64_t, uint32_t> inject;  // request id -> device wait that fails
+    // the slots' waits sleep on this hub (its poller outlives the batchers and
+    // the workers, which are declared or joined before it goes)
+    WaitHub hub;
     PowBatcher pow;
     EncodeBatcher enc;
     uint64_t next_id = 1;
@@ -549,8 +558,8 @@ int whir_server_create(int device, const whir_params* params, uint32_t num_slots
     S->budget = (hbm_budget_bytes + 255) & ~size_t{255};
     HIPCHK(hipMalloc((void**)&S->region, S->budget));
     S->arena.init(S->region, S->budget);
-    if ((rc = S->pow.init(device, params->hash_id == WHIR_HASH_SHA2 ? HASH_SHA2 : HASH_BLAKE3)) ||
-        (rc = S->enc.init(device))) {
+    if ((rc = S->pow.init(device, params->hash_id == WHIR_HASH_SHA2 ? HASH_SHA2 : HASH_BLAKE3, &S->hub)) ||
+        (rc = S->enc.init(device, &S->hub))) {
         (void)hipFree(S->region);
         return rc;
     }
@@ -565,6 +574,7 @@ int whir_server_create(int device, const whir_params* params, uint32_t num_slots
         sl->ctx->pow_hook_arg = &S->pow;
         sl->ctx->encode_hook = &EncodeBatcher::hook;
         sl->ctx->encode_hook_arg = &S->enc;
+        sl->ctx->sig.hub = &S->hub;
         S->slots.push_back(std::move(sl));
     }
     for (auto& sl : S->slots) sl->th = std::thread(&whir_server::worker, S.get(), sl.get());

@@ -281,6 +281,18 @@ int whir_gpu::signal_wait(StreamSignal& s, hipStream_t st) {
     const uint64_t want = ++s.seq;
     signal_launch(s.d, want, st);
     HIPCHK(hipGetLastError());
+    if (s.hub) {
+        // the spin catches the completions of short waits without a context
+        // switch; then the thread sleeps until the hub's poller sees the word
+        for (int i = 0; i < 2048; ++i) {
+            if (__atomic_load_n(s.h, __ATOMIC_ACQUIRE) >= want) return 0;
+            __builtin_ia32_pause();
+        }
+        if (s.hub->wait(s.h, want, std::chrono::milliseconds(2))) return 0;
+        HIPCHK(hipStreamSynchronize(st));  // also reports stream errors
+        if (__atomic_load_n(s.h, __ATOMIC_ACQUIRE) >= want) return 0;
+        return fail("completion signal not seen after the stream synchronized", hipErrorUnknown);
+    }
     const auto t0 = std::chrono::steady_clock::now();
     for (uint32_t i = 0;; ++i) {
         if (__atomic_load_n(s.h, __ATOMIC_ACQUIRE) >= want) return 0;
