@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "gl.h"
+#include "hostsync.h"
 #include "kernels.h"
 
 namespace whir_gpu {
@@ -57,7 +58,6 @@ struct DevBuf {
 // earlier work: about 6 us from a short kernel's end to the host seeing it,
 // against about 11 us through hipStreamSynchronize.  h / d: host and device
 // addresses of the word; seq: the last number enqueued.
-class WaitHub;
 struct StreamSignal {
     uint64_t* h = nullptr;
     uint64_t* d = nullptr;
@@ -65,27 +65,6 @@ struct StreamSignal {
     WaitHub* hub = nullptr;  // when set, waits sleep on the hub instead of spinning
 };
 
-// Waits that sleep instead of spinning, for hosts with more waiting threads
-// than cores (a serving loop's slots): one poller thread reads the completion
-// word of every registered wait and wakes each waiter, through its own
-// condition variable, once its word has reached the wanted value.  The
-// poller sleeps while nothing is registered.
-class WaitHub {
-public:
-    WaitHub();
-    ~WaitHub();
-    // true once *word >= want, false if that was not seen within `timeout`
-    bool wait(const uint64_t* word, uint64_t want, std::chrono::microseconds timeout);
-
-private:
-    struct Waiter;
-    void poll();
-    std::mutex mu_;
-    std::condition_variable cv_;
-    std::vector<Waiter*> waits_;
-    bool stop_ = false;
-    std::thread poller_;
-};
 int signal_init(StreamSignal& s);  // allocates the word once
 void signal_free(StreamSignal& s);
 // Enqueues the signal on st and waits for it: polls, then yields, and after

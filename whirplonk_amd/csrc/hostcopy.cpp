@@ -12,103 +12,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <atomic>
-#include <condition_variable>
 #include <cstring>
 #include <functional>
 #include <mutex>
-#include <thread>
-#include <vector>
 
 #include "ctx.h"
 
 namespace whir_gpu {
 namespace {
-
-// Host threads for the chunk copies: run(n, fn) calls fn(0..n-1), the calling
-// thread taking parts too, and returns when all are done and every worker
-// that joined the job has left it (so no worker can claim a part of the next
-// job with this job's function).  One job at a time.
-class CopyPool {
-public:
-    static CopyPool& get() {
-        static CopyPool pool;
-        return pool;
-    }
-    size_t threads() const { return workers_.size() + 1; }
-
-    void run(size_t n, const std::function<void(size_t)>& fn) {
-        std::lock_guard<std::mutex> one(job_mu_);
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            fn_ = &fn;
-            n_ = n;
-            next_.store(0);
-            done_ = 0;
-            ++gen_;
-        }
-        cv_.notify_all();
-        size_t mine = 0;
-        for (size_t i; (i = next_.fetch_add(1)) < n;) {
-            fn(i);
-            ++mine;
-        }
-        std::unique_lock<std::mutex> lk(mu_);
-        done_ += mine;
-        done_cv_.wait(lk, [&] { return done_ == n_ && active_ == 0; });
-        fn_ = nullptr;  // a worker that wakes later does not join this job
-    }
-
-    ~CopyPool() {
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            stop_ = true;
-        }
-        cv_.notify_all();
-        for (auto& t : workers_) t.join();
-    }
-
-private:
-    CopyPool() {
-        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-        const unsigned n = std::min(7u, hw > 2 ? hw / 2 - 1 : 0u);
-        for (unsigned i = 0; i < n; ++i) workers_.emplace_back([this] { loop(); });
-    }
-    void loop() {
-        uint64_t seen = 0;
-        for (;;) {
-            const std::function<void(size_t)>* fn;
-            size_t n;
-            {
-                std::unique_lock<std::mutex> lk(mu_);
-                cv_.wait(lk, [&] { return stop_ || (gen_ != seen && fn_); });
-                if (stop_) return;
-                seen = gen_;
-                fn = fn_;
-                n = n_;
-                ++active_;
-            }
-            size_t mine = 0;
-            for (size_t i; (i = next_.fetch_add(1)) < n;) {
-                (*fn)(i);
-                ++mine;
-            }
-            std::lock_guard<std::mutex> lk(mu_);
-            done_ += mine;
-            --active_;
-            if (done_ == n_ && active_ == 0) done_cv_.notify_all();
-        }
-    }
-
-    std::vector<std::thread> workers_;
-    std::mutex job_mu_, mu_;
-    std::condition_variable cv_, done_cv_;
-    const std::function<void(size_t)>* fn_ = nullptr;
-    size_t n_ = 0, done_ = 0, active_ = 0;
-    std::atomic<size_t> next_{0};
-    uint64_t gen_ = 0;
-    bool stop_ = false;
-};
 
 constexpr size_t STAGE_CHUNK = size_t{16} << 20;  // bytes per staging chunk
 constexpr size_t STAGED_MIN = size_t{4} << 20;    // smaller copies take one plain DMA
@@ -210,68 +121,6 @@ int copy_d2h(whir_ctx* ctx, void* dst, const void* src, size_t bytes) {
     }
     return copy_d2h_each(ctx, src, bytes, 1,
                          [dst](size_t off, const uint8_t* p, size_t len) { std::memcpy((uint8_t*)dst + off, p, len); });
-}
-
-}  // namespace whir_gpu
-
-// ------------------------------------------------------------------ WaitHub
-
-namespace whir_gpu {
-
-struct WaitHub::Waiter {
-    const uint64_t* word;
-    uint64_t want;
-    bool done = false;
-    std::condition_variable cv;
-};
-
-WaitHub::WaitHub() : poller_([this] { poll(); }) {}
-
-WaitHub::~WaitHub() {
-    {
-        std::lock_guard<std::mutex> lk(mu_);
-        stop_ = true;
-    }
-    cv_.notify_all();
-    poller_.join();
-}
-
-bool WaitHub::wait(const uint64_t* word, uint64_t want, std::chrono::microseconds timeout) {
-    Waiter w;
-    w.word = word;
-    w.want = want;
-    std::unique_lock<std::mutex> lk(mu_);
-    if (__atomic_load_n(word, __ATOMIC_ACQUIRE) >= want) return true;
-    waits_.push_back(&w);
-    cv_.notify_all();  // the poller, if it sleeps
-    if (w.cv.wait_for(lk, timeout, [&] { return w.done; })) return true;
-    waits_.erase(std::find(waits_.begin(), waits_.end(), &w));  // still registered: the poller did not see it
-    return __atomic_load_n(word, __ATOMIC_ACQUIRE) >= want;
-}
-
-void WaitHub::poll() {
-    std::unique_lock<std::mutex> lk(mu_);
-    while (!stop_) {
-        if (waits_.empty()) {
-            cv_.wait(lk);
-            continue;
-        }
-        for (size_t i = 0; i < waits_.size();) {
-            Waiter* w = waits_[i];
-            if (__atomic_load_n(w->word, __ATOMIC_ACQUIRE) >= w->want) {
-                w->done = true;
-                w->cv.notify_one();
-                waits_[i] = waits_.back();
-                waits_.pop_back();
-            } else {
-                ++i;
-            }
-        }
-        // let waiters register and the woken ones take the lock
-        lk.unlock();
-        for (int k = 0; k < 16; ++k) __builtin_ia32_pause();
-        lk.lock();
-    }
 }
 
 }  // namespace whir_gpu
