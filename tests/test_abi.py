@@ -77,3 +77,19 @@ def test_no_fallback_when_library_missing(tmp_path, monkeypatch):
     with pytest.raises(ImportError):
         W.load()
     importlib.reload(W)
+
+
+def test_null_handles_are_refused():
+    """Entry points given a null batch, server or context report an error
+    instead of touching a device."""
+    lib = W.load()
+    a, b = ctypes.c_double(7.0), ctypes.c_double(7.0)
+    assert lib.whir_batch_form(None, ctypes.byref(a), ctypes.byref(b)) == -1
+    assert lib.whir_batch_run(None) != 0
+    out = np.zeros(11, dtype=np.uint64)
+    assert lib.whir_server_stats(None, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))) != 0
+    h = ctypes.c_void_p()
+    sizes = np.array([1 << 14], dtype=np.uint64)
+    # a batch form outside -1..1 is rejected before any device work
+    assert lib.whir_batch_create_form(None, 1, sizes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), 2, 4, 1, 0, 5,
+                                      ctypes.byref(h)) != 0

@@ -127,6 +127,7 @@ def test_batch_sha256(ctx):
     finally:
         ctx.set_hash(W.HASH_BLAKE3)
     with b:
+        assert b.form()[0] == 0  # the level-synchronous form is BLAKE3 only
         for k, c in enumerate(polys):
             b.upload(k, c)
         want = []
