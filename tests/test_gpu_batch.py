@@ -82,7 +82,7 @@ def test_batch_roots_and_replays(ctx):
 
 
 @pytest.mark.parametrize("rate,lgs", [(2, list(range(22, 13, -1)) + [14, 12]), (1, [18, 15, 14, 11]),
-                                       (3, [21, 17, 14])])
+                                       (3, [21, 17, 14]), (4, [22, 16, 14])])
 def test_batch_level_synchronous(ctx, rate, lgs):
     """The level-synchronous form (one first-pass launch and one second-pass
     launch over all polynomials, trees level by level): every root equals the
@@ -92,7 +92,7 @@ def test_batch_level_synchronous(ctx, rate, lgs):
     (committed per polynomial in the same graph); whole node arrays and
     codewords against the oracle for a few."""
     sizes = [1 << lg for lg in lgs]
-    polys = _polys(sizes, 0x1E7E1 + rate)
+    polys = _polys(sizes, 0x1E7E1 + rate)  # rate 4: 2^22 coefficients -> 2^22 rows, 16 cosets
     print("level batch rate", rate, sizes, flush=True)
     want = [_commit_ref(c, rate=rate) for c in polys]
     with W.Batch(ctx, sizes, log_inv_rate=rate, folding_factor=4, branches=2, autotune=False, form=1) as b:
