@@ -372,20 +372,38 @@ __global__ __launch_bounds__(256) void term_sweep_kernel(uint64_t* __restrict__ 
     st3(acc, i, sum);
 }
 
-// partial sums of f_i * hi[i >> L] * lo[i & mask] (one eq term)
+// partial sums of f_i * hi[i >> L] * lo[i & mask] (one eq term), as
+//   sum_l lo[l] * (sum_h f[h 2^L + l] hi[h]):
+// the stride (a multiple of 2^L: mle_eval_launch keeps the grid a multiple of
+// 2^L / 256 when 2^L > 256) leaves a lane's column l = i & mask fixed across
+// its iterations, so the lane accumulates f_i hi[h] without reduction and
+// multiplies its total by lo[l] once.  When 2^L >= 256 a workgroup's 256
+// indices share one row h, whose hi entry is then a uniform (scalar) load.
 __global__ __launch_bounds__(256) void mle_eval_kernel(const uint64_t* __restrict__ f, int f_base, uint64_t n,
                                                        uint32_t L, const uint64_t* __restrict__ hi,
                                                        const uint64_t* __restrict__ lo, uint64_t* partials) {
     gl::e3acc s = gl::e3acc_zero();
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x, mask = (1ull << L) - 1;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        e3 w = gl::e3_mul(ld3(hi, i >> L), ld3(lo, i & mask));
-        if (f_base)
-            gl::e3_mac_base(s, w, f[i]);
-        else
-            gl::e3_mac(s, w, ld3(f, i));
+    const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (L >= 8) {
+        for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
+            const e3 w = ld3(hi, base >> L);
+            const uint64_t i = base + threadIdx.x;
+            if (f_base)
+                gl::e3_mac_base(s, w, f[i]);
+            else
+                gl::e3_mac(s, w, ld3(f, i));
+        }
+    } else {
+        for (uint64_t i = i0; i < n; i += stride) {
+            const e3 w = ld3(hi, i >> L);
+            if (f_base)
+                gl::e3_mac_base(s, w, f[i]);
+            else
+                gl::e3_mac(s, w, ld3(f, i));
+        }
     }
-    e3 acc[1] = {gl::e3acc_reduce(s)};
+    e3 acc[1] = {gl::e3_mul(gl::e3acc_reduce(s), ld3(lo, i0 & mask))};
     block_sum<1>(acc);
     store_block_totals<1>(acc, partials);
 }
@@ -535,6 +553,9 @@ void term_accumulate_launch(uint64_t* d_acc, uint64_t n, uint32_t L, const uint6
 unsigned mle_eval_launch(const uint64_t* f, bool f_base, uint64_t n, uint32_t L, const uint64_t* hi,
                          const uint64_t* lo, uint64_t* partials, hipStream_t st, bool device_sum) {
     unsigned g = std::min<unsigned>(blocks_of(n), SUMCHECK_MAX_BLOCKS);
+    // the kernel needs a stride (256 g) that is a multiple of 2^L
+    const unsigned gq = L > 8 ? (1u << (L - 8)) : 1u;
+    g = std::max(gq, g / gq * gq);
     hipLaunchKernelGGL(mle_eval_kernel, dim3(g), dim3(256), 0, st, f, f_base ? 1 : 0, n, L, hi, lo, partials);
     if (device_sum) hipLaunchKernelGGL(red::sum_blocks_kernel<1>, dim3(1), dim3(256), 0, st, partials, g, partials);
     return g;
