@@ -27,6 +27,7 @@
 #include "reduce_dev.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace whir_gpu {
 namespace {
@@ -372,37 +373,59 @@ __global__ __launch_bounds__(256) void term_sweep_kernel(uint64_t* __restrict__ 
     st3(acc, i, sum);
 }
 
+// sum over a lane's indices i = b + threadIdx.x (b = blockIdx.x * 256 + k *
+// stride) of f_i * hi[i >> L], accumulated without reduction; the loads of U
+// iterations (8 for base-field f, 4 for Ext3) are issued before their products.  ROW: 2^L >= 256, so the
+// workgroup's 256 indices share one row and its hi entry is a uniform load.
+template <bool F_BASE, bool ROW>
+__device__ __forceinline__ gl::e3acc mle_sweep(const uint64_t* __restrict__ f, uint64_t n, uint32_t L,
+                                              const uint64_t* __restrict__ hi) {
+    using V = typename std::conditional<F_BASE, uint64_t, e3>::type;
+    constexpr int U = F_BASE ? 8 : 4;
+    gl::e3acc s = gl::e3acc_zero();
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x, t = threadIdx.x;
+    auto ld = [&](uint64_t i) -> V {
+        if constexpr (F_BASE)
+            return f[i];
+        else
+            return ld3(f, i);
+    };
+    auto row = [&](uint64_t b) { return ld3(hi, (ROW ? b : b + t) >> L); };
+    auto mac = [&](const e3& w, const V& v) {
+        if constexpr (F_BASE)
+            gl::e3_mac_base(s, w, v);
+        else
+            gl::e3_mac(s, w, v);
+    };
+    uint64_t b = (uint64_t)blockIdx.x * blockDim.x;
+    for (; b + (U - 1) * stride + t < n; b += U * stride) {
+        V v[U];
+        e3 w[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            v[k] = ld(b + k * stride + t);
+            w[k] = row(b + k * stride);
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) mac(w[k], v[k]);
+    }
+    for (; b + t < n; b += stride) mac(row(b), ld(b + t));
+    return s;
+}
+
 // partial sums of f_i * hi[i >> L] * lo[i & mask] (one eq term), as
 //   sum_l lo[l] * (sum_h f[h 2^L + l] hi[h]):
 // the stride (a multiple of 2^L: mle_eval_launch keeps the grid a multiple of
 // 2^L / 256 when 2^L > 256) leaves a lane's column l = i & mask fixed across
 // its iterations, so the lane accumulates f_i hi[h] without reduction and
-// multiplies its total by lo[l] once.  When 2^L >= 256 a workgroup's 256
-// indices share one row h, whose hi entry is then a uniform (scalar) load.
+// multiplies its total by lo[l] once.
 __global__ __launch_bounds__(256) void mle_eval_kernel(const uint64_t* __restrict__ f, int f_base, uint64_t n,
                                                        uint32_t L, const uint64_t* __restrict__ hi,
                                                        const uint64_t* __restrict__ lo, uint64_t* partials) {
-    gl::e3acc s = gl::e3acc_zero();
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x, mask = (1ull << L) - 1;
-    const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (L >= 8) {
-        for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x; base < n; base += stride) {
-            const e3 w = ld3(hi, base >> L);
-            const uint64_t i = base + threadIdx.x;
-            if (f_base)
-                gl::e3_mac_base(s, w, f[i]);
-            else
-                gl::e3_mac(s, w, ld3(f, i));
-        }
-    } else {
-        for (uint64_t i = i0; i < n; i += stride) {
-            const e3 w = ld3(hi, i >> L);
-            if (f_base)
-                gl::e3_mac_base(s, w, f[i]);
-            else
-                gl::e3_mac(s, w, ld3(f, i));
-        }
-    }
+    const bool row = L >= 8;
+    const gl::e3acc s = f_base ? (row ? mle_sweep<true, true>(f, n, L, hi) : mle_sweep<true, false>(f, n, L, hi))
+                               : (row ? mle_sweep<false, true>(f, n, L, hi) : mle_sweep<false, false>(f, n, L, hi));
+    const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, mask = (1ull << L) - 1;
     e3 acc[1] = {gl::e3_mul(gl::e3acc_reduce(s), ld3(lo, i0 & mask))};
     block_sum<1>(acc);
     store_block_totals<1>(acc, partials);
