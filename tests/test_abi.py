@@ -93,3 +93,8 @@ def test_null_handles_are_refused():
     # a batch form outside -1..1 is rejected before any device work
     assert lib.whir_batch_create_form(None, 1, sizes.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), 2, 4, 1, 0, 5,
                                       ctypes.byref(h)) != 0
+    # copies through a null context, or to or from a null pointer
+    buf = np.zeros(4, dtype=np.uint64)
+    p = buf.ctypes.data_as(ctypes.c_void_p)
+    assert lib.whir_memcpy_h2d(None, p, p, 32) != 0
+    assert lib.whir_memcpy_d2h(None, p, p, 32) != 0

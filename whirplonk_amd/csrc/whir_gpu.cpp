@@ -530,16 +530,20 @@ int whir_dev_free(whir_ctx* ctx, void* p) {
     if (p) HIPCHK(hipFree(p));
     return 0;
 }
+// (the context's lock: large copies go through its staging chunks)
 int whir_memcpy_h2d(whir_ctx* ctx, void* dst, const void* src, size_t bytes) {
     if (!bytes) return 0;
+    if (!ctx || !dst || !src) return bad_arg("null argument");
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
     if (int rc = copy_h2d(ctx, dst, src, bytes)) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     return 0;
 }
 int whir_memcpy_d2h(whir_ctx* ctx, void* dst, const void* src, size_t bytes) {
     if (!bytes) return 0;
-    if (int rc = copy_d2h(ctx, dst, src, bytes)) return rc;
-    return 0;
+    if (!ctx || !dst || !src) return bad_arg("null argument");
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    return copy_d2h(ctx, dst, src, bytes);
 }
 
 int whir_dev_copy(whir_ctx* ctx, void* dst, const void* src, size_t bytes) {
