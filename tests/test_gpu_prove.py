@@ -137,8 +137,11 @@ def test_geometric_golden(ctx, fixtures):
     ctx.free(d)
 
 
-@pytest.mark.parametrize("nv,base", [(0, True), (1, False), (5, True), (10, False), (12, True), (16, False),
-                                     (20, True)])
+# nv < 8: one lane per row entry; nv 8-10: rows of 2^L >= 256 (the grid a
+# multiple of 2^L / 256); below nv 21 (Ext3) and 22 (base) a lane's loop runs
+# single steps only, from there on groups of loads in flight (two at nv 23)
+@pytest.mark.parametrize("nv,base", [(0, True), (1, False), (5, True), (7, False), (8, True), (9, False), (10, False),
+                                     (12, True), (16, False), (20, True), (21, False), (22, True), (23, True)])
 def test_mle_evaluate(ctx, nv, base):
     rnd = random.Random(nv)
     n = 1 << nv
