@@ -179,3 +179,37 @@ def test_staged_host_copies(nbytes):
         assert np.array_equal(pin.array, src[::-1])
         assert np.array_equal(ctx.download(d, n), src[::-1])
         ctx.free(d)
+
+
+def test_staged_copies_from_threads():
+    """Several host threads copying through one context at once: the copies
+    share the context's staging chunks, so each takes the context's lock and
+    every buffer still arrives and returns intact."""
+    import threading
+
+    nthreads, n = 4, (40 << 20) // 8
+    srcs = [W.lcg_uniform(1234 + t, n)[0] for t in range(nthreads)]
+    errors = []
+    print("staged copies from", nthreads, "threads", flush=True)
+    with W.Context() as ctx:
+        bufs = [ctx.alloc(8 * n) for _ in range(nthreads)]
+
+        def work(t):
+            try:
+                for _ in range(3):
+                    W._check(W.load().whir_memcpy_h2d(ctx.h, bufs[t], W._p64(srcs[t]), 8 * n))
+                    back = np.empty(n, dtype=np.uint64)
+                    W._check(W.load().whir_memcpy_d2h(ctx.h, W._p64(back), bufs[t], 8 * n))
+                    if not np.array_equal(back, srcs[t]):
+                        errors.append(f"thread {t}: data changed")
+            except Exception as e:  # reported below, on the test's thread
+                errors.append(f"thread {t}: {e}")
+
+        threads = [threading.Thread(target=work, args=(t,)) for t in range(nthreads)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+        for b in bufs:
+            ctx.free(b)
+    assert not errors, errors
