@@ -10,8 +10,9 @@
 #include "../../include/whir_gpu.h"
 
 #include <chrono>
-#include <memory>
 #include <cstring>
+#include <functional>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -38,10 +39,17 @@ struct DevWitness {
     size_t row_u64() const { return num_cols * (ext ? 3 : 1); }
 };
 
-// Opened rows (irs_commit::Evaluations): points and rows, row-major.
+// Opened rows (irs_commit::Evaluations): points and rows, row-major.  The
+// rows and path hashes arrive in the context's mapped buffer; until
+// finish_open has read them (and hinted them) the opening is pending.
 struct Opened {
     std::vector<uint64_t> points;  // base field (gen^index)
     std::vector<uint64_t> rows;    // count x row_u64
+    bool pending = false;
+    const void* buf = nullptr;  // ctx->hio when the gather was enqueued
+    uint64_t seq = 0;           // the context's last completion number then
+    size_t ru = 0, cols = 0, nh = 0, hoff = 0;
+    bool ext = false;
 };
 
 uint32_t lg(uint64_t n) { return log2u(n); }
@@ -295,11 +303,16 @@ int read_pair(whir_ctx* ctx, unsigned g, e3& c0, e3& c2) {
 // sumcheck::Config::prove over (a, b) of length n: a_out receives the folded
 // a (may equal a when a is Ext3); b is folded in place.  Returns the
 // challenges in `rs`.
+// `settle`, when given, runs after the first round's sums have arrived and
+// before `sum` is read (or, with no rounds, at once): it may finish host work
+// that waits on earlier launches of the stream, such as an opening whose
+// rows update `sum`.
 int sumcheck_prove(whir_prover* P, const uint64_t* a, bool a_base, uint64_t* a_out, uint64_t* b, size_t n,
-                   size_t rounds, uint64_t round_pow, e3& sum, std::vector<e3>& rs) {
+                   size_t rounds, uint64_t round_pow, e3& sum, std::vector<e3>& rs,
+                   const std::function<int()>& settle = {}) {
     whir_ctx* ctx = P->ctx;
     rs.clear();
-    if (rounds == 0) return 0;
+    if (rounds == 0) return settle ? settle() : 0;
     Phase ph(P, PH_SUMCHECK);
     int rc;
     if ((rc = ctx->ensure_hpart())) return rc;
@@ -308,6 +321,7 @@ int sumcheck_prove(whir_prover* P, const uint64_t* a, bool a_base, uint64_t* a_o
     unsigned g = sumcheck_compute_launch(a, a_base, b, n, part, ctx->stream, false);
     HIPCHK(hipGetLastError());
     if ((rc = read_pair(ctx, g, c0, c2))) return rc;
+    if (settle && (rc = settle())) return rc;
     for (size_t r = 0; r < rounds; ++r) {
         e3 c1 = gl::e3_sub(gl::e3_sub(sum, gl::e3_add(c0, c0)), c2);
         P->ts.message_e3(c0);
@@ -399,7 +413,10 @@ int commit_witness(whir_prover* P, const whir_host::IrsConfig& c, const uint64_t
 }
 
 // irs_commit::Config::open for one witness: squeeze the indices, hint the
-// opened rows (one vector of elements), then the Merkle path hashes.
+// opened rows (one vector of elements), then the Merkle path hashes.  The
+// gather is only enqueued here; finish_open reads its output and appends the
+// hints (hints are not absorbed, so appending them later changes no
+// challenge), usually after the next wait the prove loop needs anyway.
 int open_witness(whir_prover* P, const whir_host::IrsConfig& c, const DevWitness& w, Opened& o) {
     whir_ctx* ctx = P->ctx;
     int rc;
@@ -427,10 +444,32 @@ int open_witness(whir_prover* P, const whir_host::IrsConfig& c, const DevWitness
     gather_open_launch(w.matrix, ru, back_d + ioff, k, back_d, w.nodes, back_d + ioff + k, nh,
                        (uint8_t*)(back_d + hoff), ctx->stream);
     HIPCHK(hipGetLastError());
-    if ((rc = ctx->wait())) return rc;
-    o.rows.assign(back, back + k * ru);
-    P->ts.hint_vector(o.rows.data(), k * w.num_cols, w.ext ? 24 : 8);
-    for (size_t h = 0; h < nh; ++h) P->ts.hint(back + hoff + 4 * h, 32);
+    o.pending = true;
+    o.buf = back;
+    o.seq = ctx->sig.seq;
+    o.ru = ru;
+    o.cols = w.num_cols;
+    o.nh = nh;
+    o.hoff = hoff;
+    o.ext = w.ext;
+    return 0;
+}
+
+// Reads a pending opening's rows and hashes and appends its hints; waits
+// first unless a completion signal enqueued after its gather has been seen.
+// Nothing may grow the context's mapped buffer between the two calls.
+int finish_open(whir_prover* P, Opened& o) {
+    if (!o.pending) return 0;
+    whir_ctx* ctx = P->ctx;
+    int rc;
+    if (ctx->sig.seq == o.seq && (rc = ctx->wait())) return rc;
+    if (ctx->hio != o.buf) return bad_arg("the opening's buffer moved before its rows were read");
+    const uint64_t* back = (const uint64_t*)o.buf;
+    const size_t k = o.points.size();
+    o.rows.assign(back, back + k * o.ru);
+    P->ts.hint_vector(o.rows.data(), k * o.cols, o.ext ? 24 : 8);
+    for (size_t h = 0; h < o.nh; ++h) P->ts.hint(back + o.hoff + 4 * h, 32);
+    o.pending = false;
     return 0;
 }
 
@@ -567,18 +606,11 @@ int prove_impl(whir_prover* P, size_t num_forms, const uint64_t* points, const u
             o_cols = prev->num_cols;
         }
         // STIR constraints (whir_prover.hpp:286-345)
-        std::vector<e3> stir_eval(w.ood_values);
         std::vector<e3> eqw = last_fr.empty() ? std::vector<e3>{E3_ONE} : eq_weights(last_fr);
         std::vector<e3> tp;
         for (const e3& v : vrlc)
             for (const e3& e : eqw) tp.push_back(gl::e3_mul(v, e));
         if (tp.size() != o_cols) return bad_arg("opened row width does not match the folding weights");
-        for (size_t r = 0; r < o.points.size(); ++r) {
-            e3 acc = E3_ZERO;
-            for (size_t c = 0; c < o_cols; ++c)
-                acc = gl::e3_add(acc, gl::e3_mul(row_elem(o, r, c, o_cols, o_ext), tp[c]));
-            stir_eval.push_back(acc);
-        }
         const size_t nstir = w.ood_points.size() + o.points.size();
         std::vector<e3> srlc = P->ts.geometric(nstir);
         {
@@ -589,10 +621,24 @@ int prove_impl(whir_prover* P, size_t num_forms, const uint64_t* points, const u
                 T.power(e3{o.points[j], 0, 0}, srlc[w.ood_points.size() + j], L);
             if (nstir && (rc = terms_apply(ctx, T, cov, lv, true, P))) return rc;
         }
-        for (size_t j = 0; j < nstir; ++j) sum = gl::e3_add(sum, gl::e3_mul(srlc[j], stir_eval[j]));
+        // the opened rows' values at the query points join the claimed sum
+        // once they have arrived: after the round sumcheck's first sums (one
+        // wait for both), before its first message
+        auto settle = [&]() -> int {
+            if (int e = finish_open(P, o)) return e;
+            std::vector<e3> stir_eval(w.ood_values);
+            for (size_t r = 0; r < o.points.size(); ++r) {
+                e3 acc = E3_ZERO;
+                for (size_t c = 0; c < o_cols; ++c)
+                    acc = gl::e3_add(acc, gl::e3_mul(row_elem(o, r, c, o_cols, o_ext), tp[c]));
+                stir_eval.push_back(acc);
+            }
+            for (size_t j = 0; j < nstir; ++j) sum = gl::e3_add(sum, gl::e3_mul(srlc[j], stir_eval[j]));
+            return 0;
+        };
         // round sumcheck
         if ((rc = sumcheck_prove(P, vec, false, vec, cov, cur, rc_cfg.sumcheck.num_rounds,
-                                 rc_cfg.sumcheck.round_pow, sum, fr)))
+                                 rc_cfg.sumcheck.round_pow, sum, fr, settle)))
             return rc;
         P->eval_point.insert(P->eval_point.end(), fr.begin(), fr.end());
         last_fr = fr;
@@ -614,16 +660,15 @@ int prove_impl(whir_prover* P, size_t num_forms, const uint64_t* points, const u
         for (size_t i = 0; i < cur; ++i) P->ts.message(fin + 3 * i, 24);
     }
     if ((rc = pow_prove(P, cfg.final_pow))) return rc;
-    {
-        Opened o;
-        if (!prev) {
-            if ((rc = open_witness(P, cfg.initial, P->initial, o))) return rc;
-        } else {
-            if ((rc = open_witness(P, *prev_cfg, *prev, o))) return rc;
-        }
+    Opened fo;
+    if (!prev) {
+        if ((rc = open_witness(P, cfg.initial, P->initial, fo))) return rc;
+    } else {
+        if ((rc = open_witness(P, *prev_cfg, *prev, fo))) return rc;
     }
+    // the final opening's hints, after the final sumcheck's first sums
     if ((rc = sumcheck_prove(P, vec, false, vec, cov, cur, cfg.final_sumcheck.num_rounds,
-                             cfg.final_sumcheck.round_pow, sum, fr)))
+                             cfg.final_sumcheck.round_pow, sum, fr, [&] { return finish_open(P, fo); })))
         return rc;
     P->eval_point.insert(P->eval_point.end(), fr.begin(), fr.end());
     return sync(P);
