@@ -267,7 +267,7 @@ def prove_leg(steps: int, warmup: int, cpu: bool, hash_id: int = 1, nv: int = PR
     pid, sid = W.domain_separator_cli(params)
     ctx = W.Context(0)
     d = ctx.upload(coeffs)
-    ev = np.array([ctx.mle_evaluate(d, True, pts[f]) for f in range(len(pts))], dtype=np.uint64)
+    ev = ctx.mle_evaluate_many(d, True, pts)
     ws_bytes = W.workspace_bytes(1 << nv, params)
     ws = ctx.alloc(ws_bytes)
 
@@ -412,7 +412,7 @@ def served_leg(n_req: int, offered: float, slots: int, budget_gb: float, cpu: bo
 
     def plain(nv, pin, pts):
         W._check(W.load().whir_memcpy_h2d(ctx.h, dbuf[nv], pin.ptr, 8 << nv))
-        ev = np.array([ctx.mle_evaluate(dbuf[nv], True, pts[f]) for f in range(len(pts))], dtype=np.uint64)
+        ev = ctx.mle_evaluate_many(dbuf[nv], True, pts)  # every constraint in one pass, as the server does
         with W.Prover(ctx, params, 1 << nv, pid, sid, workspace=(wbuf[nv], ws[nv])) as pr:
             pr.commit(dbuf[nv], 1)
             pr.prove(pts, ev)

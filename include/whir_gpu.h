@@ -380,6 +380,12 @@ int whir_dev_geometric_accumulate(whir_ctx* ctx, uint64_t* d_acc, size_t n, cons
  * extension of d_f (2^num_variables entries, base or Ext3) at point. */
 int whir_dev_mle_evaluate(whir_ctx* ctx, const uint64_t* d_f, int f_is_base, size_t num_variables,
                           const uint64_t* point, uint64_t* out);
+/* No reference counterpart: whir_dev_mle_evaluate at `count` points (points:
+ * count x num_variables Ext3, out: count Ext3), for callers that evaluate
+ * several constraints of one vector (the prover's MLE constraints, a serving
+ * loop's requests): up to four points share one pass over d_f and one wait. */
+int whir_dev_mle_evaluate_many(whir_ctx* ctx, const uint64_t* d_f, int f_is_base, size_t num_variables,
+                               const uint64_t* points, size_t count, uint64_t* out);
 
 /* --------------------------------------------------------- batched commit */
 

@@ -154,6 +154,24 @@ def test_mle_evaluate(ctx, nv, base):
     ctx.free(d)
 
 
+# 1-4 points share a launch (5 and 6 take two); base and Ext3 f, rows of
+# 2^L below and above 256, single steps and groups of loads per lane
+@pytest.mark.parametrize("nv,base,count", [(0, True, 2), (3, False, 3), (7, True, 4), (9, True, 5), (10, False, 4),
+                                           (12, True, 3), (16, False, 2), (20, True, 6), (21, False, 3),
+                                           (22, True, 3), (23, True, 4)])
+def test_mle_evaluate_many(ctx, nv, base, count):
+    rnd = random.Random(100 + nv + count)
+    n = 1 << nv
+    f, _ = W.lcg_uniform(2000 + nv, n * (1 if base else 3))
+    pts = np.array([_e3s(rnd, nv) for _ in range(count)], dtype=np.uint64).reshape(count, nv, 3)
+    d = ctx.upload(f)
+    print("mle many", nv, base, count, flush=True)
+    got = ctx.mle_evaluate_many(d, base, pts)
+    for k in range(count):
+        assert np.array_equal(got[k], oracle.mle_evaluate(f, pts[k], base)), k
+    ctx.free(d)
+
+
 def test_mle_golden(ctx, fixtures):
     for c in fixtures["utils"]["multilinear_extend"]:
         d = ctx.upload(np.array(c["evals"], dtype=np.uint64))

@@ -121,6 +121,7 @@ _SIGS = [
     ("whir_dev_eq_accumulate", _int, [_vp, _vp, _sz, _u64p, _u64p, _int]),
     ("whir_dev_geometric_accumulate", _int, [_vp, _vp, _sz, _u64p, _u64p, _sz]),
     ("whir_dev_mle_evaluate", _int, [_vp, _vp, _int, _sz, _u64p, _u64p]),
+    ("whir_dev_mle_evaluate_many", _int, [_vp, _vp, _int, _sz, _u64p, _sz, _u64p]),
     ("whir_autotune_lde", _int, [_vp, _sz, _sz, _sz, _int, ctypes.POINTER(ctypes.c_double),
                                  ctypes.POINTER(ctypes.c_uint32)]),
     ("whir_tuning_export", _sz, [_u64p, _sz]),
@@ -467,6 +468,18 @@ class Context:
         pt = _u64(point).reshape(-1)
         out = np.empty(3, dtype=np.uint64)
         _check(load().whir_dev_mle_evaluate(self.h, d_f, int(f_is_base), pt.size // 3, _p64(pt), _p64(out)))
+        return out
+
+    def mle_evaluate_many(self, d_f, f_is_base, points) -> np.ndarray:
+        """The evaluations at points (count x num_variables x 3) -> (count, 3)."""
+        pts = _u64(points)
+        if pts.ndim != 3 or pts.shape[2] != 3:
+            raise ValueError("points must have shape (count, num_variables, 3)")
+        count, nv = pts.shape[0], pts.shape[1]
+        out = np.empty((count, 3), dtype=np.uint64)
+        flat = np.ascontiguousarray(pts.reshape(-1)) if pts.size else np.zeros(1, dtype=np.uint64)
+        _check(load().whir_dev_mle_evaluate_many(self.h, d_f, int(f_is_base), nv, _p64(flat), count,
+                                                 _p64(out) if count else None))
         return out
 
 

@@ -211,10 +211,14 @@ uint64_t term_limb_bytes(uint32_t nv, uint32_t L, uint32_t terms);
 void term_accumulate_launch(uint64_t* d_acc, uint64_t n, uint32_t L, const uint64_t* hi_e, const uint64_t* lo_e,
                             uint32_t te, const uint64_t* hi_b, const uint64_t* lo_b, uint32_t tb, const uint32_t* hl,
                             bool accumulate, hipStream_t st);
-// sum_i f_i hi[i >> L] lo[i & mask] into partials[0..3) (device_sum), or
-// per block (partials[3b..3b+3)); returns the number of blocks
+// For each of `points` (1..MLE_MAX_POINTS) eq terms p, whose tables are
+// hi + 3 (n >> L) p and lo + 3 2^L p: sum_i f_i hi_p[i >> L] lo_p[i & mask],
+// per block into partials[3 (points b + p) ..] (at most 3 * points *
+// SUMCHECK_MAX_BLOCKS / (points > 2 ? 2 : 1) words); returns the number of
+// blocks (0 for a bad point count).
+constexpr uint32_t MLE_MAX_POINTS = 4;
 unsigned mle_eval_launch(const uint64_t* f, bool f_base, uint64_t n, uint32_t L, const uint64_t* hi,
-                         const uint64_t* lo, uint64_t* partials, hipStream_t st, bool device_sum = true);
+                         const uint64_t* lo, uint32_t points, uint64_t* partials, hipStream_t st);
 // fold (a, b) of length n (>= 4) by r to length n/2, and (c0, c2) of the folded
 // pair into partials[0..6) (device_sum), or per block as sumcheck_compute_launch;
 // Ext3 inputs may be folded in place.  Returns the number of blocks.

@@ -373,16 +373,16 @@ __global__ __launch_bounds__(256) void term_sweep_kernel(uint64_t* __restrict__ 
     st3(acc, i, sum);
 }
 
-// sum over a lane's indices i = b + threadIdx.x (b = blockIdx.x * 256 + k *
-// stride) of f_i * hi[i >> L], accumulated without reduction; the loads of U
-// iterations (8 for base-field f, 4 for Ext3) are issued before their products.  ROW: 2^L >= 256, so the
-// workgroup's 256 indices share one row and its hi entry is a uniform load.
-template <bool F_BASE, bool ROW>
-__device__ __forceinline__ gl::e3acc mle_sweep(const uint64_t* __restrict__ f, uint64_t n, uint32_t L,
-                                              const uint64_t* __restrict__ hi) {
+// For each of K points: the sum over a lane's indices i = b + threadIdx.x
+// (b = blockIdx.x * 256 + k * stride) of f_i * hi_p[i >> L], accumulated
+// without reduction; every load of f serves all K points, and the loads of U
+// iterations are issued before their products.  ROW: 2^L >= 256, so the
+// workgroup's 256 indices share one row and its hi entries are uniform loads.
+template <bool F_BASE, bool ROW, int K>
+__device__ __forceinline__ void mle_sweep(const uint64_t* __restrict__ f, uint64_t n, uint32_t L,
+                                          const uint64_t* __restrict__ hi, uint64_t H, gl::e3acc* s) {
     using V = typename std::conditional<F_BASE, uint64_t, e3>::type;
-    constexpr int U = F_BASE ? 8 : 4;
-    gl::e3acc s = gl::e3acc_zero();
+    constexpr int U = std::max(1, (F_BASE ? 8 : 4) / K);
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x, t = threadIdx.x;
     auto ld = [&](uint64_t i) -> V {
         if constexpr (F_BASE)
@@ -390,45 +390,64 @@ __device__ __forceinline__ gl::e3acc mle_sweep(const uint64_t* __restrict__ f, u
         else
             return ld3(f, i);
     };
-    auto row = [&](uint64_t b) { return ld3(hi, (ROW ? b : b + t) >> L); };
-    auto mac = [&](const e3& w, const V& v) {
+    auto mac = [&](gl::e3acc& acc, const e3& w, const V& v) {
         if constexpr (F_BASE)
-            gl::e3_mac_base(s, w, v);
+            gl::e3_mac_base(acc, w, v);
         else
-            gl::e3_mac(s, w, v);
+            gl::e3_mac(acc, w, v);
     };
-    uint64_t b = (uint64_t)blockIdx.x * blockDim.x;
-    for (; b + (U - 1) * stride + t < n; b += U * stride) {
+    // all points' products for the U iterations from index base b
+    auto step = [&](uint64_t b, int u) {
         V v[U];
-        e3 w[U];
 #pragma unroll
-        for (int k = 0; k < U; ++k) {
-            v[k] = ld(b + k * stride + t);
-            w[k] = row(b + k * stride);
-        }
+        for (int k = 0; k < U; ++k)
+            if (k < u) v[k] = ld(b + k * stride + t);
 #pragma unroll
-        for (int k = 0; k < U; ++k) mac(w[k], v[k]);
-    }
-    for (; b + t < n; b += stride) mac(row(b), ld(b + t));
-    return s;
+        for (int k = 0; k < U; ++k)
+            if (k < u) {
+                const uint64_t row = (ROW ? b + k * stride : b + k * stride + t) >> L;
+#pragma unroll
+                for (int p = 0; p < K; ++p) mac(s[p], ld3(hi, (uint64_t)p * H + row), v[k]);
+            }
+    };
+#pragma unroll
+    for (int p = 0; p < K; ++p) s[p] = gl::e3acc_zero();
+    uint64_t b = (uint64_t)blockIdx.x * blockDim.x;
+    for (; b + (U - 1) * stride + t < n; b += U * stride) step(b, U);
+    for (; b + t < n; b += stride) step(b, 1);
 }
 
-// partial sums of f_i * hi[i >> L] * lo[i & mask] (one eq term), as
-//   sum_l lo[l] * (sum_h f[h 2^L + l] hi[h]):
-// the stride (a multiple of 2^L: mle_eval_launch keeps the grid a multiple of
-// 2^L / 256 when 2^L > 256) leaves a lane's column l = i & mask fixed across
-// its iterations, so the lane accumulates f_i hi[h] without reduction and
-// multiplies its total by lo[l] once.
+// For each of K points p: partial sums of f_i * hi_p[i >> L] * lo_p[i & mask]
+// (one eq term per point; point p's tables at hi + 3 H p and lo + 3 2^L p),
+// as sum_l lo_p[l] * (sum_h f[h 2^L + l] hi_p[h]): the stride (a multiple of
+// 2^L: mle_eval_launch keeps the grid a multiple of 2^L / 256 when 2^L > 256)
+// leaves a lane's column l = i & mask fixed across its iterations, so the
+// lane accumulates f_i hi_p[h] without reduction and multiplies its total by
+// lo_p[l] once.  Block totals: partials[3 (K b + p) ..].
+template <int K>
 __global__ __launch_bounds__(256) void mle_eval_kernel(const uint64_t* __restrict__ f, int f_base, uint64_t n,
                                                        uint32_t L, const uint64_t* __restrict__ hi,
                                                        const uint64_t* __restrict__ lo, uint64_t* partials) {
     const bool row = L >= 8;
-    const gl::e3acc s = f_base ? (row ? mle_sweep<true, true>(f, n, L, hi) : mle_sweep<true, false>(f, n, L, hi))
-                               : (row ? mle_sweep<false, true>(f, n, L, hi) : mle_sweep<false, false>(f, n, L, hi));
-    const uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, mask = (1ull << L) - 1;
-    e3 acc[1] = {gl::e3_mul(gl::e3acc_reduce(s), ld3(lo, i0 & mask))};
-    block_sum<1>(acc);
-    store_block_totals<1>(acc, partials);
+    const uint64_t H = n >> L, LO = 1ull << L;
+    gl::e3acc s[K];
+    if (f_base) {
+        if (row)
+            mle_sweep<true, true, K>(f, n, L, hi, H, s);
+        else
+            mle_sweep<true, false, K>(f, n, L, hi, H, s);
+    } else {
+        if (row)
+            mle_sweep<false, true, K>(f, n, L, hi, H, s);
+        else
+            mle_sweep<false, false, K>(f, n, L, hi, H, s);
+    }
+    const uint64_t l = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) & (LO - 1);
+    e3 acc[K];
+#pragma unroll
+    for (int p = 0; p < K; ++p) acc[p] = gl::e3_mul(gl::e3acc_reduce(s[p]), ld3(lo, (uint64_t)p * LO + l));
+    block_sum<K>(acc);
+    store_block_totals<K>(acc, partials);
 }
 
 // ------------------------------------------------------------ fold+compute
@@ -574,13 +593,20 @@ void term_accumulate_launch(uint64_t* d_acc, uint64_t n, uint32_t L, const uint6
 }
 
 unsigned mle_eval_launch(const uint64_t* f, bool f_base, uint64_t n, uint32_t L, const uint64_t* hi,
-                         const uint64_t* lo, uint64_t* partials, hipStream_t st, bool device_sum) {
-    unsigned g = std::min<unsigned>(blocks_of(n), SUMCHECK_MAX_BLOCKS);
+                         const uint64_t* lo, uint32_t points, uint64_t* partials, hipStream_t st) {
+    if (points < 1 || points > MLE_MAX_POINTS) return 0;
+    // 3 * points totals per block: 2048 blocks for one or two points, 1024 for three or four
+    unsigned g = std::min<unsigned>(blocks_of(n), points > 2 ? SUMCHECK_MAX_BLOCKS / 2 : SUMCHECK_MAX_BLOCKS);
     // the kernel needs a stride (256 g) that is a multiple of 2^L
     const unsigned gq = L > 8 ? (1u << (L - 8)) : 1u;
     g = std::max(gq, g / gq * gq);
-    hipLaunchKernelGGL(mle_eval_kernel, dim3(g), dim3(256), 0, st, f, f_base ? 1 : 0, n, L, hi, lo, partials);
-    if (device_sum) hipLaunchKernelGGL(red::sum_blocks_kernel<1>, dim3(1), dim3(256), 0, st, partials, g, partials);
+    const int fb = f_base ? 1 : 0;
+    switch (points) {
+        case 1: hipLaunchKernelGGL(mle_eval_kernel<1>, dim3(g), dim3(256), 0, st, f, fb, n, L, hi, lo, partials); break;
+        case 2: hipLaunchKernelGGL(mle_eval_kernel<2>, dim3(g), dim3(256), 0, st, f, fb, n, L, hi, lo, partials); break;
+        case 3: hipLaunchKernelGGL(mle_eval_kernel<3>, dim3(g), dim3(256), 0, st, f, fb, n, L, hi, lo, partials); break;
+        default: hipLaunchKernelGGL(mle_eval_kernel<4>, dim3(g), dim3(256), 0, st, f, fb, n, L, hi, lo, partials);
+    }
     return g;
 }
 

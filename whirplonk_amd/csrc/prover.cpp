@@ -995,33 +995,47 @@ int whir_dev_geometric_accumulate(whir_ctx* ctx, uint64_t* d_acc, size_t n, cons
     return terms_apply(ctx, T, d_acc, nv, true);
 }
 
+int whir_dev_mle_evaluate_many(whir_ctx* ctx, const uint64_t* d_f, int f_is_base, size_t num_variables,
+                               const uint64_t* points, size_t count, uint64_t* out) {
+    if (!ctx || !d_f || (count && !out) || (num_variables && count && !points)) return bad_arg("null argument");
+    if (num_variables > 30) return bad_arg("num_variables above 30");
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    static_assert(3 * 2 * SUMCHECK_MAX_BLOCKS <= whir_ctx::HPART_U64 &&
+                      3 * MLE_MAX_POINTS * (SUMCHECK_MAX_BLOCKS / 2) <= whir_ctx::HPART_U64,
+                  "the evaluation's block totals fit the mapped partials");
+    const uint32_t nv = (uint32_t)num_variables, L = split_bits(nv);
+    const uint64_t H = 1ull << (nv - L), LO = 1ull << L;
+    int rc;
+    if ((rc = ctx->ensure_hpart())) return rc;
+    // up to MLE_MAX_POINTS points share one term-table launch, one pass over f
+    // and one wait
+    for (size_t p0 = 0; p0 < count; p0 += MLE_MAX_POINTS) {
+        const uint32_t K = (uint32_t)std::min<size_t>(MLE_MAX_POINTS, count - p0);
+        Terms T;
+        for (uint32_t k = 0; k < K; ++k) T.eq(points + 3 * nv * (p0 + k), nv, E3_ONE);
+        if ((rc = ctx->ensure(ctx->tdesc, (T.desc_e.size() + T.coords.size() + 4) * 8))) return rc;
+        if ((rc = ctx->ensure(ctx->ttab, K * (H + LO) * 24 + 256))) return rc;
+        std::vector<uint64_t> all(T.desc_e);
+        all.insert(all.end(), T.coords.begin(), T.coords.end());
+        uint64_t* dd = (uint64_t*)ctx->tdesc.p;
+        if ((rc = ctx->stage_desc(all, dd))) return rc;
+        uint64_t* hi = (uint64_t*)ctx->ttab.p;
+        uint64_t* lo = hi + 3 * K * H;
+        term_tables_launch(dd, K, nullptr, 0, dd + T.desc_e.size(), nv, L, hi, lo, nullptr, nullptr, nullptr,
+                           ctx->stream);
+        const unsigned blocks = mle_eval_launch(d_f, f_is_base != 0, 1ull << nv, L, hi, lo, K, ctx->hpart_d, ctx->stream);
+        HIPCHK(hipGetLastError());
+        e3 v[MLE_MAX_POINTS];
+        if ((rc = ctx->host_sums(blocks, (int)K, v))) return rc;
+        for (uint32_t k = 0; k < K; ++k) std::memcpy(out + 3 * (p0 + k), &v[k], 24);
+    }
+    return 0;
+}
+
 int whir_dev_mle_evaluate(whir_ctx* ctx, const uint64_t* d_f, int f_is_base, size_t num_variables,
                           const uint64_t* point, uint64_t* out) {
     if (!ctx || !d_f || (num_variables && !point) || !out) return bad_arg("null argument");
-    if (num_variables > 30) return bad_arg("num_variables above 30");
-    std::lock_guard<std::recursive_mutex> g(ctx->mu);
-    const uint32_t nv = (uint32_t)num_variables, L = split_bits(nv);
-    Terms T;
-    T.eq(point, nv, E3_ONE);
-    int rc;
-    const uint64_t H = 1ull << (nv - L), LO = 1ull << L;
-    if ((rc = ctx->ensure(ctx->tdesc, (T.desc_e.size() + T.coords.size() + 4) * 8))) return rc;
-    if ((rc = ctx->ensure(ctx->ttab, (H + LO) * 24 + 256))) return rc;
-    if ((rc = ctx->ensure_hpart())) return rc;
-    std::vector<uint64_t> all(T.desc_e);
-    all.insert(all.end(), T.coords.begin(), T.coords.end());
-    uint64_t* dd = (uint64_t*)ctx->tdesc.p;
-    if ((rc = ctx->stage_desc(all, dd))) return rc;
-    uint64_t* hi = (uint64_t*)ctx->ttab.p;
-    uint64_t* lo = hi + 3 * H;
-    term_tables_launch(dd, 1, nullptr, 0, dd + T.desc_e.size(), nv, L, hi, lo, nullptr, nullptr, nullptr, ctx->stream);
-    unsigned blocks = mle_eval_launch(d_f, f_is_base != 0, 1ull << nv, L, hi, lo, ctx->hpart_d, ctx->stream, false);
-    HIPCHK(hipGetLastError());
-    e3 v;
-    // (also the stream sync that keeps `all` alive until its copy has run)
-    if ((rc = ctx->host_sums(blocks, 1, &v))) return rc;
-    std::memcpy(out, &v, 24);
-    return 0;
+    return whir_dev_mle_evaluate_many(ctx, d_f, f_is_base, num_variables, point, 1, out);
 }
 
 }  // extern "C"

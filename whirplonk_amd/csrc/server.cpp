@@ -473,13 +473,16 @@ void whir_server::run(Slot& s, Request& r) {
     const uint32_t nv = log2u(r.size);
     const size_t nvec = params.batch_size;
     int rc;
-    if (!r.have_evals) {
+    if (!r.have_evals && r.num_forms) {
+        // every constraint of a vector in one pass over it
         r.evals.assign(3 * r.num_forms * nvec, 0);
-        for (size_t f = 0; f < r.num_forms; ++f)
-            for (size_t v = 0; v < nvec; ++v)
-                if ((rc = whir_dev_mle_evaluate(s.ctx, d_coeffs + v * r.size, 1, nv, r.points.data() + 3 * nv * f,
-                                                r.evals.data() + 3 * (f * nvec + v))))
-                    return fail_with(rc);
+        std::vector<uint64_t> ev(3 * r.num_forms);
+        for (size_t v = 0; v < nvec; ++v) {
+            if ((rc = whir_dev_mle_evaluate_many(s.ctx, d_coeffs + v * r.size, 1, nv, r.points.data(), r.num_forms,
+                                                 ev.data())))
+                return fail_with(rc);
+            for (size_t f = 0; f < r.num_forms; ++f) std::memcpy(r.evals.data() + 3 * (f * nvec + v), &ev[3 * f], 24);
+        }
     }
     whir_prover* P = nullptr;
     if ((rc = whir_prover_create_in(s.ctx, &params, r.size, protocol_id, session_id, nullptr, 0, blk, r.ws_bytes,
