@@ -671,7 +671,9 @@ int prove_impl(whir_prover* P, size_t num_forms, const uint64_t* points, const u
                              cfg.final_sumcheck.round_pow, sum, fr, [&] { return finish_open(P, fo); })))
         return rc;
     P->eval_point.insert(P->eval_point.end(), fr.begin(), fr.end());
-    return sync(P);
+    // with no final rounds, reading the final opening was the last wait and
+    // nothing has been enqueued since; otherwise the last fold is in flight
+    return cfg.final_sumcheck.num_rounds == 0 ? 0 : sync(P);
 }
 
 whir_host::Params to_params(const whir_params* p) {
