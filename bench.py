@@ -614,10 +614,11 @@ def main():
     ap.add_argument("--requests", type=int, default=2000, help="served requests")
     ap.add_argument("--offered", type=float, default=0.0,
                     help="served offered rate (requests/s, Poisson); 0: 0.8x the measured plain-path rate")
-    # 24 slots within 96 GiB of the 288 GB of HBM (capacity measured 2-4 %
-    # above 16 slots in 64 GiB, with a longer tail under heavy load)
-    ap.add_argument("--slots", type=int, default=24)
-    ap.add_argument("--budget-gb", type=float, default=96.0)
+    # 40 slots within 160 GiB of the 288 GB of HBM: capacity measured 2-3 %
+    # above 24 slots in 96 GiB and level with 48 in 192 GiB, p99 under heavy
+    # load about 52 ms against 41 ms at 24 slots
+    ap.add_argument("--slots", type=int, default=40)
+    ap.add_argument("--budget-gb", type=float, default=160.0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 512 MiB device copy kernel (reference for HBM counter scaling)")
     args = ap.parse_args()
