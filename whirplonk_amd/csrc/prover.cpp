@@ -1026,6 +1026,7 @@ int whir_dev_mle_evaluate_many(whir_ctx* ctx, const uint64_t* d_f, int f_is_base
         term_tables_launch(dd, K, nullptr, 0, dd + T.desc_e.size(), nv, L, hi, lo, nullptr, nullptr, nullptr,
                            ctx->stream);
         const unsigned blocks = mle_eval_launch(d_f, f_is_base != 0, 1ull << nv, L, hi, lo, K, ctx->hpart_d, ctx->stream);
+        if (!blocks) return bad_arg("bad MLE point count");
         HIPCHK(hipGetLastError());
         e3 v[MLE_MAX_POINTS];
         if ((rc = ctx->host_sums(blocks, (int)K, v))) return rc;
