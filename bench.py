@@ -39,7 +39,10 @@ Beside the headline line's workload it measures, in the same JSON line:
     capacity, Poisson arrivals at a fixed offered rate (default 0.8x the
     measured plain-path rate) and at 0.8x the measured serving capacity, each
     with p50/p99 latency, and the plain one-at-a-time GPU path on the same
-    requests.
+    requests;
+  * prove_sizes (not a default leg): the prove at each served size (rate 1/4,
+    or the sizes given by --prove-sizes) on the plain path, one size after
+    another, for per-size timings and kernel profiles.
 
     python bench.py [--gpus 1] [--steps 30] [--warmup 3] [--no-cpu]
                     [--legs commit,prove,prove_sha2,prove_nv24,batched,served]
@@ -611,6 +614,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-host", action="store_true", help="skip the host-resident commit timing")
     ap.add_argument("--legs", default="commit,prove,prove_sha2,prove_nv24,batched,served")
+    ap.add_argument("--prove-sizes", default="16,18,20,22,24", help="num_variables of the prove_sizes leg")
     ap.add_argument("--requests", type=int, default=2000, help="served requests")
     ap.add_argument("--offered", type=float, default=0.0,
                     help="served offered rate (requests/s, Poisson); 0: 0.8x the measured plain-path rate")
@@ -654,6 +658,12 @@ def main():
     out["served"] = (served_leg(args.requests, args.offered, args.slots, args.budget_gb, not args.no_cpu)
                      if "served" in legs else None)
     done("served")
+    if "prove_sizes" in legs:
+        # at most 20 proofs per size, the largest first
+        sizes = sorted({int(v) for v in args.prove_sizes.split(",")}, reverse=True)
+        out["prove_sizes"] = {str(nv): prove_leg(min(args.steps, 20), args.warmup, False, nv=nv, rate=2)
+                              for nv in sizes}
+        done("prove_sizes")
     print(json.dumps(out))
 
 
